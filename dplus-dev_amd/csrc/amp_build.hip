@@ -1,0 +1,329 @@
+// amp_build.hip -- atomic amplitude grid builder for gfx950.
+//
+// Computes, at every node q of the Jacobian spherical grid,
+//   A(q) = scale * sum_ion f~_ion(|q|) * sum_{j in ion} exp(i q.r_j)
+//        + scale * contrast * sum_box prod_a sinc(q_a n_a D/2) n_a D exp(i q.c_box)
+// which is PDBAmplitude::calcAmplitude (Backend/Backend/PDBAmplitude.cpp:243-317)
+// evaluated on the grid (Backend/Backend/Grid.cpp:760-826), the work the
+// reference GPU path splits over CombinedPDBandDummyAtomJacobForSingleIonKernel
+// and CalcPDBVoxelSolventKernelJacobSphr (PDBAmplitudeCUDA.cu:15-186).
+//
+// Design (MI355X):
+//  * Friedel symmetry: for real form factors A(-q) = conj(A(q)); every shell is
+//    antipodally closed, so only its canonical half (3i(3i+1) nodes) is summed
+//    and the conjugate is stored at the antipode.  Half the sincos work.
+//  * Atoms are sorted by ion and cut into tiles of <= kTile atoms of one ion.
+//    The atom loop index is wave-uniform, so atom coordinates arrive through
+//    scalar loads (s_load) and feed v_fma as SGPR operands: no LDS traffic and
+//    no barriers in the inner loop.
+//  * Mixed precision: q is pre-scaled to revolutions, the phase is an FP32 FMA
+//    chain (or an FP64 one, kPhase64), reduced with v_fract and fed to the
+//    hardware v_sin_f32 / v_cos_f32.  Each tile is summed in FP32 and folded
+//    into FP64 accumulators with the ion's form factor at the node's shell:
+//    a two-level compensated sum whose FP32 partials never exceed kTile terms.
+//  * FP64 mode: the same structure with double coordinates and double sincos.
+//  * Each thread owns K canonical nodes, so every scalar atom load feeds K
+//    phase evaluations.
+
+#include "common.hpp"
+
+namespace dplus {
+
+constexpr int kTile = 256;
+
+struct NodeInfo {
+    long long idx, anti;
+    int shell;
+};
+
+// canonical node c -> (storage index, antipode index, shell)
+__device__ __forceinline__ NodeInfo canonical_node(long long c, double step, double &qx, double &qy,
+                                                   double &qz) {
+    NodeInfo n;
+    int i = shell_of_half(c);
+    long long h = c - half_base(i);
+    long long j = h / (6LL * i), k = h % (6LL * i);
+    n.shell = i;
+    n.idx = index_from_indices(i, j, k);
+    n.anti = index_from_indices(i, 3LL * i - j, (k + 3LL * i) % (6LL * i));
+    double q = (double)i * step;
+    double th = kPi * (double)j / (double)(3 * i);
+    double ph = k2Pi * (double)k / (double)(6 * i);
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    qx = q * st * cp;
+    qy = q * st * sp;
+    qz = q * ct;
+    return n;
+}
+
+template <int K, bool kPhase64>
+__global__ __launch_bounds__(256) void build_atoms_mixed(
+    const float4 *__restrict__ atoms, const double4 *__restrict__ atoms64,
+    const int4 *__restrict__ tiles, int ntiles, const double *__restrict__ ff, int nion,
+    double step, long long nhalf, float4 *__restrict__ grid, double scale) {
+    const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * K;
+    float qx[K], qy[K], qz[K];
+    double dqx[K], dqy[K], dqz[K];
+    NodeInfo node[K];
+    double accR[K], accI[K];
+    const double inv2pi = 1.0 / k2Pi;
+#pragma unroll
+    for (int p = 0; p < K; p++) {
+        long long c = c0 + p;
+        if (c >= nhalf) c = nhalf - 1;  // duplicate work, store skipped below
+        double x, y, z;
+        node[p] = canonical_node(c, step, x, y, z);
+        dqx[p] = x * inv2pi;
+        dqy[p] = y * inv2pi;
+        dqz[p] = z * inv2pi;
+        qx[p] = (float)dqx[p];
+        qy[p] = (float)dqy[p];
+        qz[p] = (float)dqz[p];
+        accR[p] = 0.0;
+        accI[p] = 0.0;
+    }
+    for (int t = 0; t < ntiles; t++) {
+        const int4 tl = tiles[t];
+        float sr[K], si[K];
+#pragma unroll
+        for (int p = 0; p < K; p++) sr[p] = si[p] = 0.f;
+        const int a1 = tl.x + tl.y;
+#pragma unroll 2
+        for (int a = tl.x; a < a1; a++) {
+            if constexpr (kPhase64) {
+                const double4 at = atoms64[a];
+#pragma unroll
+                for (int p = 0; p < K; p++) {
+                    double ph = fma(dqx[p], at.x, fma(dqy[p], at.y, dqz[p] * at.z));
+                    float r = (float)(ph - floor(ph));
+                    sr[p] += __builtin_amdgcn_cosf(r);
+                    si[p] += __builtin_amdgcn_sinf(r);
+                }
+            } else {
+                const float4 at = atoms[a];
+#pragma unroll
+                for (int p = 0; p < K; p++) {
+                    float ph = fmaf(qx[p], at.x, fmaf(qy[p], at.y, qz[p] * at.z));
+                    float r = __builtin_amdgcn_fractf(ph);
+                    sr[p] += __builtin_amdgcn_cosf(r);
+                    si[p] += __builtin_amdgcn_sinf(r);
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < K; p++) {
+            double f = ff[(size_t)node[p].shell * nion + tl.z];
+            accR[p] = fma(f, (double)sr[p], accR[p]);
+            accI[p] = fma(f, (double)si[p], accI[p]);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < K; p++) {
+        if (c0 + p < nhalf) {
+            float re = (float)(accR[p] * scale), im = (float)(accI[p] * scale);
+            grid[node[p].idx] = make_float4(re, im, 0.f, 0.f);
+            grid[node[p].anti] = make_float4(re, -im, 0.f, 0.f);
+        }
+    }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void build_atoms_fp64(
+    const double4 *__restrict__ atoms, const int4 *__restrict__ tiles, int ntiles,
+    const double *__restrict__ ff, int nion, double step, long long nhalf,
+    double4 *__restrict__ grid, double scale) {
+    const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * K;
+    double qx[K], qy[K], qz[K], accR[K], accI[K];
+    NodeInfo node[K];
+#pragma unroll
+    for (int p = 0; p < K; p++) {
+        long long c = c0 + p;
+        if (c >= nhalf) c = nhalf - 1;
+        node[p] = canonical_node(c, step, qx[p], qy[p], qz[p]);
+        accR[p] = accI[p] = 0.0;
+    }
+    for (int t = 0; t < ntiles; t++) {
+        const int4 tl = tiles[t];
+        double sr[K], si[K];
+#pragma unroll
+        for (int p = 0; p < K; p++) sr[p] = si[p] = 0.0;
+        const int a1 = tl.x + tl.y;
+        for (int a = tl.x; a < a1; a++) {
+            const double4 at = atoms[a];
+#pragma unroll
+            for (int p = 0; p < K; p++) {
+                double ph = fma(qx[p], at.x, fma(qy[p], at.y, qz[p] * at.z));
+                double s, c;
+                sincos(ph, &s, &c);
+                sr[p] += c;
+                si[p] += s;
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < K; p++) {
+            double f = ff[(size_t)node[p].shell * nion + tl.z];
+            accR[p] = fma(f, sr[p], accR[p]);
+            accI[p] = fma(f, si[p], accI[p]);
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < K; p++) {
+        if (c0 + p < nhalf) {
+            double re = accR[p] * scale, im = accI[p] * scale;
+            grid[node[p].idx] = make_double4(re, im, 0.0, 0.0);
+            grid[node[p].anti] = make_double4(re, -im, 0.0, 0.0);
+        }
+    }
+}
+
+// Solvent boxes (hydration layer or excluded volume), added onto the grid:
+// contrast * prod_a [sinc(q_a n_a D/2) n_a D] * exp(i q.c)
+// (PDBAmplitude.cpp:277-314; PDBAmplitudeCUDA.cu:15-91).  Boxes arrive sorted
+// by their x extent so the x factor is reused (PDBAmplitude.cpp:157-188).
+__device__ __forceinline__ void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ __forceinline__ void sincos_t(double x, double *s, double *c) { sincos(x, s, c); }
+
+template <typename T4, typename T>
+__global__ __launch_bounds__(256) void add_boxes(const int4 *__restrict__ dims,
+                                                 const float4 *__restrict__ com, int nbox,
+                                                 double vox, double contrast, double step,
+                                                 long long nhalf, T4 *__restrict__ grid,
+                                                 double scale) {
+    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nhalf) return;
+    double dx, dy, dz;
+    NodeInfo n = canonical_node(c, step, dx, dy, dz);
+    const T qx = (T)dx, qy = (T)dy, qz = (T)dz;
+    const T hv = (T)(0.5 * vox);
+    T sr = 0, si = 0, vax = 0;
+    int prev = -1;
+    for (int b = 0; b < nbox; b++) {
+        const int4 d = dims[b];
+        const float4 cm = com[b];
+        if (d.x != prev) {
+            prev = d.x;
+            T len = (T)d.x * (T)vox, x = qx * (T)d.x * hv;
+            vax = (qx == (T)0 || d.x == 0) ? len : sin(x) / x * len;
+        }
+        T va = vax;
+        {
+            T len = (T)d.y * (T)vox, x = qy * (T)d.y * hv;
+            va *= (qy == (T)0 || d.y == 0) ? len : sin(x) / x * len;
+        }
+        {
+            T len = (T)d.z * (T)vox, x = qz * (T)d.z * hv;
+            va *= (qz == (T)0 || d.z == 0) ? len : sin(x) / x * len;
+        }
+        T ph = qx * (T)cm.x + qy * (T)cm.y + qz * (T)cm.z;
+        T s, co;
+        sincos_t(ph, &s, &co);
+        sr += va * co;
+        si += va * s;
+    }
+    const T f = (T)(contrast * scale);
+    T4 g = grid[n.idx];
+    g.x += sr * f;
+    g.y += si * f;
+    grid[n.idx] = g;
+    T4 a = grid[n.anti];
+    a.x += sr * f;
+    a.y -= si * f;
+    grid[n.anti] = a;
+}
+
+template <typename T4>
+__global__ void set_origin(T4 *grid, double re) {
+    T4 g = grid[0];
+    g.x += re;
+    grid[0] = g;
+}
+
+// reference layout (complex<double> per node) <-> device node layout
+template <typename T4>
+__global__ void pack_values(const T4 *__restrict__ grid, long long n, double2 *__restrict__ data,
+                            double2 *__restrict__ coeffs) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    T4 g = grid[i];
+    if (data) data[i] = make_double2((double)g.x, (double)g.y);
+    if (coeffs) coeffs[i] = make_double2((double)g.z, (double)g.w);
+}
+
+template <typename T4, typename T>
+__global__ void unpack_values(const double2 *__restrict__ data, long long n, T4 *__restrict__ grid) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double2 d = data[i];
+    T4 g;
+    g.x = (T)d.x;
+    g.y = (T)d.y;
+    g.z = (T)0;
+    g.w = (T)0;
+    grid[i] = g;
+}
+
+// ----------------------------------------------------------------- launchers
+static inline unsigned blocks_for(long long n, int per_block) {
+    return (unsigned)((n + per_block - 1) / per_block);
+}
+
+constexpr int kNodesPerThread = 4;
+
+void launch_build_atoms(bool fp64, bool phase64, const void *atoms, const void *atoms64,
+                        const int4 *tiles, int ntiles, const double *ff, int nion, double step,
+                        long long nhalf, void *grid, double scale, hipStream_t s) {
+    const int K = kNodesPerThread;
+    unsigned nb = blocks_for(nhalf, 256 * K);
+    if (fp64) {
+        build_atoms_fp64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, step,
+                                               nhalf, (double4 *)grid, scale);
+    } else if (phase64) {
+        build_atoms_mixed<K, true><<<nb, 256, 0, s>>>((const float4 *)atoms, (const double4 *)atoms64,
+                                                      tiles, ntiles, ff, nion, step, nhalf,
+                                                      (float4 *)grid, scale);
+    } else {
+        build_atoms_mixed<K, false><<<nb, 256, 0, s>>>((const float4 *)atoms, (const double4 *)atoms64,
+                                                       tiles, ntiles, ff, nion, step, nhalf,
+                                                       (float4 *)grid, scale);
+    }
+}
+
+void launch_add_boxes(bool fp64, const int4 *dims, const float4 *com, int nbox, double vox,
+                      double contrast, double step, long long nhalf, void *grid, double scale,
+                      hipStream_t s) {
+    unsigned nb = blocks_for(nhalf, 256);
+    if (fp64)
+        add_boxes<double4, double><<<nb, 256, 0, s>>>(dims, com, nbox, vox, contrast, step, nhalf,
+                                                       (double4 *)grid, scale);
+    else
+        add_boxes<float4, float><<<nb, 256, 0, s>>>(dims, com, nbox, vox, contrast, step, nhalf,
+                                                     (float4 *)grid, scale);
+}
+
+void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s) {
+    if (fp64)
+        set_origin<double4><<<1, 1, 0, s>>>((double4 *)grid, re);
+    else
+        set_origin<float4><<<1, 1, 0, s>>>((float4 *)grid, re);
+}
+
+void launch_pack(bool fp64, const void *grid, long long n, double2 *data, double2 *coeffs,
+                 hipStream_t s) {
+    unsigned nb = blocks_for(n, 256);
+    if (fp64)
+        pack_values<double4><<<nb, 256, 0, s>>>((const double4 *)grid, n, data, coeffs);
+    else
+        pack_values<float4><<<nb, 256, 0, s>>>((const float4 *)grid, n, data, coeffs);
+}
+
+void launch_unpack(bool fp64, const double2 *data, long long n, void *grid, hipStream_t s) {
+    unsigned nb = blocks_for(n, 256);
+    if (fp64)
+        unpack_values<double4, double><<<nb, 256, 0, s>>>(data, n, (double4 *)grid);
+    else
+        unpack_values<float4, float><<<nb, 256, 0, s>>>(data, n, (float4 *)grid);
+}
+
+}  // namespace dplus

@@ -1,0 +1,790 @@
+// capi.cpp -- host side of libdplus_mi355x: device management, grid objects,
+// the PDB amplitude builder driver, orientation-average drivers and the C ABI
+// declared in include/dplus_mi355x.h.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/dplus_mi355x.h"
+#include "common.hpp"
+#include "device.hpp"
+
+namespace dplus {
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_last_error = "{\"code\":0,\"message\":\"\"}";
+
+std::string json_escape(const std::string &s) {
+    std::string o;
+    for (char c : s) {
+        if (c == '"' || c == '\\') {
+            o += '\\';
+            o += c;
+        } else if (c == '\n') {
+            o += "\\n";
+        } else if ((unsigned char)c < 0x20) {
+            char b[8];
+            snprintf(b, sizeof b, "\\u%04x", c);
+            o += b;
+        } else {
+            o += c;
+        }
+    }
+    return o;
+}
+
+int set_error(int code, const std::string &msg) {
+    g_last_error = "{\"code\":" + std::to_string(code) + ",\"message\":\"" + json_escape(msg) + "\"}";
+    return code;
+}
+
+void check_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw Error(kErrGeneral, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ device
+Device &Device::get() {
+    static Device d;
+    return d;
+}
+
+Device::Device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        ok = false;
+        return;
+    }
+    const char *env = getenv("DPLUS_DEVICE");
+    id = env ? atoi(env) : 0;
+    if (id >= n) id = 0;
+    if (hipSetDevice(id) != hipSuccess) {
+        ok = false;
+        return;
+    }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, id) == hipSuccess) {
+        name = p.gcnArchName;
+        cus = p.multiProcessorCount;
+        mem = p.totalGlobalMem;
+    }
+    ok = hipStreamCreate(&stream) == hipSuccess;
+}
+
+void Device::require() {
+    if (!ok) throw Error(kErrNoGpu, "no usable HIP device (libdplus_mi355x runs on the GPU only)");
+    check_hip(hipSetDevice(id), "hipSetDevice");
+}
+
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+void DevBuf::ensure(size_t bytes) {
+    if (bytes <= n) return;
+    release();
+    if (bytes == 0) return;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        p = nullptr;
+        throw Error(kErrOutOfMemory, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    n = bytes;
+}
+void DevBuf::upload(const void *src, size_t bytes, hipStream_t s) {
+    ensure(bytes);
+    if (bytes) check_hip(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+}
+
+// spline multiplier tables, one set per shell count, kept for the process
+struct SplineTables {
+    DevBuf tab, off;
+};
+static std::mutex g_tab_mu;
+static std::map<int, SplineTables *> g_tabs;
+
+void build_spline_tables(int nshells, std::vector<double> &tab, std::vector<long long> &off);
+void launch_splines(bool fp64, void *grid, int nshells, const double *tab, const long long *off,
+                    hipStream_t s);
+
+static SplineTables *tables_for(int nshells, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = g_tabs.find(nshells);
+    if (it != g_tabs.end()) return it->second;
+    std::vector<double> tab;
+    std::vector<long long> off;
+    build_spline_tables(nshells, tab, off);
+    SplineTables *t = new SplineTables;
+    t->tab.upload(tab.data(), tab.size() * sizeof(double), s);
+    t->off.upload(off.data(), off.size() * sizeof(long long), s);
+    check_hip(hipStreamSynchronize(s), "spline tables");
+    g_tabs[nshells] = t;
+    return t;
+}
+
+// ------------------------------------------------------------------ grid
+Grid::Grid(int gs, double qm, int prec) {
+    if (gs % 2) gs++;
+    if (gs < 2 || gs > 4000 || !(qm > 0)) throw Error(kErrInvalidArgs, "invalid grid size or qmax");
+    grid_size = gs;
+    qmax = qm;
+    precision = prec;
+    N = actual_size(gs);
+    step = qmax / (double)(gs / 2);
+    P = total_points(gs);
+    Device::get().require();
+    dev.ensure((size_t)P * node_bytes());
+    check_hip(hipMemsetAsync(dev.p, 0, (size_t)P * node_bytes(), Device::get().stream), "hipMemset");
+}
+
+void Grid::ensure_host() {
+    if (host_data.size() != (size_t)(2 * P)) host_data.assign(2 * P, 0.0);
+    if (host_interp.size() != (size_t)(2 * P)) host_interp.assign(2 * P, 0.0);
+}
+
+void launch_pack(bool fp64, const void *grid, long long n, double2 *data, double2 *coeffs, hipStream_t s);
+void launch_unpack(bool fp64, const double2 *data, long long n, void *grid, hipStream_t s);
+
+void Grid::splines(hipStream_t s) {
+    SplineTables *t = tables_for(N, s);
+    launch_splines(fp64(), dev.p, N, (const double *)t->tab.p, (const long long *)t->off.p, s);
+    check_hip(hipGetLastError(), "splines launch");
+}
+
+void Grid::download(hipStream_t s) {
+    ensure_host();
+    DevBuf tmp;
+    tmp.ensure((size_t)P * 2 * sizeof(double2));
+    double2 *d = (double2 *)tmp.p;
+    launch_pack(fp64(), dev.p, P, d, d + P, s);
+    check_hip(hipMemcpyAsync(host_data.data(), d, P * sizeof(double2), hipMemcpyDeviceToHost, s), "D2H");
+    check_hip(hipMemcpyAsync(host_interp.data(), d + P, P * sizeof(double2), hipMemcpyDeviceToHost, s), "D2H");
+    check_hip(hipStreamSynchronize(s), "download");
+}
+
+void Grid::upload(hipStream_t s) {
+    ensure_host();
+    DevBuf tmp;
+    tmp.upload(host_data.data(), P * sizeof(double2), s);
+    launch_unpack(fp64(), (const double2 *)tmp.p, P, dev.p, s);
+    splines(s);
+    check_hip(hipStreamSynchronize(s), "upload");
+}
+
+// ------------------------------------------------------------------ PDB builder
+void launch_build_atoms(bool fp64, bool phase64, const void *atoms, const void *atoms64, const int4 *tiles,
+                        int ntiles, const double *ff, int nion, double step, long long nhalf, void *grid,
+                        double scale, hipStream_t s);
+void launch_add_boxes(bool fp64, const int4 *dims, const float4 *com, int nbox, double vox, double contrast,
+                      double step, long long nhalf, void *grid, double scale, hipStream_t s);
+void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s);
+
+// Atomic Form Factor.cpp:11-27 (cube-mean radius), 91-115 (dummy-atom
+// solvent), 160-176 (4 Gaussians + c); evaluated in double per shell radius
+void form_factor_table(const dplus_pdb_desc *d, const std::vector<int> &counts, int N, double step,
+                       std::vector<double> &ff) {
+    const int nion = d->nion;
+    double rm = 0.0;
+    if (d->flags & DPLUS_CALC_DUMMY_SOLVENT) {
+        double num = 0.0, den = 0.0;
+        for (int i = 0; i < nion; i++) {
+            num += d->rad[i] * d->rad[i] * d->rad[i] * counts[i];
+            den += counts[i];
+        }
+        rm = den > 0 ? std::cbrt(num / den) : 0.0;
+    }
+    ff.assign((size_t)(N + 1) * nion, 0.0);
+    for (int s = 0; s <= N; s++) {
+        const double q = s * step;
+        const double sqq = q * q / (100.0 * 157.913670417429737901351855998);
+        for (int i = 0; i < nion; i++) {
+            const double *c = d->coef9 + 9 * i;
+            double f = 0.0;
+            if (d->flags & DPLUS_CALC_ATOMIC_FORMFACTORS) {
+                f = c[8];
+                for (int k = 0; k < 4; k++) f += c[2 * k] * std::exp(-sqq * c[2 * k + 1]);
+            }
+            double sv = 0.0;
+            if (d->flags & DPLUS_CALC_DUMMY_SOLVENT) {
+                const double r = d->rad[i];
+                sv = 4.1887902047863909846 * r * r * r * std::exp(-(rm * rm) * (q * q) * 0.2067834969664667) *
+                     d->sol_ed;
+                sv *= d->c1 * d->c1 * d->c1 *
+                      std::exp(-0.6822178052976590 * q * q * rm * rm * (d->c1 * d->c1 - 1.0));
+            }
+            ff[(size_t)s * nion + i] = f - sv;
+        }
+    }
+}
+
+void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s) {
+    if (d->natoms < 0 || d->nion <= 0 || d->nion > 255) throw Error(kErrInvalidArgs, "bad atom/ion counts");
+    for (int a = 0; a < d->natoms; a++)
+        if (d->ion[a] < 0 || d->ion[a] >= d->nion) throw Error(kErrInvalidArgs, "ion index out of range");
+    // sort atoms by ion (ExtractRelevantCoeffs, PDBReaderLib.cpp:1443-1509) and tile them
+    std::vector<int> order(d->natoms);
+    for (int a = 0; a < d->natoms; a++) order[a] = a;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d->ion[x] < d->ion[y]; });
+    std::vector<float4> at32(std::max(1, d->natoms));
+    std::vector<double4> at64(std::max(1, d->natoms));
+    std::vector<int> counts(d->nion, 0);
+    for (int k = 0; k < d->natoms; k++) {
+        const int a = order[k];
+        const double *p = d->xyz + 3 * a;
+        at32[k] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.f);
+        at64[k] = make_double4(p[0], p[1], p[2], 0.0);
+        counts[d->ion[a]]++;
+    }
+    std::vector<int4> tiles;
+    {
+        int k = 0;
+        while (k < d->natoms) {
+            const int ion = d->ion[order[k]];
+            int e = k;
+            while (e < d->natoms && d->ion[order[e]] == ion && e - k < 256) e++;
+            tiles.push_back(make_int4(k, e - k, ion, 0));
+            k = e;
+        }
+    }
+    std::vector<double> ff;
+    form_factor_table(d, counts, g.N, g.step, ff);
+    // origin: sum f~(0) over atoms + contrast * total box volume
+    double origin = 0.0;
+    if (d->flags & (DPLUS_CALC_ATOMIC_FORMFACTORS | DPLUS_CALC_DUMMY_SOLVENT))
+        for (int i = 0; i < d->nion; i++) origin += counts[i] * ff[i];
+    DevBuf b32, b64, bt, bff, bdims, bcom;
+    b32.upload(at32.data(), at32.size() * sizeof(float4), s);
+    b64.upload(at64.data(), at64.size() * sizeof(double4), s);
+    bt.upload(tiles.data(), std::max<size_t>(1, tiles.size()) * sizeof(int4), s);
+    bff.upload(ff.data(), ff.size() * sizeof(double), s);
+    const long long nhalf = half_base(g.N + 1);
+    const bool atoms_on = (d->flags & (DPLUS_CALC_ATOMIC_FORMFACTORS | DPLUS_CALC_DUMMY_SOLVENT)) && d->natoms > 0;
+    const bool phase64 = getenv("DPLUS_PHASE64") != nullptr;
+    launch_build_atoms(g.fp64(), phase64, b32.p, b64.p, (const int4 *)bt.p, atoms_on ? (int)tiles.size() : 0,
+                       (const double *)bff.p, d->nion, g.step, nhalf, g.dev.p, d->scale, s);
+    check_hip(hipGetLastError(), "build_atoms launch");
+    if (d->nbox > 0 && d->box_contrast != 0.0) {
+        // sort boxes by their extents (SortSolventBoxEntry, PDBAmplitude.cpp:161-188)
+        std::vector<int> bo(d->nbox);
+        for (int b = 0; b < d->nbox; b++) bo[b] = b;
+        auto key = [&](int b) {
+            return std::make_tuple(d->box_dims[3 * b], d->box_dims[3 * b + 1], d->box_dims[3 * b + 2],
+                                   d->box_com[3 * b], d->box_com[3 * b + 1], d->box_com[3 * b + 2]);
+        };
+        std::sort(bo.begin(), bo.end(), [&](int x, int y) { return key(x) < key(y); });
+        std::vector<int4> dims(d->nbox);
+        std::vector<float4> com(d->nbox);
+        double vol = 0.0;
+        for (int k = 0; k < d->nbox; k++) {
+            const int b = bo[k];
+            dims[k] = make_int4(d->box_dims[3 * b], d->box_dims[3 * b + 1], d->box_dims[3 * b + 2], 0);
+            com[k] = make_float4((float)d->box_com[3 * b], (float)d->box_com[3 * b + 1], (float)d->box_com[3 * b + 2], 0.f);
+            vol += (double)dims[k].x * dims[k].y * dims[k].z * d->vox_step * d->vox_step * d->vox_step;
+        }
+        origin += d->box_contrast * vol;
+        bdims.upload(dims.data(), dims.size() * sizeof(int4), s);
+        bcom.upload(com.data(), com.size() * sizeof(float4), s);
+        launch_add_boxes(g.fp64(), (const int4 *)bdims.p, (const float4 *)bcom.p, d->nbox, d->vox_step,
+                         d->box_contrast, g.step, nhalf, g.dev.p, d->scale, s);
+        check_hip(hipGetLastError(), "add_boxes launch");
+    }
+    launch_set_origin(g.fp64(), g.dev.p, origin * d->scale, s);
+    g.splines(s);
+    if (host_copy)
+        g.download(s);
+    else
+        check_hip(hipStreamSynchronize(s), "pdb build");
+}
+
+// ------------------------------------------------------------------ orientation averages
+void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
+               const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
+               double step, uint64_t seed, long long dir0, int nblk, double *partial, double *out,
+               hipStream_t s);
+void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
+                       const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
+                       double step, uint64_t seed, int iter, int ppb, double *bounds, double *I, double *hist,
+                       int nq, hipStream_t s);
+void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
+                        const double4 *trans, int n, const double *qxyz, double *out, hipStream_t s);
+
+// shell groups (OAJacob.cu:234-249): consecutive q with L*step < q <= (L+1)*step
+std::vector<GroupDev> make_groups(const double *q, int nq, double step) {
+    std::vector<GroupDev> gs;
+    int layer = 0, q0 = 0;
+    while (q0 < nq) {
+        while (q[q0] > (double)(layer + 1) * step) layer++;
+        int e = q0 + 1;
+        while (e < nq && !(q[e] > (double)(layer + 1) * step)) e++;
+        gs.push_back({layer, q0, e - q0, 0});
+        q0 = e;
+        layer++;
+    }
+    return gs;
+}
+
+static bool converged3(const std::vector<double> &hist, int k, int q0, int m, int nq, double eps) {
+    if (!(k > 2 && eps > 0.0)) return false;
+    for (int j = 0; j < m; j++) {
+        const double c = hist[(size_t)k * nq + q0 + j];
+        for (int b = 1; b <= 3; b++)
+            if (std::fabs(1.0 - c / hist[(size_t)(k - b) * nq + q0 + j]) > eps) return false;
+    }
+    return true;
+}
+
+DockSet::DockSet(int nleaf, const dplus_leaf *leaves, hipStream_t s) {
+    if (nleaf <= 0) throw Error(kErrInvalidArgs, "no leaves");
+    std::vector<const void *> gp;
+    std::vector<LeafGeo> geo;
+    std::vector<RotDev> rots;
+    std::vector<double4> tr;
+    for (int l = 0; l < nleaf; l++) {
+        const dplus_leaf &L = leaves[l];
+        if (!L.grid) throw Error(kErrInvalidArgs, "leaf without grid");
+        Grid *g = (Grid *)L.grid;
+        if (l == 0) {
+            grid_size = g->grid_size;
+            qmax = g->qmax;
+            step = g->step;
+            fp64 = g->fp64();
+        } else if (g->grid_size != grid_size || g->qmax != qmax || g->fp64() != fp64) {
+            throw Error(kErrInvalidArgs, "all leaves of one orientation average must share grid size, qmax and precision");
+        }
+        gp.push_back(g->dev.p);
+        geo.push_back({g->N, g->step});
+        int t0 = 0;
+        for (int r = 0; r < L.nrot; r++) {
+            RotDev rd;
+            rd.ang = make_double4(L.rot[4 * r], L.rot[4 * r + 1], L.rot[4 * r + 2], L.rot[4 * r + 3]);
+            rd.leaf = l;
+            rd.t0 = (int)tr.size();
+            rd.nt = L.ntrans[r];
+            rd.pad = 0;
+            if (rd.nt < 0) throw Error(kErrInvalidArgs, "negative translation count");
+            for (int t = 0; t < rd.nt; t++)
+                tr.push_back(make_double4(L.trans[3 * (t0 + t)], L.trans[3 * (t0 + t) + 1], L.trans[3 * (t0 + t) + 2], 0.0));
+            t0 += rd.nt;
+            rots.push_back(rd);
+        }
+    }
+    if (tr.empty()) tr.push_back(make_double4(0, 0, 0, 0));
+    nrot = (int)rots.size();
+    grids.upload(gp.data(), gp.size() * sizeof(void *), s);
+    geos.upload(geo.data(), geo.size() * sizeof(LeafGeo), s);
+    rotb.upload(rots.data(), std::max<size_t>(1, rots.size()) * sizeof(RotDev), s);
+    trb.upload(tr.data(), tr.size() * sizeof(double4), s);
+}
+
+void orientation_average(const DockSet &D, int nq, const double *q, const dplus_oa_params *p, double *out,
+                         dplus_oa_stats *st, hipStream_t s) {
+    for (int i = 1; i < nq; i++)
+        if (q[i] < q[i - 1]) throw Error(kErrInvalidArgs, "q must be ascending");
+    if (nq <= 0) return;
+    if (q[0] < 0) throw Error(kErrInvalidArgs, "q must be non-negative");
+    if (q[nq - 1] > D.qmax * (1.0 + 1e-12) + 0.0) {
+        // the reference integrates up to the grid's qmax; beyond it the extra shells are extrapolation
+        if (q[nq - 1] > (D.grid_size / 2 + 1) * D.step) throw Error(kErrInvalidArgs, "q beyond the grid");
+    }
+    const std::vector<GroupDev> groups = make_groups(q, nq, D.step);
+    const int ng = (int)groups.size();
+    int mstride = 1;
+    for (auto &g : groups) mstride = std::max(mstride, g.m);
+    DevBuf bq, bg, bact;
+    bq.upload(q, nq * sizeof(double), s);
+    bg.upload(groups.data(), ng * sizeof(GroupDev), s);
+    std::vector<int> act(ng);
+    for (int i = 0; i < ng; i++) act[i] = i;
+    bact.upload(act.data(), ng * sizeof(int), s);
+    if (p->method == DPLUS_OA_MC) {
+        const int chunk = p->chunk > 0 ? p->chunk : 8192;
+        if (chunk % 256) throw Error(kErrInvalidArgs, "MC chunk must be a multiple of 256");
+        long long nchunks = (std::max<long long>(1, p->max_dirs) + chunk - 1) / chunk;
+        std::vector<double> hist((size_t)nchunks * nq, 0.0), sum(nq, 0.0);
+        DevBuf bpart, bout;
+        long long used = 0;
+        if (p->eps <= 0.0) {
+            // one launch covers every direction of every group
+            const long long total = nchunks * chunk;
+            const long long nblk_ll = total / 256;
+            if (nblk_ll > 2147483647LL) throw Error(kErrInvalidArgs, "too many directions");
+            const int nblk = (int)nblk_ll;
+            bpart.ensure((size_t)ng * nblk * mstride * sizeof(double));
+            bout.ensure((size_t)ng * mstride * sizeof(double));
+            launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                      (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
+                      (const double *)bq.p, D.step, p->seed, 0, nblk, (double *)bpart.p, (double *)bout.p, s);
+            check_hip(hipGetLastError(), "mc launch");
+            std::vector<double> r((size_t)ng * mstride);
+            check_hip(hipMemcpyAsync(r.data(), bout.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check_hip(hipStreamSynchronize(s), "mc");
+            for (int gi = 0; gi < ng; gi++)
+                for (int j = 0; j < groups[gi].m; j++)
+                    out[groups[gi].q0 + j] = r[(size_t)gi * mstride + j] / (double)total;
+            used = total;
+        } else {
+            // chunked with the reference's three-back convergence test (GPUHybridCalc.cu:705-720)
+            const int nblk = chunk / 256;
+            bpart.ensure((size_t)ng * nblk * mstride * sizeof(double));
+            bout.ensure((size_t)ng * mstride * sizeof(double));
+            std::vector<int> active = act;
+            std::vector<int> done_k(ng, -1);
+            std::vector<double> r((size_t)ng * mstride);
+            for (long long k = 0; k < nchunks && !active.empty(); k++) {
+                bact.upload(active.data(), active.size() * sizeof(int), s);
+                launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                          (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, (int)active.size(),
+                          mstride, (const double *)bq.p, D.step, p->seed, k * chunk, nblk, (double *)bpart.p,
+                          (double *)bout.p, s);
+                check_hip(hipGetLastError(), "mc launch");
+                check_hip(hipMemcpyAsync(r.data(), bout.p, active.size() * mstride * sizeof(double),
+                                         hipMemcpyDeviceToHost, s), "D2H");
+                check_hip(hipStreamSynchronize(s), "mc chunk");
+                std::vector<int> next;
+                for (size_t ai = 0; ai < active.size(); ai++) {
+                    const GroupDev &G = groups[active[ai]];
+                    for (int j = 0; j < G.m; j++) {
+                        sum[G.q0 + j] += r[ai * mstride + j];
+                        hist[(size_t)k * nq + G.q0 + j] = sum[G.q0 + j] / ((double)(k + 1) * chunk);
+                    }
+                    if (converged3(hist, (int)k, G.q0, G.m, nq, p->eps) || k == nchunks - 1)
+                        done_k[active[ai]] = (int)k;
+                    else
+                        next.push_back(active[ai]);
+                }
+                active.swap(next);
+            }
+            for (int gi = 0; gi < ng; gi++) {
+                const int k = done_k[gi];
+                used = std::max<long long>(used, (long long)(k + 1) * chunk);
+                for (int j = 0; j < groups[gi].m; j++) out[groups[gi].q0 + j] = hist[(size_t)k * nq + groups[gi].q0 + j];
+            }
+        }
+        if (st) st->dirs_used = used;
+    } else if (p->method == DPLUS_OA_VEGAS) {
+        const int iters = std::max(1, p->vegas_iterations);
+        const int ppb = std::max(1, p->vegas_points_per_bin);
+        const int npts = 1024 * ppb;
+        std::vector<double> bounds((size_t)ng * 66);
+        for (int g = 0; g < ng; g++)
+            for (int i = 0; i <= 32; i++) {
+                bounds[(size_t)g * 66 + i] = (double)i / 32;
+                bounds[(size_t)g * 66 + 33 + i] = (double)i / 32;
+            }
+        DevBuf bb, bI, bh;
+        bb.upload(bounds.data(), bounds.size() * sizeof(double), s);
+        bI.ensure((size_t)ng * npts * mstride * sizeof(double));
+        bh.ensure((size_t)iters * nq * sizeof(double));
+        check_hip(hipMemsetAsync(bh.p, 0, (size_t)iters * nq * sizeof(double), s), "memset");
+        std::vector<double> hist((size_t)iters * nq, 0.0);
+        std::vector<int> active = act, done_k(ng, iters - 1);
+        int used = 0;
+        for (int k = 0; k < iters && !active.empty(); k++) {
+            if (k > 0) bact.upload(active.data(), active.size() * sizeof(int), s);
+            launch_vegas_iter(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                              (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
+                              (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k, ppb,
+                              (double *)bb.p, (double *)bI.p, (double *)bh.p, nq, s);
+            check_hip(hipGetLastError(), "vegas launch");
+            used = k + 1;
+            if (p->eps > 0.0) {
+                check_hip(hipMemcpyAsync(hist.data() + (size_t)k * nq, (double *)bh.p + (size_t)k * nq,
+                                         nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+                check_hip(hipStreamSynchronize(s), "vegas iter");
+                std::vector<int> next;
+                for (int g : active) {
+                    if (converged3(hist, k, groups[g].q0, groups[g].m, nq, p->eps))
+                        done_k[g] = k;
+                    else
+                        next.push_back(g);
+                }
+                active.swap(next);
+            }
+        }
+        check_hip(hipMemcpyAsync(hist.data(), bh.p, hist.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        check_hip(hipStreamSynchronize(s), "vegas");
+        for (int g = 0; g < ng; g++)
+            for (int j = 0; j < groups[g].m; j++) out[groups[g].q0 + j] = hist[(size_t)done_k[g] * nq + groups[g].q0 + j];
+        if (st) st->iterations_used = used;
+    } else if (p->method == DPLUS_OA_GK) {
+        gk_average(D, nq, q, p->eps, p->gk_max_depth > 0 ? p->gk_max_depth : (1LL << 40), out, st, s);
+    } else {
+        throw Error(kErrInvalidArgs, "unknown orientation method");
+    }
+}
+
+}  // namespace dplus
+
+using namespace dplus;
+
+#define API_GUARD(...)                                      \
+    try {                                                   \
+        __VA_ARGS__;                                        \
+        return kOk;                                         \
+    } catch (const Error &e) {                              \
+        return set_error(e.code, e.what());                 \
+    } catch (const std::exception &e) {                     \
+        return set_error(kErrGeneral, e.what());            \
+    } catch (...) {                                         \
+        return set_error(kErrGeneral, "unknown exception"); \
+    }
+
+extern "C" {
+
+const char *dplus_last_error(void) { return g_last_error.c_str(); }
+
+int dplus_device_info(char *buf, int len) {
+    API_GUARD({
+        Device &d = Device::get();
+        std::string s = "{\"ok\":" + std::string(d.ok ? "true" : "false") + ",\"device\":" + std::to_string(d.id) +
+                        ",\"arch\":\"" + json_escape(d.name) + "\",\"cus\":" + std::to_string(d.cus) +
+                        ",\"memory\":" + std::to_string(d.mem) + "}";
+        if (buf && len > 0) {
+            strncpy(buf, s.c_str(), len - 1);
+            buf[len - 1] = 0;
+        }
+    })
+}
+
+int dplus_grid_create(int grid_size, double qmax, int precision, dplus_grid **out) {
+    API_GUARD({
+        if (!out) throw Error(kErrInvalidArgs, "null output");
+        if (precision != DPLUS_PRECISION_FP64 && precision != DPLUS_PRECISION_MIXED)
+            throw Error(kErrInvalidArgs, "unknown precision");
+        *out = (dplus_grid *)new Grid(grid_size, qmax, precision);
+    })
+}
+
+void dplus_grid_destroy(dplus_grid *g) { delete (Grid *)g; }
+
+int dplus_grid_info(const dplus_grid *gh, int *grid_size, double *qmax, double *step, int *actual_size,
+                    long long *points, int *precision) {
+    API_GUARD({
+        const Grid *g = (const Grid *)gh;
+        if (!g) throw Error(kErrInvalidArgs, "null grid");
+        if (grid_size) *grid_size = g->grid_size;
+        if (qmax) *qmax = g->qmax;
+        if (step) *step = g->step;
+        if (actual_size) *actual_size = g->N;
+        if (points) *points = g->P;
+        if (precision) *precision = g->precision;
+    })
+}
+
+double *dplus_grid_data(dplus_grid *gh) {
+    Grid *g = (Grid *)gh;
+    if (!g) return nullptr;
+    g->ensure_host();
+    return g->host_data.data();
+}
+
+double *dplus_grid_interp(dplus_grid *gh) {
+    Grid *g = (Grid *)gh;
+    if (!g) return nullptr;
+    g->ensure_host();
+    return g->host_interp.data();
+}
+
+long long dplus_grid_index_from_indices(int qi, long long ti, long long pi) { return index_from_indices(qi, ti, pi); }
+
+void dplus_grid_indices_from_index(long long index, int *qi, long long *ti, long long *pi) {
+    if (index <= 0) {
+        *qi = 0;
+        *ti = 0;
+        *pi = 0;
+        return;
+    }
+    long long q = (long long)std::cbrt((double)index / 6.0);
+    if (q < 1) q = 1;
+    while (q > 1 && shell_base(q) > index) q--;
+    while (shell_base(q + 1) <= index) q++;
+    long long rem = index - shell_base(q);
+    *qi = (int)q;
+    *ti = rem / (6 * q);
+    *pi = rem % (6 * q);
+}
+
+int dplus_grid_upload(dplus_grid *gh) {
+    API_GUARD({
+        Grid *g = (Grid *)gh;
+        if (!g) throw Error(kErrInvalidArgs, "null grid");
+        Device::get().require();
+        g->upload(Device::get().stream);
+    })
+}
+
+int dplus_grid_calculate_splines(dplus_grid *gh) {
+    API_GUARD({
+        Grid *g = (Grid *)gh;
+        if (!g) throw Error(kErrInvalidArgs, "null grid");
+        Device::get().require();
+        g->splines(Device::get().stream);
+        check_hip(hipStreamSynchronize(Device::get().stream), "splines");
+    })
+}
+
+int dplus_grid_download(dplus_grid *gh) {
+    API_GUARD({
+        Grid *g = (Grid *)gh;
+        if (!g) throw Error(kErrInvalidArgs, "null grid");
+        Device::get().require();
+        g->download(Device::get().stream);
+    })
+}
+
+}  // extern "C"
+namespace dplus {
+void launch_sphr_batch(bool fp64, const void *g, int N, double step, int n, const double *q, const double *th,
+                       const double *ph, double2 *out, hipStream_t s);
+void launch_seeded_samples(bool fp64, const void *g, int N, double step, double q, long long n,
+                           const double *phi, const double *v2, double *out, hipStream_t s);
+}  // namespace dplus
+extern "C" {
+
+int dplus_grid_get_sphr(dplus_grid *gh, int n, const double *q, const double *th, const double *ph, double *out) {
+    API_GUARD({
+        Grid *g = (Grid *)gh;
+        if (!g || n < 0) throw Error(kErrInvalidArgs, "bad arguments");
+        if (n == 0) return kOk;
+        for (int i = 0; i < n; i++)
+            if (!(q[i] >= 0 && q[i] <= (g->N - 2) * g->step && th[i] >= 0 && th[i] <= kPi && ph[i] >= 0 &&
+                  ph[i] <= k2Pi))
+                throw Error(kErrInvalidArgs, "lookup point outside the grid (q in [0, qmax], theta in [0,pi], phi in [0,2pi])");
+        hipStream_t s = Device::get().stream;
+        DevBuf bq, bt, bp, bo;
+        bq.upload(q, n * sizeof(double), s);
+        bt.upload(th, n * sizeof(double), s);
+        bp.upload(ph, n * sizeof(double), s);
+        bo.ensure(n * sizeof(double2));
+        launch_sphr_batch(g->fp64(), g->dev.p, g->N, g->step, n, (const double *)bq.p, (const double *)bt.p,
+                          (const double *)bp.p, (double2 *)bo.p, s);
+        check_hip(hipGetLastError(), "sphr launch");
+        check_hip(hipMemcpyAsync(out, bo.p, n * sizeof(double2), hipMemcpyDeviceToHost, s), "D2H");
+        check_hip(hipStreamSynchronize(s), "sphr");
+    })
+}
+
+int dplus_grid_calculate_intensity(dplus_grid *gh, double q, double theta, double eps, unsigned seed,
+                                   unsigned long long iterations, double phi_min, double phi_max, double *result) {
+    API_GUARD({
+        Grid *g = (Grid *)gh;
+        if (!g || !result || iterations == 0) throw Error(kErrInvalidArgs, "bad arguments");
+        if ((phi_min < 0) || (phi_max > k2Pi) || (phi_min > phi_max))
+            throw Error(kErrGeneral, "phi_min or phi_max are invalid");
+        if (!(q >= 0 && q <= (g->N - 2) * g->step)) throw Error(kErrInvalidArgs, "q outside the grid");
+        hipStream_t s = Device::get().stream;
+        if (q == 0.0) {
+            double o[2];
+            const double z = 0.0;
+            DevBuf bz, bo;
+            bz.upload(&z, sizeof z, s);
+            bo.ensure(sizeof(double2));
+            launch_sphr_batch(g->fp64(), g->dev.p, g->N, g->step, 1, (const double *)bz.p, (const double *)bz.p,
+                              (const double *)bz.p, (double2 *)bo.p, s);
+            check_hip(hipMemcpyAsync(o, bo.p, sizeof o, hipMemcpyDeviceToHost, s), "D2H");
+            check_hip(hipStreamSynchronize(s), "intensity");
+            *result = o[0] * o[0] + o[1] * o[1];
+            return kOk;
+        }
+        // Grid.cpp:415-425: phi ~ U(phi_min, phi_max), v2 ~ U(0, 2) from std::mt19937(seed)
+        if (theta < 0) {
+            phi_min = 0.0;
+            phi_max = k2Pi;
+        }
+        const long long n = (long long)iterations;
+        std::vector<double> phi(n), v2(n);
+        std::mt19937 rng;
+        rng.seed(seed);
+        std::uniform_real_distribution<double> ranU2(0.0, 2.0);
+        std::uniform_real_distribution<double> ranPHI(phi_min, phi_max);
+        for (long long i = 0; i < n; i++) {
+            phi[i] = ranPHI(rng);
+            v2[i] = ranU2(rng);
+            if (theta >= 0) v2[i] = std::cos(theta) + 1.0;
+        }
+        DevBuf bphi, bv2, bo;
+        bphi.upload(phi.data(), n * sizeof(double), s);
+        bv2.upload(v2.data(), n * sizeof(double), s);
+        bo.ensure(n * sizeof(double));
+        launch_seeded_samples(g->fp64(), g->dev.p, g->N, g->step, q, n, (const double *)bphi.p,
+                              (const double *)bv2.p, (double *)bo.p, s);
+        check_hip(hipGetLastError(), "seeded launch");
+        std::vector<double> I(n);
+        check_hip(hipMemcpyAsync(I.data(), bo.p, n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        check_hip(hipStreamSynchronize(s), "intensity");
+        // running mean and the convergence test of Grid.cpp:456-467
+        std::vector<double> res(n);
+        const unsigned long long minIter = iterations / 20;
+        double acc = 0.0;
+        for (long long i = 0; i < n; i++) {
+            acc += I[i];
+            res[i] = acc / (double)(i + 1);
+            if ((unsigned long long)i >= minIter && eps > 0.0) {
+                if (std::fabs(1.0 - res[i] / res[i >> 1]) < eps && std::fabs(1.0 - res[i] / res[(i << 1) / 3]) < eps &&
+                    std::fabs(1.0 - res[i] / res[(3 * i) >> 2]) < eps) {
+                    *result = res[i];
+                    return kOk;
+                }
+            }
+        }
+        *result = res[n - 1];
+    })
+}
+
+int dplus_pdb_build_grid(dplus_grid *gh, const dplus_pdb_desc *d, int host_copy) {
+    API_GUARD({
+        Grid *g = (Grid *)gh;
+        if (!g || !d) throw Error(kErrInvalidArgs, "null argument");
+        Device::get().require();
+        pdb_build(*g, d, host_copy != 0, Device::get().stream);
+    })
+}
+
+int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const double *q, const dplus_oa_params *p,
+                              double *intensity, dplus_oa_stats *st) {
+    API_GUARD({
+        if (!leaves || !q || !p || !intensity || nq < 0) throw Error(kErrInvalidArgs, "null argument");
+        Device::get().require();
+        auto t0 = std::chrono::steady_clock::now();
+        hipStream_t s = Device::get().stream;
+        DockSet D(nleaf, leaves, s);
+        if (st) memset(st, 0, sizeof *st);
+        orientation_average(D, nq, q, p, intensity, st, s);
+        if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    })
+}
+
+int dplus_dock_grid(dplus_grid *outh, int nleaf, const dplus_leaf *leaves, double scale) {
+    API_GUARD({
+        Grid *out = (Grid *)outh;
+        if (!out || !leaves) throw Error(kErrInvalidArgs, "null argument");
+        Device::get().require();
+        hipStream_t s = Device::get().stream;
+        DockSet D(nleaf, leaves, s);
+        if (D.fp64 != out->fp64()) throw Error(kErrInvalidArgs, "output precision differs from the leaves'");
+        dock_grid(D, *out, scale, s);
+    })
+}
+
+int dplus_synth_walk(unsigned long long seed, int n, double step, double min_sep, double ax, double ay, double az,
+                     double *xyz) {
+    API_GUARD({
+        if (n < 0 || !xyz || step <= 0 || min_sep < 0 || ax <= 0 || ay <= 0 || az <= 0)
+            throw Error(kErrInvalidArgs, "bad walk parameters");
+        synth_walk(seed, n, step, min_sep, ax, ay, az, xyz);
+    })
+}
+
+}  // extern "C"

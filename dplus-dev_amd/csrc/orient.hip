@@ -1,0 +1,662 @@
+// orient.hip -- grid lookups, OA-time docking and orientation averages.
+//
+// Lookup: GetAmpAtPointInPlaneJacob (Backend/GPU/CommonJacobGPUMethods.cu:69-154)
+//   = cubic Hermite in phi on 4 theta rows + 4-point spline in theta, and the
+//   4-plane q spline (OAJacob.cu:57-113, HybridOA.cu:78-133; GetSphr
+//   Backend/Backend/Grid.cpp:487-548 for single points).
+// Docking: AddAmplitudeAtPoint (HybridOA.cu:17-159): for each rotation
+//   (R = Rx(a)Ry(b)Rz(c), Amplitude.h:557-573) A_leaf(R^T q) * sum_t e^{i q.t} * scale.
+// MC: directions (theta = acos(2u-1), phi = 2 pi v) from a counter-based Philox
+//   stream keyed by (seed; direction, shell group), so any launch shape and any
+//   batch composition reproduce the same numbers (the reference draws them from
+//   a clock-seeded MTGP32, GPUHybridCalc.cu:196-198).
+// VEGAS: 32x32 (cos theta, phi) bins, bin-volume weighted estimate, per-bin
+//   variance marginals and the damped re-binning of HybridOA.cu:181-449.
+//
+// Node layout: one T4 per grid node = (Re A, Im A, Re dA/dphi-spline, Im ...),
+// so the two phi neighbours of a lookup are one contiguous 2*sizeof(T4) read.
+
+#include "common.hpp"
+#include "device.hpp"
+
+namespace dplus {
+
+template <typename T>
+struct cplx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> hermite(cplx<T> p1, cplx<T> p2, cplx<T> d1, cplx<T> d2, T t) {
+    T t2 = t * t, t3 = t2 * t;
+    cplx<T> r;
+    r.x = p1.x + d1.x * t + ((T)3 * (p2.x - p1.x) - (T)2 * d1.x - d2.x) * t2 +
+          ((T)2 * (p1.x - p2.x) + d1.x + d2.x) * t3;
+    r.y = p1.y + d1.y * t + ((T)3 * (p2.y - p1.y) - (T)2 * d1.y - d2.y) * t2 +
+          ((T)2 * (p1.y - p2.y) + d1.y + d2.y) * t3;
+    return r;
+}
+
+// PeriodicSplineSolver.h:227-234 / CommonJacobGPUMethods.cu:34-63
+template <typename T>
+__device__ __forceinline__ void four_point(const cplx<T> y[4], cplx<T> &d2, cplx<T> &d3) {
+    const T a = (T)(7. / 15.), b = (T)(14. / 15.), c = (T)(4. / 15.), e = (T)(2. / 15.);
+    d2.x = -a * (y[1].x - y[0].x) + b * (y[2].x - y[0].x) - c * (y[3].x - y[1].x) + e * (y[3].x - y[2].x);
+    d2.y = -a * (y[1].y - y[0].y) + b * (y[2].y - y[0].y) - c * (y[3].y - y[1].y) + e * (y[3].y - y[2].y);
+    d3.x = e * (y[1].x - y[0].x) - c * (y[2].x - y[0].x) + b * (y[3].x - y[1].x) - a * (y[3].x - y[2].x);
+    d3.y = e * (y[1].y - y[0].y) - c * (y[2].y - y[0].y) + b * (y[3].y - y[1].y) - a * (y[3].y - y[2].y);
+}
+
+// one theta-phi plane (shell >= 1); kZeroTiny reproduces the CPU variant's
+// snapping of interpolation fractions below 1e-10 (Grid.cpp:1242-1245)
+template <typename T, typename T4, bool kZeroTiny>
+__device__ cplx<T> plane_lookup(const T4 *__restrict__ plane, int shell, double theta, double phi) {
+    const int phiPoints = 6 * shell, thePoints = 3 * shell;
+    const double edge = k2Pi / (double)phiPoints;
+    const int tI = (int)((theta / kPi) * (double)thePoints);
+    int pI = (int)((phi / k2Pi) * (double)phiPoints);
+    double tphd = phi / edge - (double)pI;
+    double tthd = theta / edge - (double)tI;
+    if (kZeroTiny) {
+        if (fabs(tthd) < 1.0e-10) tthd = 0.0;
+        if (fabs(tphd) < 1.0e-10) tphd = 0.0;
+    }
+    const T tph = (T)tphd, tth = (T)tthd;
+    pI = (pI == phiPoints) ? 0 : pI;
+    const int pI1 = (pI + 1) % phiPoints;
+    const int half = phiPoints / 2;
+    cplx<T> nn[4];
+#pragma unroll
+    for (int rl = -1; rl < 3; rl++) {
+        int theInd = (thePoints + 1 + tI + rl) % (thePoints + 1);
+        int pos1, pos2;
+        if (theInd != tI + rl) {
+            if (tI + rl > thePoints) theInd = (tI + rl) - (thePoints + 1);
+            pos1 = phiPoints * theInd + pI + ((2 * pI >= phiPoints) ? -half : half);
+            pos2 = phiPoints * theInd + pI1 + ((2 * pI1 >= phiPoints) ? -half : half);
+        } else {
+            pos1 = phiPoints * theInd + pI;
+            pos2 = phiPoints * theInd + pI1;
+        }
+        const T4 a = plane[pos1], b = plane[pos2];
+        nn[rl + 1] = hermite<T>({a.x, a.y}, {b.x, b.y}, {a.z, a.w}, {b.z, b.w}, tph);
+    }
+    cplx<T> d1, d2;
+    four_point<T>(nn, d1, d2);
+    return hermite<T>(nn[1], nn[2], d1, d2, tth);
+}
+
+__device__ __forceinline__ void antipode(double th, double ph, double &nth, double &nph) {
+    double a = kPi - th, b = ph + kPi;
+    if (b < 0) b = k2Pi + (b - (int)(b / k2Pi) * k2Pi);
+    if (b >= k2Pi) b = b - (int)(b / k2Pi) * k2Pi;
+    nth = fmin(fmax(a, 0.0), kPi);
+    nph = b;
+}
+
+// planes layer-1 .. layer+2 and the q-spline derivatives (OAJacob.cu:57-98)
+template <typename T, typename T4>
+__device__ __forceinline__ void group_planes(const T4 *__restrict__ g, int layer, double th, double ph,
+                                             cplx<T> amps[4], cplx<T> &d1, cplx<T> &d2) {
+#pragma unroll
+    for (int i = -1; i <= 2; i++) {
+        const int lqi = i + layer;
+        if (lqi == -1) {
+            double nth, nph;
+            antipode(th, ph, nth, nph);
+            amps[i + 1] = plane_lookup<T, T4, false>(g + 1, 1, nth, nph);
+        } else if (lqi == 0) {
+            const T4 o = g[0];
+            amps[i + 1] = {o.x, o.y};
+        } else {
+            amps[i + 1] = plane_lookup<T, T4, false>(g + shell_base(lqi), lqi, th, ph);
+        }
+    }
+    four_point<T>(amps, d1, d2);
+}
+
+// R^T q with R = Rx(a) Ry(b) Rz(c): HybridOA.cu:60-63
+__device__ __forceinline__ void rotate_t(const double4 r, const double q[3], double o[3]) {
+    double s1, c1, s2, c2, s3, c3;
+    sincos(r.x, &s1, &c1);
+    sincos(r.y, &s2, &c2);
+    sincos(r.z, &s3, &c3);
+    o[0] = (c2 * c3) * q[0] + (c1 * s3 + c3 * s1 * s2) * q[1] + (s1 * s3 - c1 * c3 * s2) * q[2];
+    o[1] = -(c2 * s3) * q[0] + (c1 * c3 - s1 * s2 * s3) * q[1] + (c3 * s1 + c1 * s2 * s3) * q[2];
+    o[2] = (s2)*q[0] - (c2 * s1) * q[1] + (c1 * c2) * q[2];
+}
+
+// phase factor e^{i q d} for the docking translations: FP64 argument, reduced
+// to revolutions in FP64 and evaluated by the hardware sin/cos in mixed mode
+template <typename T>
+__device__ __forceinline__ void phase_sincos(double qd, T &s, T &c);
+template <>
+__device__ __forceinline__ void phase_sincos<float>(double qd, float &s, float &c) {
+    double r = qd * (1.0 / k2Pi);
+    float f = (float)(r - floor(r));
+    s = __builtin_amdgcn_sinf(f);
+    c = __builtin_amdgcn_cosf(f);
+}
+template <>
+__device__ __forceinline__ void phase_sincos<double>(double qd, double &s, double &c) {
+    sincos(qd, &s, &c);
+}
+
+constexpr int kMaxM = 8;  // q-values of one shell group handled per pass
+
+
+// accumulate the docked amplitude along unit direction qv for q-values qs[0..m)
+template <typename T, typename T4>
+__device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ grids,
+                                                 const RotDev *__restrict__ rots, int nrot,
+                                                 const double4 *__restrict__ trans, int layer,
+                                                 double step, const double qv[3],
+                                                 const double *__restrict__ qs, int m,
+                                                 cplx<T> acc[kMaxM]) {
+    for (int r = 0; r < nrot; r++) {
+        const RotDev rd = rots[r];
+        double rv[3];
+        rotate_t(rd.ang, qv, rv);
+        if (fabs(rv[2]) > 1.) rv[2] = (rv[2] > 0.) ? 1. : -1.;
+        const double nth = acos(rv[2]);
+        double nph = atan2(rv[1], rv[0]);
+        if (nph < 0.0) nph += k2Pi;
+        cplx<T> amps[4], d1, d2;
+        group_planes<T, T4>(grids[rd.leaf], layer, nth, nph, amps, d1, d2);
+        const T sc = (T)rd.ang.w;
+        const bool trivial = (rd.nt == 1) && trans[rd.t0].x == 0.0 && trans[rd.t0].y == 0.0 &&
+                             trans[rd.t0].z == 0.0;
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) {
+            if (j < m) {
+                const T t = (T)((qs[j] - layer * step) / step);
+                cplx<T> a = hermite<T>(amps[1], amps[2], d1, d2, t);
+                if (trivial) {
+                    acc[j].x += a.x * sc;
+                    acc[j].y += a.y * sc;
+                } else {
+                    T sumc = 0, sums = 0;
+                    for (int tr = 0; tr < rd.nt; tr++) {
+                        const double4 tt = trans[rd.t0 + tr];
+                        const double qd = qs[j] * (tt.x * qv[0] + tt.y * qv[1] + tt.z * qv[2]);
+                        T s, c;
+                        phase_sincos<T>(qd, s, c);
+                        sumc += c;
+                        sums += s;
+                    }
+                    acc[j].x += (a.x * sumc - a.y * sums) * sc;
+                    acc[j].y += (a.y * sumc + a.x * sums) * sc;
+                }
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ------------------------------------------------------------------ MC
+// grid: x = direction block, y = active group.  mstride >= every group's m.
+// partial[(gi * nblk + bx) * mstride + j] = sum over the block's directions of |A_j|^2
+template <typename T, typename T4>
+__global__ __launch_bounds__(256) void mc_kernel(const T4 *const *__restrict__ grids,
+                                                 const RotDev *__restrict__ rots, int nrot,
+                                                 const double4 *__restrict__ trans,
+                                                 const GroupDev *__restrict__ groups,
+                                                 const int *__restrict__ active, int mstride,
+                                                 const double *__restrict__ qs, double step,
+                                                 uint64_t seed, long long dir0, int nblk,
+                                                 double *__restrict__ partial) {
+    const int gi = blockIdx.y;
+    const GroupDev G = groups[active[gi]];
+    const long long d = dir0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double u, v;
+    uniform_pair(seed, (uint32_t)d, (uint32_t)(d >> 32), (uint32_t)G.layer, kStreamMC, u, v);
+    const double th = acos(2.0 * u - 1.0), ph = k2Pi * v;
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    const double qv[3] = {cp * st, sp * st, ct};
+    __shared__ double red[4][kMaxM];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int jb = 0; jb < G.m; jb += kMaxM) {
+        const int m = min(kMaxM, G.m - jb);
+        cplx<T> acc[kMaxM];
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) acc[j] = {(T)0, (T)0};
+        docked_amplitude<T, T4>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, acc);
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) {
+            double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
+            I = wave_sum(I);
+            if (lane == 0) red[w][j] = I;
+        }
+        __syncthreads();
+        if (threadIdx.x < m) {
+            const int j = threadIdx.x;
+            partial[((size_t)gi * nblk + blockIdx.x) * mstride + jb + j] =
+                red[0][j] + red[1][j] + red[2][j] + red[3][j];
+        }
+        __syncthreads();
+    }
+}
+
+// sum the block partials of every active group in a fixed order:
+// out[gi * mstride + j]
+__global__ void mc_reduce(const GroupDev *__restrict__ groups, const int *__restrict__ active,
+                          const double *__restrict__ partial, int nblk, int mstride,
+                          double *__restrict__ out) {
+    const int gi = blockIdx.x;
+    const int m = groups[active[gi]].m;
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        double s = 0.0;
+        for (int b = 0; b < nblk; b++) s += partial[((size_t)gi * nblk + b) * mstride + j];
+        out[(size_t)gi * mstride + j] = s;
+    }
+}
+
+// ------------------------------------------------------------------ VEGAS
+constexpr int kVB = 32;  // bins per axis (GPUHybridCalc.cu:30-31)
+
+// bounds[g*66 + 0..32] theta-axis boundaries in u = (1-cos)/2, [33..65] phi-axis in phi/2pi
+template <typename T, typename T4>
+__global__ __launch_bounds__(256) void vegas_sample(const T4 *const *__restrict__ grids,
+                                                    const RotDev *__restrict__ rots, int nrot,
+                                                    const double4 *__restrict__ trans,
+                                                    const GroupDev *__restrict__ groups,
+                                                    const int *__restrict__ active, int mstride,
+                                                    const double *__restrict__ qs, double step,
+                                                    uint64_t seed, int iter, int ppb,
+                                                    const double *__restrict__ bounds,
+                                                    double *__restrict__ I) {
+    const int gi = blockIdx.y;
+    const int g = active[gi];
+    const GroupDev G = groups[g];
+    const int npts = kVB * kVB * ppb;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= npts) return;
+    const int bin = idx / ppb, ti = bin / kVB, pj = bin % kVB;
+    const double *bt = bounds + (size_t)g * 66, *bp = bt + 33;
+    double u, v;
+    uniform_pair(seed, (uint32_t)idx, (uint32_t)iter, (uint32_t)G.layer, kStreamVegas, u, v);
+    const double th = acos(1. - 2. * (u * (bt[ti + 1] - bt[ti]) + bt[ti]));
+    const double ph = (v * (bp[pj + 1] - bp[pj]) + bp[pj]) * 2. * kPi;
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(ph, &sp, &cp);
+    const double qv[3] = {cp * st, sp * st, ct};
+    double *o = I + ((size_t)gi * npts + idx) * mstride;
+    for (int jb = 0; jb < G.m; jb += kMaxM) {
+        const int m = min(kMaxM, G.m - jb);
+        cplx<T> acc[kMaxM];
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) acc[j] = {(T)0, (T)0};
+        docked_amplitude<T, T4>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, acc);
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++)
+            if (j < m) o[jb + j] = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
+    }
+}
+
+// serial damped re-binning of one axis (HybridOA.cu:181-263); boundaries the
+// serial sweep does not reach keep their previous value
+__device__ void vegas_resize(const double *var, double *b) {
+    double total = 0.0;
+    for (int i = 0; i < kVB; i++) total += var[i];
+    if (total == 0.) return;
+    double w[kVB], nb[kVB];
+    for (int i = 0; i < kVB; i++) {
+        double x;
+        if (i == 0)
+            x = (var[0] + var[1]) / 2.;
+        else if (i == kVB - 1)
+            x = (var[i] + var[i - 1]) / 2.;
+        else
+            x = (var[i - 1] + var[i] + var[i + 1]) / 3.;
+        x = total / x;
+        x = pow((x - 1.) / (x * log(x)), 1.5);
+        if (x != x) x = 0.;
+        w[i] = x;
+    }
+    double tw = 0.0;
+    for (int i = 0; i < kVB; i++) tw += w[i];
+    for (int i = 0; i < kVB; i++) nb[i] = b[i + 1];
+    double xold, xnew = 0., dw = 0., pts = tw / kVB;
+    int j = 0;
+    for (int i = 0; i < kVB; i++) {
+        dw += w[i];
+        xold = xnew;
+        xnew = b[i + 1];
+        for (; dw > pts; j++) {
+            dw -= pts;
+            if (j < kVB) nb[j] = xnew - dw * (xnew - xold) / w[i];
+        }
+    }
+    for (int i = 0; i < kVB - 1; i++) b[1 + i] = nb[i];
+}
+
+// one block (1024 threads = the 32x32 bins) per active group: integral
+// estimate of the iteration, per-bin variance marginals, re-binning and the
+// running mean (HybridVEGASBinReduceToIKernel, HybridVEGASBinReduceKernel,
+// HybridVEGASResizeGridKernel; GPUHybridCalc.cu:654-701).
+// hist[iter * nq + q] holds the running mean after iteration iter.
+__global__ __launch_bounds__(1024) void vegas_reduce(const GroupDev *__restrict__ groups,
+                                                     const int *__restrict__ active, int mstride,
+                                                     int ppb, int iter, const double *__restrict__ I,
+                                                     double *__restrict__ bounds,
+                                                     double *__restrict__ hist, int nq) {
+    const int gi = blockIdx.x;
+    const int g = active[gi];
+    const GroupDev G = groups[g];
+    const int npts = kVB * kVB * ppb;
+    const int bin = threadIdx.x, ti = bin / kVB, pj = bin % kVB;
+    double *bt = bounds + (size_t)g * 66, *bp = bt + 33;
+    const double vol = (bt[ti + 1] - bt[ti]) * (bp[pj + 1] - bp[pj]);
+    const double *Ib = I + ((size_t)gi * npts + (size_t)bin * ppb) * mstride;
+    __shared__ double sv[1024];
+    __shared__ double se[16][kMaxM];
+    __shared__ double marg[2][kVB];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // per-bin variance of vol * sum_q I over the bin's points (Welford)
+    double mean = 0., ssq = 0.;
+    for (int p = 0; p < ppb; p++) {
+        double so = 0.;
+        for (int j = 0; j < G.m; j++) so += Ib[(size_t)p * mstride + j];
+        const double diff = vol * so - mean;
+        mean += diff / (double)(p + 1);
+        ssq += diff * diff * (double)p / (double)(p + 1);
+    }
+    sv[bin] = ssq * ppb;
+    // integral: sum over bins of vol * (bin mean), q-batch by q-batch
+    for (int jb = 0; jb < G.m; jb += kMaxM) {
+        const int m = min(kMaxM, G.m - jb);
+#pragma unroll
+        for (int j = 0; j < kMaxM; j++) {
+            double e = 0.0;
+            if (j < m)
+                for (int p = 0; p < ppb; p++) e += Ib[(size_t)p * mstride + jb + j];
+            double c = wave_sum(e * vol / (double)ppb);
+            if (lane == 0) se[w][j] = c;
+        }
+        __syncthreads();
+        if (threadIdx.x < m) {
+            const int j = threadIdx.x;
+            double e = 0.0;
+            for (int k = 0; k < 16; k++) e += se[k][j];
+            const int q = G.q0 + jb + j;
+            if (iter > 0) e = (hist[(size_t)(iter - 1) * nq + q] * (double)iter + e) / (double)(iter + 1);
+            hist[(size_t)iter * nq + q] = e;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kVB) {
+        double a = 0.0, b = 0.0;
+        for (int k = 0; k < kVB; k++) {
+            a += sv[threadIdx.x * kVB + k];  // theta bin threadIdx.x
+            b += sv[k * kVB + threadIdx.x];  // phi bin threadIdx.x
+        }
+        marg[0][threadIdx.x] = a;
+        marg[1][threadIdx.x] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) vegas_resize(marg[0], bt);
+    if (threadIdx.x == 64) vegas_resize(marg[1], bp);
+}
+
+// ------------------------------------------------------------------ single points
+// GetSphr (Grid.cpp:487-548): the CPU-path lookup used by getAmplitude and GK
+template <typename T, typename T4>
+__device__ cplx<T> get_sphr(const T4 *__restrict__ g, int N, double step, double rr, double th,
+                            double ph) {
+    const T4 o = g[0];
+    if (rr == 0.0) return {o.x, o.y};
+    double frc = rr / step;
+    frc = __longlong_as_double(__double_as_longlong(frc) + 8);
+    const int fInd = (int)frc;
+    const double frac = frc - (double)fInd;
+    cplx<T> p[4];
+    p[1] = fInd > 0 ? plane_lookup<T, T4, true>(g + shell_base(fInd), fInd, th, ph) : cplx<T>{o.x, o.y};
+    if (frac < 1.e-9) return p[1];
+    p[2] = plane_lookup<T, T4, true>(g + shell_base(fInd + 1), fInd + 1, th, ph);
+    if (fInd == 0) {
+        double nth, nph;
+        antipode(th, ph, nth, nph);
+        p[0] = plane_lookup<T, T4, true>(g + 1, 1, nth, nph);
+    } else if (fInd == 1) {
+        p[0] = {o.x, o.y};
+    } else {
+        p[0] = plane_lookup<T, T4, true>(g + shell_base(fInd - 1), fInd - 1, th, ph);
+    }
+    p[3] = (fInd + 2 < N) ? plane_lookup<T, T4, true>(g + shell_base(fInd + 2), fInd + 2, th, ph) : p[2];
+    cplx<T> d1, d2;
+    four_point<T>(p, d1, d2);
+    return hermite<T>(p[1], p[2], d1, d2, (T)frac);
+}
+
+// GetCart (Grid.cpp:384-392)
+template <typename T, typename T4>
+__device__ __forceinline__ cplx<T> get_cart(const T4 *__restrict__ g, int N, double step, double x,
+                                            double y, double z) {
+    double r = sqrt(x * x + y * y + z * z);
+    if (fabs(z) > r) r = fabs(z);
+    if (r == 0.0) return get_sphr<T, T4>(g, N, step, 0.0, 0.0, 0.0);
+    const double th = acos(z / r);
+    double ph = atan2(y, x);
+    if (ph < 0.0) ph += k2Pi;
+    return get_sphr<T, T4>(g, N, step, r, th, ph);
+}
+
+// batch GetSphr at (q, theta, phi) triples
+template <typename T, typename T4>
+__global__ void sphr_batch(const T4 *__restrict__ g, int N, double step, int n,
+                           const double *__restrict__ q, const double *__restrict__ th,
+                           const double *__restrict__ ph, double2 *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    cplx<T> a = get_sphr<T, T4>(g, N, step, q[i], th[i], ph[i]);
+    out[i] = make_double2((double)a.x, (double)a.y);
+}
+
+// |A|^2 of JacobianSphereGrid::CalculateIntensity's samples (Grid.cpp:421-466):
+// phi[i] and v2[i] in [0,2), theta = acos(v2-1), lookup through GetCart
+template <typename T, typename T4>
+__global__ void seeded_samples(const T4 *__restrict__ g, int N, double step, double q, long long n,
+                               const double *__restrict__ phi, const double *__restrict__ v2,
+                               double *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double theta = acos(v2[i] - 1.0);
+    double st, ct, sp, cp;
+    sincos(theta, &st, &ct);
+    sincos(phi[i], &sp, &cp);
+    cplx<T> a = get_cart<T, T4>(g, N, step, q * st * cp, q * st * sp, q * ct);
+    out[i] = (double)a.x * (double)a.x + (double)a.y * (double)a.y;
+}
+
+// |A(q)|^2 of docked leaves at cartesian q (Amplitude.cpp:3035-3056 composed
+// with Symmetries.cpp:697-716): the integrand of Gauss-Kronrod
+
+template <typename T, typename T4>
+__global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
+                                  const RotDev *__restrict__ rots, int nrot,
+                                  const double4 *__restrict__ trans, int n,
+                                  const double *__restrict__ qxyz, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double q[3] = {qxyz[3 * i], qxyz[3 * i + 1], qxyz[3 * i + 2]};
+    double ax = 0.0, ay = 0.0;
+    for (int r = 0; r < nrot; r++) {
+        const RotDev rd = rots[r];
+        double rv[3];
+        rotate_t(rd.ang, q, rv);
+        const LeafGeo G = geo[rd.leaf];
+        cplx<T> a = get_cart<T, T4>(grids[rd.leaf], G.N, G.step, rv[0], rv[1], rv[2]);
+        double sc = 0.0, ss = 0.0;
+        for (int tr = 0; tr < rd.nt; tr++) {
+            const double4 t = trans[rd.t0 + tr];
+            double s, c;
+            sincos(q[0] * t.x + q[1] * t.y + q[2] * t.z, &s, &c);
+            sc += c;
+            ss += s;
+        }
+        ax += ((double)a.x * sc - (double)a.y * ss) * rd.ang.w;
+        ay += ((double)a.y * sc + (double)a.x * ss) * rd.ang.w;
+    }
+    out[i] = ax * ax + ay * ay;
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot,
+               const double4 *trans, const GroupDev *groups, const int *active, int nactive,
+               int mstride, const double *qs, double step, uint64_t seed, long long dir0, int nblk,
+               double *partial, double *out, hipStream_t s) {
+    dim3 grid(nblk, nactive);
+    if (fp64)
+        mc_kernel<double, double4><<<grid, 256, 0, s>>>((const double4 *const *)grids, rots, nrot, trans,
+                                                         groups, active, mstride, qs, step, seed, dir0,
+                                                         nblk, partial);
+    else
+        mc_kernel<float, float4><<<grid, 256, 0, s>>>((const float4 *const *)grids, rots, nrot, trans,
+                                                       groups, active, mstride, qs, step, seed, dir0,
+                                                       nblk, partial);
+    mc_reduce<<<nactive, 64, 0, s>>>(groups, active, partial, nblk, mstride, out);
+}
+
+void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, int nrot,
+                       const double4 *trans, const GroupDev *groups, const int *active, int nactive,
+                       int mstride, const double *qs, double step, uint64_t seed, int iter, int ppb,
+                       double *bounds, double *I, double *hist, int nq, hipStream_t s) {
+    const int npts = kVB * kVB * ppb;
+    dim3 grid((npts + 255) / 256, nactive);
+    if (fp64)
+        vegas_sample<double, double4><<<grid, 256, 0, s>>>((const double4 *const *)grids, rots, nrot,
+                                                            trans, groups, active, mstride, qs, step,
+                                                            seed, iter, ppb, bounds, I);
+    else
+        vegas_sample<float, float4><<<grid, 256, 0, s>>>((const float4 *const *)grids, rots, nrot, trans,
+                                                          groups, active, mstride, qs, step, seed, iter,
+                                                          ppb, bounds, I);
+    vegas_reduce<<<nactive, 1024, 0, s>>>(groups, active, mstride, ppb, iter, I, bounds, hist, nq);
+}
+
+void launch_sphr_batch(bool fp64, const void *g, int N, double step, int n, const double *q,
+                       const double *th, const double *ph, double2 *out, hipStream_t s) {
+    unsigned nb = (n + 255) / 256;
+    if (fp64)
+        sphr_batch<double, double4><<<nb, 256, 0, s>>>((const double4 *)g, N, step, n, q, th, ph, out);
+    else
+        sphr_batch<float, float4><<<nb, 256, 0, s>>>((const float4 *)g, N, step, n, q, th, ph, out);
+}
+
+void launch_seeded_samples(bool fp64, const void *g, int N, double step, double q, long long n,
+                           const double *phi, const double *v2, double *out, hipStream_t s) {
+    unsigned nb = (unsigned)((n + 255) / 256);
+    if (fp64)
+        seeded_samples<double, double4><<<nb, 256, 0, s>>>((const double4 *)g, N, step, q, n, phi, v2, out);
+    else
+        seeded_samples<float, float4><<<nb, 256, 0, s>>>((const float4 *)g, N, step, q, n, phi, v2, out);
+}
+
+void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots,
+                        int nrot, const double4 *trans, int n, const double *qxyz, double *out,
+                        hipStream_t s) {
+    unsigned nb = (n + 255) / 256;
+    if (fp64)
+        docked_cart_batch<double, double4><<<nb, 256, 0, s>>>((const double4 *const *)grids, geo, rots,
+                                                               nrot, trans, n, qxyz, out);
+    else
+        docked_cart_batch<float, float4><<<nb, 256, 0, s>>>((const float4 *const *)grids, geo, rots, nrot,
+                                                             trans, n, qxyz, out);
+}
+
+}  // namespace dplus
+
+namespace dplus {
+
+// ------------------------------------------------------------------ grid-level docking
+__device__ __forceinline__ void node_indices(long long index, int &qi, long long &ti, long long &pi) {
+    if (index <= 0) {
+        qi = 0;
+        ti = pi = 0;
+        return;
+    }
+    long long q = (long long)cbrt((double)index / 6.0);
+    if (q < 1) q = 1;
+    while (q > 1 && shell_base(q) > index) q--;
+    while (shell_base(q + 1) <= index) q++;
+    const long long rem = index - shell_base(q);
+    qi = (int)q;
+    ti = rem / (6 * q);
+    pi = rem % (6 * q);
+}
+
+// ManualSymmetry::calcAmplitude (Symmetries.cpp:697-716) at every node of the
+// output grid (Grid.cpp:760-826 node vectors), sub-amplitudes through GetCart
+template <typename T, typename T4>
+__global__ void dock_grid_kernel(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
+                                 const RotDev *__restrict__ rots, int nrot, const double4 *__restrict__ trans,
+                                 T4 *__restrict__ out, double step_out, long long P, double scale) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= P) return;
+    int qi;
+    long long ti, pi;
+    node_indices(idx, qi, ti, pi);
+    double q[3] = {0.0, 0.0, 0.0};
+    if (qi > 0) {
+        const double qI = (double)qi * step_out;
+        const double tI = kPi * (double)ti / (double)(3 * qi);
+        const double pI = k2Pi * (double)pi / (double)(6 * qi);
+        double st, ct, sp, cp;
+        sincos(tI, &st, &ct);
+        sincos(pI, &sp, &cp);
+        q[0] = qI * st * cp;
+        q[1] = qI * st * sp;
+        q[2] = qI * ct;
+    }
+    double ax = 0.0, ay = 0.0;
+    for (int r = 0; r < nrot; r++) {
+        const RotDev rd = rots[r];
+        double rv[3];
+        rotate_t(rd.ang, q, rv);
+        const LeafGeo G = geo[rd.leaf];
+        cplx<T> a = get_cart<T, T4>(grids[rd.leaf], G.N, G.step, rv[0], rv[1], rv[2]);
+        double sc = 0.0, ss = 0.0;
+        for (int tr = 0; tr < rd.nt; tr++) {
+            const double4 t = trans[rd.t0 + tr];
+            double s, c;
+            sincos(q[0] * t.x + q[1] * t.y + q[2] * t.z, &s, &c);
+            sc += c;
+            ss += s;
+        }
+        ax += ((double)a.x * sc - (double)a.y * ss) * rd.ang.w;
+        ay += ((double)a.y * sc + (double)a.x * ss) * rd.ang.w;
+    }
+    T4 o;
+    o.x = (T)(ax * scale);
+    o.y = (T)(ay * scale);
+    o.z = (T)0;
+    o.w = (T)0;
+    out[idx] = o;
+}
+
+void dock_grid(const DockSet &D, Grid &out, double scale, hipStream_t s) {
+    const unsigned nb = (unsigned)((out.P + 255) / 256);
+    if (D.fp64)
+        dock_grid_kernel<double, double4><<<nb, 256, 0, s>>>((const double4 *const *)D.grids.p,
+                                                              (const LeafGeo *)D.geos.p, (const RotDev *)D.rotb.p,
+                                                              D.nrot, (const double4 *)D.trb.p,
+                                                              (double4 *)out.dev.p, out.step, out.P, scale);
+    else
+        dock_grid_kernel<float, float4><<<nb, 256, 0, s>>>((const float4 *const *)D.grids.p,
+                                                            (const LeafGeo *)D.geos.p, (const RotDev *)D.rotb.p,
+                                                            D.nrot, (const double4 *)D.trb.p, (float4 *)out.dev.p,
+                                                            out.step, out.P, scale);
+    check_hip(hipGetLastError(), "dock_grid launch");
+    out.splines(s);
+    check_hip(hipStreamSynchronize(s), "dock_grid");
+}
+
+}  // namespace dplus

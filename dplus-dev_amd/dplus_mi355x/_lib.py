@@ -1,0 +1,135 @@
+"""ctypes binding of libdplus_mi355x (include/dplus_mi355x.h).
+
+The shared library is built in-tree by ``make -C dplus-dev_amd``; loading
+fails loudly when it is missing -- there is no CPU fallback.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdplus_mi355x.so")
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+OA_MC, OA_VEGAS, OA_GK = 0, 1, 2
+CALC_ATOMIC_FORMFACTORS = 1
+CALC_DUMMY_SOLVENT = 2
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+class DplusError(RuntimeError):
+    """Error raised by the library: ``code`` follows the reference's ErrorCode enum
+    (Common/Common.h:184-228), the message is the library's."""
+
+    def __init__(self, code, message):
+        self.error_code = code
+        super().__init__(message)
+
+
+class PdbDesc(ctypes.Structure):
+    _fields_ = [
+        ("natoms", ctypes.c_int), ("xyz", _D), ("ion", _I), ("nion", ctypes.c_int), ("coef9", _D),
+        ("rad", _D), ("sol_ed", ctypes.c_double), ("c1", ctypes.c_double), ("flags", ctypes.c_int),
+        ("scale", ctypes.c_double), ("nbox", ctypes.c_int), ("box_dims", _I), ("box_com", _D),
+        ("vox_step", ctypes.c_double), ("box_contrast", ctypes.c_double),
+    ]
+
+
+class Leaf(ctypes.Structure):
+    _fields_ = [("grid", ctypes.c_void_p), ("nrot", ctypes.c_int), ("rot", _D), ("ntrans", _I), ("trans", _D)]
+
+
+class OAParams(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("max_dirs", ctypes.c_longlong),
+        ("chunk", ctypes.c_int), ("eps", ctypes.c_double), ("vegas_iterations", ctypes.c_int),
+        ("vegas_points_per_bin", ctypes.c_int), ("gk_max_depth", ctypes.c_longlong),
+    ]
+
+
+class OAStats(ctypes.Structure):
+    _fields_ = [("dirs_used", ctypes.c_longlong), ("iterations_used", ctypes.c_int),
+                ("evaluations", ctypes.c_longlong), ("seconds", ctypes.c_double)]
+
+
+# every symbol of include/dplus_mi355x.h with its ctypes signature
+SIGNATURES = {
+    "dplus_last_error": (ctypes.c_char_p, []),
+    "dplus_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "dplus_grid_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "dplus_grid_destroy": (None, [ctypes.c_void_p]),
+    "dplus_grid_info": (ctypes.c_int, [ctypes.c_void_p, _I, _D, _D, _I, ctypes.POINTER(ctypes.c_longlong), _I]),
+    "dplus_grid_data": (_D, [ctypes.c_void_p]),
+    "dplus_grid_interp": (_D, [ctypes.c_void_p]),
+    "dplus_grid_index_from_indices": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong]),
+    "dplus_grid_indices_from_index": (None, [ctypes.c_longlong, _I, ctypes.POINTER(ctypes.c_longlong),
+                                             ctypes.POINTER(ctypes.c_longlong)]),
+    "dplus_grid_upload": (ctypes.c_int, [ctypes.c_void_p]),
+    "dplus_grid_calculate_splines": (ctypes.c_int, [ctypes.c_void_p]),
+    "dplus_grid_download": (ctypes.c_int, [ctypes.c_void_p]),
+    "dplus_grid_get_sphr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D]),
+    "dplus_grid_calculate_intensity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_uint, ctypes.c_ulonglong,
+                                                      ctypes.c_double, ctypes.c_double, _D]),
+    "dplus_pdb_build_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PdbDesc), ctypes.c_int]),
+    "dplus_orientation_average": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_int, _D,
+                                                 ctypes.POINTER(OAParams), _D, ctypes.POINTER(OAStats)]),
+    "dplus_dock_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_double]),
+    "dplus_synth_walk": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C dplus-dev_amd` "
+                              "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().dplus_last_error().decode()
+        try:
+            d = json.loads(msg)
+            raise DplusError(d.get("code", rc), d.get("message", msg))
+        except ValueError:
+            raise DplusError(rc, msg)
+
+
+def dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(512)
+    check(lib().dplus_device_info(buf, 512))
+    return json.loads(buf.value.decode())

@@ -1,0 +1,134 @@
+"""Jacobian spherical grids, PDB amplitude leaves, docking and orientation
+averages over the C ABI of libdplus_mi355x."""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+class Grid:
+    """A Jacobian spherical grid resident on the GPU (JacobianSphereGrid,
+    Backend/Backend/Grid.h:231-327).  ``data``/``interp`` are zero-copy numpy
+    views of the host mirror in the reference layout (2*P doubles)."""
+
+    def __init__(self, grid_size, q_max, precision=L.PRECISION_MIXED):
+        h = ctypes.c_void_p()
+        L.check(L.lib().dplus_grid_create(int(grid_size), float(q_max), int(precision), ctypes.byref(h)))
+        self._h = h
+        gs, qm, st, n, pts, pr = (ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int(),
+                                  ctypes.c_longlong(), ctypes.c_int())
+        L.check(L.lib().dplus_grid_info(h, ctypes.byref(gs), ctypes.byref(qm), ctypes.byref(st), ctypes.byref(n),
+                                        ctypes.byref(pts), ctypes.byref(pr)))
+        self.grid_size, self.q_max, self.step_size = gs.value, qm.value, st.value
+        self.actual_size, self.points, self.precision = n.value, pts.value, pr.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def data(self):
+        p = L.lib().dplus_grid_data(self._h)
+        return np.ctypeslib.as_array(p, shape=(2 * self.points,))
+
+    @property
+    def interp(self):
+        p = L.lib().dplus_grid_interp(self._h)
+        return np.ctypeslib.as_array(p, shape=(2 * self.points,))
+
+    def upload(self):
+        L.check(L.lib().dplus_grid_upload(self._h))
+
+    def download(self):
+        L.check(L.lib().dplus_grid_download(self._h))
+
+    def calculate_splines(self):
+        L.check(L.lib().dplus_grid_calculate_splines(self._h))
+
+    def get_sphr(self, q, theta, phi):
+        q, theta, phi = (np.atleast_1d(L.f64(x)) for x in (q, theta, phi))
+        n = len(q)
+        out = np.zeros(2 * n)
+        L.check(L.lib().dplus_grid_get_sphr(self._h, n, L.dp(q), L.dp(theta), L.dp(phi), L.dp(out)))
+        return out[0::2] + 1j * out[1::2]
+
+    def calculate_intensity(self, q, theta=None, epsilon=1e-3, seed=0, iterations=1000000, phi_min=0.0,
+                            phi_max=2 * np.pi):
+        r = ctypes.c_double()
+        th = -1.0 if theta is None else float(theta)
+        L.check(L.lib().dplus_grid_calculate_intensity(self._h, float(q), th, float(epsilon), int(seed),
+                                                       int(iterations), float(phi_min), float(phi_max),
+                                                       ctypes.byref(r)))
+        return r.value
+
+    def build_pdb(self, xyz, ion, coef9, rad, sol_ed=0.0, c1=1.0, flags=L.CALC_ATOMIC_FORMFACTORS, scale=1.0,
+                  box_dims=None, box_com=None, vox_step=0.0, box_contrast=0.0, host_copy=True):
+        """PDBAmplitude::calculateGrid for atoms (nm) of ion types ion[i]."""
+        xyz, ion = L.f64(xyz).reshape(-1, 3), L.i32(ion)
+        coef9, rad = L.f64(coef9).reshape(-1, 9), L.f64(rad)
+        if box_dims is None:
+            bd, bc, nb = np.zeros(3, np.int32), np.zeros(3), 0
+        else:
+            bd, bc, nb = L.i32(box_dims).reshape(-1, 3), L.f64(box_com).reshape(-1, 3), len(box_dims)
+        d = L.PdbDesc(len(ion), L.dp(xyz), L.ip(ion), len(coef9), L.dp(coef9), L.dp(rad), sol_ed, c1, flags, scale,
+                      nb, L.ip(bd), L.dp(bc), vox_step, box_contrast)
+        L.check(L.lib().dplus_pdb_build_grid(self._h, ctypes.byref(d), 1 if host_copy else 0))
+
+    def __del__(self):
+        try:
+            if self._h:
+                L.lib().dplus_grid_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def index_from_indices(qi, ti, pi):
+    return int(L.lib().dplus_grid_index_from_indices(qi, ti, pi))
+
+
+def indices_from_index(idx):
+    qi, ti, pi = ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong()
+    L.lib().dplus_grid_indices_from_index(idx, ctypes.byref(qi), ctypes.byref(ti), ctypes.byref(pi))
+    return qi.value, ti.value, pi.value
+
+
+class Docking:
+    """Leaves of a hybrid orientation average: each leaf is a Grid seen under
+    rotations rot[r] = (alpha, beta, gamma [rad], scale) with translations."""
+
+    def __init__(self, leaves):
+        self._keep = []
+        arr = (L.Leaf * len(leaves))()
+        for i, (grid, rot, ntrans, trans) in enumerate(leaves):
+            rot, ntrans, trans = L.f64(rot).reshape(-1, 4), L.i32(ntrans), L.f64(trans).reshape(-1, 3)
+            self._keep += [grid, rot, ntrans, trans]
+            arr[i] = L.Leaf(grid.handle, len(rot), L.dp(rot), L.ip(ntrans), L.dp(trans))
+        self.arr, self.n = arr, len(leaves)
+
+    @staticmethod
+    def single(grid):
+        return Docking([(grid, [[0, 0, 0, 1]], [1], [[0, 0, 0]])])
+
+
+def orientation_average(dock, q, method=L.OA_MC, seed=1, max_dirs=1 << 20, chunk=8192, eps=0.0,
+                        vegas_iterations=10, vegas_points_per_bin=4, gk_max_depth=1 << 40):
+    q = L.f64(q)
+    out = np.zeros(len(q))
+    p = L.OAParams(method, seed, max_dirs, chunk, eps, vegas_iterations, vegas_points_per_bin, gk_max_depth)
+    st = L.OAStats()
+    L.check(L.lib().dplus_orientation_average(dock.n, dock.arr, len(q), L.dp(q), ctypes.byref(p), L.dp(out),
+                                              ctypes.byref(st)))
+    return out, st
+
+
+def dock_grid(out, dock, scale=1.0):
+    L.check(L.lib().dplus_dock_grid(out.handle, dock.n, dock.arr, float(scale)))
+
+
+def synth_walk(seed, n, step=0.38, min_sep=0.12, axes=(3.0, 3.0, 3.0)):
+    out = np.zeros(3 * n)
+    L.check(L.lib().dplus_synth_walk(int(seed), int(n), float(step), float(min_sep), float(axes[0]),
+                                     float(axes[1]), float(axes[2]), L.dp(out)))
+    return out.reshape(-1, 3)

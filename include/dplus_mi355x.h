@@ -1,0 +1,160 @@
+/*
+ * dplus_mi355x.h -- C ABI of libdplus_mi355x, the MI355X-native scattering
+ * hot path of D+ (amplitude grids, symmetry docking, orientation averages).
+ *
+ * Plain C types only.  Every function returns 0 on success or an error code
+ * of the reference's ErrorCode enum (Common/Common.h:184-228); the message of
+ * the last failure on the calling thread is available from
+ * dplus_last_error() as JSON {"code": int, "message": str}, the convention
+ * PythonBackendWrapper::ConvertException uses (Backend/Backend/PythonBackendWrapper.cpp:49-60).
+ *
+ * All device work runs on one GPU (device 0 unless DPLUS_DEVICE is set).
+ */
+#ifndef DPLUS_MI355X_H
+#define DPLUS_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPLUS_PRECISION_FP64 0  /* FP64 phases, sincos and accumulation */
+#define DPLUS_PRECISION_MIXED 1 /* FP32 bulk, FP64 range reduction and OA accumulation */
+
+#define DPLUS_OA_MC 0    /* "Monte Carlo (Mersenne Twister)" */
+#define DPLUS_OA_VEGAS 1 /* "Adaptive (VEGAS) Monte Carlo" */
+#define DPLUS_OA_GK 2    /* "Adaptive Gauss Kronrod" */
+
+#define DPLUS_CALC_ATOMIC_FORMFACTORS 1 /* Backend/GPU/atomicFormDefines.h:5 */
+#define DPLUS_CALC_DUMMY_SOLVENT 2      /* Backend/GPU/atomicFormDefines.h:6 */
+
+/* Last error of the calling thread as JSON; valid until the next call. */
+const char *dplus_last_error(void);
+
+/* Library/device description as JSON into buf (truncated to len). */
+int dplus_device_info(char *buf, int len);
+
+/* ------------------------------------------------------------------ grid */
+/* Opaque Jacobian spherical grid: device-resident node array plus an
+ * optional host mirror in the reference layout (2*P doubles, complex
+ * interleaved, Backend/Backend/Grid.cpp:1078-1111).
+ * Replaces JacobianSphereGrid(gridSize, qMax) (Backend/Backend/Grid.h:231-327,
+ * bound by PythonInterface/dplus/wrappers/Grid.pxd:20-48). */
+typedef struct dplus_grid dplus_grid;
+
+/* JacobianSphereGrid::JacobianSphereGrid(u16, double) (Grid.cpp:204-211);
+ * an odd grid_size is bumped by one as Amplitude does (Amplitude.cpp:1628). */
+int dplus_grid_create(int grid_size, double qmax, int precision, dplus_grid **out);
+void dplus_grid_destroy(dplus_grid *g);
+/* GetSize, GetQMax, stepSize, GetActualSize (= shells N), point count, precision */
+int dplus_grid_info(const dplus_grid *g, int *grid_size, double *qmax, double *step,
+                    int *actual_size, long long *points, int *precision);
+/* GetDataPointer (Grid.cpp:757, Grid.pyx:50-56): host values, 2*P doubles.
+ * The host mirror reflects the device after dplus_grid_download(). */
+double *dplus_grid_data(dplus_grid *g);
+/* GetInterpolantPointer (Grid.cpp:1207): host spline coefficients, 2*P doubles */
+double *dplus_grid_interp(dplus_grid *g);
+/* IndexFromIndices / IndicesFromIndex (Grid.cpp:1078-1111) */
+long long dplus_grid_index_from_indices(int qi, long long ti, long long pi);
+void dplus_grid_indices_from_index(long long index, int *qi, long long *ti, long long *pi);
+/* host values -> device, then CalculateSplines (Grid.cpp:1113-1151) on the GPU */
+int dplus_grid_upload(dplus_grid *g);
+/* CalculateSplines on the device values */
+int dplus_grid_calculate_splines(dplus_grid *g);
+/* device values and spline coefficients -> host mirror */
+int dplus_grid_download(dplus_grid *g);
+/* GetSphr(q, theta, phi) (Grid.cpp:487-548) for n points; out = 2n doubles */
+int dplus_grid_get_sphr(dplus_grid *g, int n, const double *q, const double *theta,
+                        const double *phi, double *out);
+/* CalculateIntensity(q, [theta,] eps, seed, iterations, phi_min, phi_max)
+ * (Grid.cpp:393-484) with the reference's std::mt19937 stream; theta < 0 draws
+ * theta at random and, like the reference's overload, ignores the phi range. */
+int dplus_grid_calculate_intensity(dplus_grid *g, double q, double theta, double eps,
+                                   unsigned seed, unsigned long long iterations, double phi_min,
+                                   double phi_max, double *result);
+
+/* ------------------------------------------------------------------ PDB leaf */
+/* Atoms and solvent description of PDBAmplitude (Backend/Backend/PDBAmplitude.cpp:1199-1336):
+ * ion[i] indexes the nion rows of coef9 (a1,b1,a2,b2,a3,b3,a4,b4,c; X-ray table,
+ * PDBReaderLib/PDBReaderLib.cpp:2260-2293) and rad (nm).  Boxes are solvent
+ * boxes (dims in voxels, centre in nm) added with box_contrast
+ * (outer-solvent layer: rho_out - rho_0, PDBAmplitude.cpp:298-314). */
+typedef struct {
+    int natoms;
+    const double *xyz; /* 3*natoms, nm */
+    const int *ion;    /* natoms */
+    int nion;
+    const double *coef9; /* 9*nion */
+    const double *rad;   /* nion */
+    double sol_ed;       /* rho_0 for the dummy-atom solvent, e/nm^3 */
+    double c1;
+    int flags; /* DPLUS_CALC_* */
+    double scale;
+    int nbox;
+    const int *box_dims;  /* 3*nbox */
+    const double *box_com; /* 3*nbox */
+    double vox_step;
+    double box_contrast;
+} dplus_pdb_desc;
+
+/* PDBAmplitude::calculateGrid (PDBAmplitude.cpp:319-435, GPU path
+ * PDBJacobianGridAmplitudeCalculation PDBAmplitudeCUDA.cu:203-436) followed by
+ * CalculateSplines; host_copy != 0 also fills the host mirror. */
+int dplus_pdb_build_grid(dplus_grid *g, const dplus_pdb_desc *d, int host_copy);
+
+/* ------------------------------------------------------------------ orientation average */
+/* A leaf of the hybrid regime (Amplitude.cpp:2028-2303): a grid seen under nrot
+ * rotations rot[4r..4r+3] = (alpha, beta, gamma [rad], scale), rotation r placed
+ * at ntrans[r] translations (3 doubles each, nm, packed in rotation order). */
+typedef struct {
+    dplus_grid *grid;
+    int nrot;
+    const double *rot;
+    const int *ntrans;
+    const double *trans;
+} dplus_leaf;
+
+typedef struct {
+    int method;             /* DPLUS_OA_* */
+    unsigned long long seed;
+    long long max_dirs;     /* MC: directions per shell group ("OrientationIterations") */
+    int chunk;              /* MC: directions between convergence checks (multiple of 256) */
+    double eps;             /* "Convergence"; <= 0 runs the full count */
+    int vegas_iterations;
+    int vegas_points_per_bin; /* 32x32 bins; points per iteration = 1024 * this */
+    long long gk_max_depth;
+} dplus_oa_params;
+
+typedef struct {
+    long long dirs_used;  /* MC directions per group actually used (max over groups) */
+    int iterations_used;  /* VEGAS iterations (max over groups) */
+    long long evaluations; /* GK integrand evaluations */
+    double seconds;       /* wall time of the call */
+} dplus_oa_stats;
+
+/* DomainModel::CalculateIntensityVector OA step: GPUCalcMCOAJacobSphrDD
+ * (Backend/GPU/OAJacob.cu:132-383), GPUHybridCalculator::ComputeIntensity
+ * (Backend/GPU/GPUHybridCalc.cu:164-720) and GaussKron2DSphereRecurs
+ * (Backend/Backend/Amplitude.cpp:1416-1467).  q must be ascending. */
+int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const double *q,
+                              const dplus_oa_params *p, double *intensity, dplus_oa_stats *st);
+
+/* ------------------------------------------------------------------ manual symmetry */
+/* ManualSymmetry::calculateGrid (DefaultModels/Symmetries.cpp:879-962, GPU
+ * ConstructManualSymmetryJacobianKernel Backend/GPU/ManSymGenJacob.cu:35-282):
+ * out(q) = scale * sum over leaves/rotations/translations of A_leaf(R^T q) e^{i q.t},
+ * evaluated at every node of out, then CalculateSplines. */
+int dplus_dock_grid(dplus_grid *out, int nleaf, const dplus_leaf *leaves, double scale);
+
+/* ------------------------------------------------------------------ synthetic inputs */
+/* Seeded self-avoiding random walk (step, minimum separation in nm) confined to
+ * the ellipsoid with semi-axes (ax, ay, az); xyz receives 3*n coordinates. */
+int dplus_synth_walk(unsigned long long seed, int n, double step, double min_sep, double ax,
+                     double ay, double az, double *xyz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPLUS_MI355X_H */

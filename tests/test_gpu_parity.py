@@ -1,0 +1,210 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the
+same seeded inputs.  Tolerances: FP64 mode 1e-9 relative on grids and 1e-6 on
+I(q) (the north-star FP64 bar); mixed mode 1e-4 relative on I(q)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-6
+MIXED_TOL = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def structure(n, seed, radius=1.5):
+    import dplus_mi355x.synth as S
+    xyz, el = S.walk_structure(seed, n, (radius, radius, radius))
+    coef9, rad, ion, order = S.structure_tables(el)
+    return xyz, ion, coef9, rad
+
+
+def oracle_ions(O, coef9, rad, ion, sol=334.0, c1=1.0):
+    flags = O.CALC_ATOMIC_FORMFACTORS | (O.CALC_DUMMY_SOLVENT if sol else 0)
+    return O.Ions(coef9, rad, np.bincount(ion, minlength=len(coef9)), sol, c1, flags)
+
+
+def gpu_grid(D, gs, qmax, prec, xyz, ion, coef9, rad, sol=334.0, **kw):
+    g = D.Grid(gs, qmax, prec)
+    flags = D.CALC_ATOMIC_FORMFACTORS | (D.CALC_DUMMY_SOLVENT if sol else 0)
+    g.build_pdb(xyz, ion, coef9, rad, sol_ed=sol, flags=flags, **kw)
+    return g
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.fixture(scope="module")
+def small(dplus, oracle):
+    say("[case] small structure: 300 atoms, gs 24, qmax 5")
+    xyz, ion, coef9, rad = structure(300, 7)
+    gs, qmax = 24, 5.0
+    ions = oracle_ions(oracle, coef9, rad, ion)
+    data = oracle.grid_fill_pdb(gs, qmax, xyz, ion, ions)
+    co = oracle.grid_splines(gs, data)
+    return dict(xyz=xyz, ion=ion, coef9=coef9, rad=rad, gs=gs, qmax=qmax, ions=ions, data=data, co=co)
+
+
+def test_device_is_gfx950(dplus):
+    info = dplus.device_info()
+    say("[device]", info)
+    assert info["ok"] and info["arch"].startswith("gfx950")
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_grid_build_matches_oracle(dplus, oracle, small, prec):
+    say(f"[case] grid build precision={prec}")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    tol = 1e-10 if prec == 0 else 2e-5
+    assert relmax(g.data, s["data"]) < tol
+    assert relmax(g.interp, s["co"]) < (1e-10 if prec == 0 else 2e-5)
+
+
+def test_splines_of_uploaded_data(dplus, oracle):
+    say("[case] splines of uploaded random data")
+    gs = 40
+    rng = np.random.default_rng(1)
+    g = dplus.Grid(gs, 5.0, 0)
+    d = rng.normal(size=2 * g.points)
+    g.data[:] = d
+    g.upload()
+    g.download()
+    assert np.array_equal(g.data, d)
+    assert relmax(g.interp, oracle.grid_splines(gs, d)) < 1e-13
+
+
+def test_get_sphr_matches_oracle(dplus, oracle, small):
+    say("[case] get_sphr")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], 0, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    rng = np.random.default_rng(2)
+    n = 400
+    q = np.concatenate([rng.uniform(0, s["qmax"], n - 4), [0.0, s["qmax"], 1.0, 2.5]])
+    th = np.concatenate([rng.uniform(0, np.pi, n - 4), [0.0, np.pi, 0.0, np.pi]])
+    ph = np.concatenate([rng.uniform(0, 2 * np.pi, n - 4), [0.0, 2 * np.pi, 0.0, 3.0]])
+    a = g.get_sphr(q, th, ph)
+    b = np.array([oracle.get_sphr(s["gs"], s["qmax"], s["data"], s["co"], *x) for x in zip(q, th, ph)])
+    assert relmax(a, b) < 1e-10
+
+
+def test_seeded_intensity_matches_oracle(dplus, oracle, small):
+    say("[case] seeded CalculateIntensity")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], 0, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    for q, eps, seed, n in ((2.3, 1e-3, 0, 20000), (0.7, 0.0, 5, 4000), (0.0, 1e-3, 1, 100)):
+        mine = g.calculate_intensity(q, None, eps, seed, n)
+        ref, _ = oracle.grid_intensity_seeded(s["gs"], s["qmax"], s["data"], s["co"], q, eps, seed, n)
+        assert mine == pytest.approx(ref, rel=1e-10)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_mc_matches_oracle(dplus, oracle, small, prec):
+    say(f"[case] MC precision={prec}")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    q = np.linspace(0, s["qmax"], 61)
+    mine, st = dplus.orientation_average(dplus.Docking.single(g), q, seed=9, max_dirs=4096, chunk=1024)
+    ref, _ = oracle.mc_oa(oracle.Docked.single(s["gs"], s["qmax"], s["data"], s["co"]), q, 9, 4096, 1024)
+    assert np.max(np.abs(mine - ref) / ref) < (FP64_TOL * 1e-3 if prec == 0 else MIXED_TOL)
+
+
+def test_mc_convergence_path_matches_oracle(dplus, oracle, small):
+    say("[case] MC with convergence")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], 0, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    q = np.linspace(0.05, s["qmax"], 40)
+    mine, st = dplus.orientation_average(dplus.Docking.single(g), q, seed=3, max_dirs=20 * 512, chunk=512,
+                                         eps=3e-3)
+    ref, used = oracle.mc_oa(oracle.Docked.single(s["gs"], s["qmax"], s["data"], s["co"]), q, 3, 20 * 512, 512,
+                             3e-3)
+    assert st.dirs_used == used
+    assert np.max(np.abs(mine - ref) / ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_vegas_matches_oracle(dplus, oracle, small, prec):
+    say(f"[case] VEGAS precision={prec}")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    q = np.linspace(0.02, s["qmax"], 45)
+    mine, st = dplus.orientation_average(dplus.Docking.single(g), q, method=dplus.OA_VEGAS, seed=4,
+                                         vegas_iterations=4, vegas_points_per_bin=2)
+    ref, _ = oracle.vegas_oa(oracle.Docked.single(s["gs"], s["qmax"], s["data"], s["co"]), q, 4, 4, 2)
+    assert np.max(np.abs(mine - ref) / ref) < (FP64_TOL if prec == 0 else MIXED_TOL)
+
+
+def test_gk_matches_oracle(dplus, oracle, small):
+    say("[case] Gauss-Kronrod")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], 0, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    q = np.array([0.0, 0.3, 1.1, 2.5, 4.0])
+    mine, st = dplus.orientation_average(dplus.Docking.single(g), q, method=dplus.OA_GK, eps=1e-4)
+    ref, ev = oracle.gk_oa(oracle.Docked.single(s["gs"], s["qmax"], s["data"], s["co"]), q, 1e-4)
+    assert st.evaluations == ev
+    assert np.max(np.abs(mine - ref) / ref) < FP64_TOL
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hybrid_docking_mc_matches_oracle(dplus, oracle, small, prec):
+    say(f"[case] hybrid docking MC precision={prec}")
+    import dplus_mi355x.synth as S
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    rows = S.helix_rows(12, 3.0, 0.9, 27.7)
+    rot, nt, tr = S.rows_to_docking(rows)
+    q = np.linspace(0.05, 4.0, 50)
+    mine, _ = dplus.orientation_average(dplus.Docking([(g, rot, nt, tr)]), q, seed=2, max_dirs=2048, chunk=1024)
+    ref, _ = oracle.mc_oa(oracle.Docked([(s["gs"], s["qmax"], s["data"], s["co"], rot, nt, tr)]), q, 2, 2048, 1024)
+    assert np.max(np.abs(mine - ref) / ref) < (FP64_TOL if prec == 0 else MIXED_TOL)
+
+
+def test_dock_grid_matches_oracle(dplus, oracle, small):
+    say("[case] grid-level manual symmetry")
+    import dplus_mi355x.synth as S
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], 0, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    rot, nt, tr = S.rows_to_docking(S.helix_rows(4, 1.2, 0.5, 40.0))
+    out = dplus.Grid(20, 4.0, 0)
+    dplus.dock_grid(out, dplus.Docking([(g, rot, nt, tr)]), 1.0)
+    out.download()
+    ref = oracle.dock_grid(20, 4.0, oracle.Docked([(s["gs"], s["qmax"], s["data"], s["co"], rot, nt, tr)]), 1.0)
+    assert relmax(out.data, ref) < 1e-10
+
+
+def test_single_atom_fixture_on_gpu(dplus, oracle):
+    """The reference's golden single-atom curve (o_explicit_SS334) from the GPU path."""
+    import json, os
+    say("[case] golden o_explicit_SS334 on the GPU")
+    rec = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "o_explicit_SS334.json")))
+    import dplus_mi355x.elements as E
+    coef9, rad, ion, _ = E.ion_tables(["O"])
+    for prec in (0, 1):
+        g = dplus.Grid(rec["domain"]["GridSize"], rec["domain"]["qMax"], prec)
+        g.build_pdb([[0, 0, 0]], ion, coef9, rad, sol_ed=334.0,
+                    flags=dplus.CALC_ATOMIC_FORMFACTORS | dplus.CALC_DUMMY_SOLVENT)
+        I, _ = dplus.orientation_average(dplus.Docking.single(g), rec["q"], seed=1, max_dirs=256, chunk=256)
+        ref = np.array(rec["intensity"])
+        assert np.max(np.abs(I - ref)) <= 2e-5 * ref.max()
+
+
+def test_friedel_and_debye_at_larger_size(dplus, oracle):
+    """2,000 atoms (configuration 1 size), gs 100: the builder output is
+    Friedel-symmetric and the MC curve follows the exact Debye sum within the
+    grid's interpolation error."""
+    say("[case] 2000 atoms gs 100 Debye")
+    xyz, ion, coef9, rad = structure(2000, 1, radius=3.0)
+    g = gpu_grid(dplus, 100, 5.0, 1, xyz, ion, coef9, rad)
+    q = np.linspace(0.0, 5.0, 101)
+    I, _ = dplus.orientation_average(dplus.Docking.single(g), q, seed=1, max_dirs=1 << 16, chunk=8192)
+    deb = oracle.debye(xyz, ion, oracle_ions(oracle, coef9, rad, ion), q)
+    err = np.abs(I - deb) / deb
+    say(f"  max rel err vs Debye {err.max():.3e} median {np.median(err):.3e}")
+    assert np.median(err) < 0.02 and err.max() < 0.2
