@@ -58,10 +58,46 @@ __device__ __forceinline__ NodeInfo canonical_node(long long c, double step, dou
     return n;
 }
 
+// weight of a tile of solvent boxes of one shape at node q (nm^-1):
+// contrast * prod_a L_a sinc(q_a L_a / 2), L_a = n_a * vox
+__device__ __forceinline__ double box_weight64(double qx, double qy, double qz, int4 d, double vox) {
+    const double q3[3] = {qx, qy, qz};
+    const int n3[3] = {d.x, d.y, d.z};
+    double w = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const double L = (double)n3[a] * vox, x = 0.5 * q3[a] * L;
+        w *= (q3[a] == 0.0 || n3[a] == 0) ? L : sin(x) / x * L;
+    }
+    return w;
+}
+
+__device__ __forceinline__ float box_weight32(float qx, float qy, float qz, int4 d, float vox) {
+    const float q3[3] = {qx, qy, qz};
+    const int n3[3] = {d.x, d.y, d.z};
+    float w = 1.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float L = (float)n3[a] * vox, x = 0.5f * q3[a] * L;
+        // sin(x) on the hardware unit (argument in revolutions, reduced to
+        // [-1/2, 1/2] by round-to-nearest so tiny negative x stay tiny);
+        // below |x| = 1e-3 the series 1 - x^2/6 is exact in FP32
+        float r = x * (float)(1.0 / k2Pi);
+        r -= rintf(r);
+        const float s = __builtin_amdgcn_sinf(r);
+        w *= (fabsf(x) < 1e-3f || n3[a] == 0) ? L * (1.0f - x * x * (1.0f / 6.0f)) : s / x * L;
+    }
+    return w;
+}
+
+// Tiles: (first, count, index, kind).  kind 0: atoms of ion `index`, weighted by
+// the ion's form factor at the node's shell; kind 1: solvent boxes of shape
+// bdims[index], weighted by contrast * box_weight at the node.
 template <int K, bool kPhase64>
 __global__ __launch_bounds__(256) void build_atoms_mixed(
     const float4 *__restrict__ atoms, const double4 *__restrict__ atoms64,
     const int4 *__restrict__ tiles, int ntiles, const double *__restrict__ ff, int nion,
+    const int4 *__restrict__ bdims, double vox, double contrast,
     double step, long long nhalf, float4 *__restrict__ grid, double scale) {
     const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * K;
     float qx[K], qy[K], qz[K];
@@ -112,11 +148,22 @@ __global__ __launch_bounds__(256) void build_atoms_mixed(
                 }
             }
         }
+        if (tl.w == 0) {
 #pragma unroll
-        for (int p = 0; p < K; p++) {
-            double f = ff[(size_t)node[p].shell * nion + tl.z];
-            accR[p] = fma(f, (double)sr[p], accR[p]);
-            accI[p] = fma(f, (double)si[p], accI[p]);
+            for (int p = 0; p < K; p++) {
+                double f = ff[(size_t)node[p].shell * nion + tl.z];
+                accR[p] = fma(f, (double)sr[p], accR[p]);
+                accI[p] = fma(f, (double)si[p], accI[p]);
+            }
+        } else {
+            const int4 d = bdims[tl.z];
+            const float k2 = (float)k2Pi;
+#pragma unroll
+            for (int p = 0; p < K; p++) {
+                double f = contrast * (double)box_weight32(qx[p] * k2, qy[p] * k2, qz[p] * k2, d, (float)vox);
+                accR[p] = fma(f, (double)sr[p], accR[p]);
+                accI[p] = fma(f, (double)si[p], accI[p]);
+            }
         }
     }
 #pragma unroll
@@ -132,8 +179,8 @@ __global__ __launch_bounds__(256) void build_atoms_mixed(
 template <int K>
 __global__ __launch_bounds__(256) void build_atoms_fp64(
     const double4 *__restrict__ atoms, const int4 *__restrict__ tiles, int ntiles,
-    const double *__restrict__ ff, int nion, double step, long long nhalf,
-    double4 *__restrict__ grid, double scale) {
+    const double *__restrict__ ff, int nion, const int4 *__restrict__ bdims, double vox,
+    double contrast, double step, long long nhalf, double4 *__restrict__ grid, double scale) {
     const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * K;
     double qx[K], qy[K], qz[K], accR[K], accI[K];
     NodeInfo node[K];
@@ -163,7 +210,8 @@ __global__ __launch_bounds__(256) void build_atoms_fp64(
         }
 #pragma unroll
         for (int p = 0; p < K; p++) {
-            double f = ff[(size_t)node[p].shell * nion + tl.z];
+            double f = (tl.w == 0) ? ff[(size_t)node[p].shell * nion + tl.z]
+                                   : contrast * box_weight64(qx[p], qy[p], qz[p], bdims[tl.z], vox);
             accR[p] = fma(f, sr[p], accR[p]);
             accI[p] = fma(f, si[p], accI[p]);
         }
@@ -178,65 +226,14 @@ __global__ __launch_bounds__(256) void build_atoms_fp64(
     }
 }
 
-// Solvent boxes (hydration layer or excluded volume), added onto the grid:
-// contrast * prod_a [sinc(q_a n_a D/2) n_a D] * exp(i q.c)
-// (PDBAmplitude.cpp:277-314; PDBAmplitudeCUDA.cu:15-91).  Boxes arrive sorted
-// by their x extent so the x factor is reused (PDBAmplitude.cpp:157-188).
-__device__ __forceinline__ void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
-__device__ __forceinline__ void sincos_t(double x, double *s, double *c) { sincos(x, s, c); }
-
-template <typename T4, typename T>
-__global__ __launch_bounds__(256) void add_boxes(const int4 *__restrict__ dims,
-                                                 const float4 *__restrict__ com, int nbox,
-                                                 double vox, double contrast, double step,
-                                                 long long nhalf, T4 *__restrict__ grid,
-                                                 double scale) {
-    long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nhalf) return;
-    double dx, dy, dz;
-    NodeInfo n = canonical_node(c, step, dx, dy, dz);
-    const T qx = (T)dx, qy = (T)dy, qz = (T)dz;
-    const T hv = (T)(0.5 * vox);
-    T sr = 0, si = 0, vax = 0;
-    int prev = -1;
-    for (int b = 0; b < nbox; b++) {
-        const int4 d = dims[b];
-        const float4 cm = com[b];
-        if (d.x != prev) {
-            prev = d.x;
-            T len = (T)d.x * (T)vox, x = qx * (T)d.x * hv;
-            vax = (qx == (T)0 || d.x == 0) ? len : sin(x) / x * len;
-        }
-        T va = vax;
-        {
-            T len = (T)d.y * (T)vox, x = qy * (T)d.y * hv;
-            va *= (qy == (T)0 || d.y == 0) ? len : sin(x) / x * len;
-        }
-        {
-            T len = (T)d.z * (T)vox, x = qz * (T)d.z * hv;
-            va *= (qz == (T)0 || d.z == 0) ? len : sin(x) / x * len;
-        }
-        T ph = qx * (T)cm.x + qy * (T)cm.y + qz * (T)cm.z;
-        T s, co;
-        sincos_t(ph, &s, &co);
-        sr += va * co;
-        si += va * s;
-    }
-    const T f = (T)(contrast * scale);
-    T4 g = grid[n.idx];
-    g.x += sr * f;
-    g.y += si * f;
-    grid[n.idx] = g;
-    T4 a = grid[n.anti];
-    a.x += sr * f;
-    a.y -= si * f;
-    grid[n.anti] = a;
-}
-
 template <typename T4>
 __global__ void set_origin(T4 *grid, double re) {
-    T4 g = grid[0];
-    g.x += re;
+    // A(0) = sum of f~(0) (+ box volumes); the spline slots of the origin are 0
+    T4 g;
+    g.x = re;
+    g.y = 0;
+    g.z = 0;
+    g.w = 0;
     grid[0] = g;
 }
 
@@ -272,34 +269,23 @@ static inline unsigned blocks_for(long long n, int per_block) {
 constexpr int kNodesPerThread = 4;
 
 void launch_build_atoms(bool fp64, bool phase64, const void *atoms, const void *atoms64,
-                        const int4 *tiles, int ntiles, const double *ff, int nion, double step,
-                        long long nhalf, void *grid, double scale, hipStream_t s) {
+                        const int4 *tiles, int ntiles, const double *ff, int nion, const int4 *bdims,
+                        double vox, double contrast, double step, long long nhalf, void *grid,
+                        double scale, hipStream_t s) {
     const int K = kNodesPerThread;
     unsigned nb = blocks_for(nhalf, 256 * K);
     if (fp64) {
-        build_atoms_fp64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, step,
-                                               nhalf, (double4 *)grid, scale);
+        build_atoms_fp64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, bdims,
+                                               vox, contrast, step, nhalf, (double4 *)grid, scale);
     } else if (phase64) {
         build_atoms_mixed<K, true><<<nb, 256, 0, s>>>((const float4 *)atoms, (const double4 *)atoms64,
-                                                      tiles, ntiles, ff, nion, step, nhalf,
-                                                      (float4 *)grid, scale);
+                                                      tiles, ntiles, ff, nion, bdims, vox, contrast,
+                                                      step, nhalf, (float4 *)grid, scale);
     } else {
         build_atoms_mixed<K, false><<<nb, 256, 0, s>>>((const float4 *)atoms, (const double4 *)atoms64,
-                                                       tiles, ntiles, ff, nion, step, nhalf,
-                                                       (float4 *)grid, scale);
+                                                       tiles, ntiles, ff, nion, bdims, vox, contrast,
+                                                       step, nhalf, (float4 *)grid, scale);
     }
-}
-
-void launch_add_boxes(bool fp64, const int4 *dims, const float4 *com, int nbox, double vox,
-                      double contrast, double step, long long nhalf, void *grid, double scale,
-                      hipStream_t s) {
-    unsigned nb = blocks_for(nhalf, 256);
-    if (fp64)
-        add_boxes<double4, double><<<nb, 256, 0, s>>>(dims, com, nbox, vox, contrast, step, nhalf,
-                                                       (double4 *)grid, scale);
-    else
-        add_boxes<float4, float><<<nb, 256, 0, s>>>(dims, com, nbox, vox, contrast, step, nhalf,
-                                                     (float4 *)grid, scale);
 }
 
 void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s) {

@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <random>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -52,6 +53,64 @@ int set_error(int code, const std::string &msg) {
 
 void check_hip(hipError_t e, const char *what) {
     if (e != hipSuccess) throw Error(kErrGeneral, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ------------------------------------------------------------------ kernel timing
+namespace {
+struct TimingState {
+    std::mutex mu;
+    bool on = false;
+    std::map<std::string, std::pair<long long, double>> acc;
+    std::vector<std::tuple<std::string, hipEvent_t, hipEvent_t>> pending;
+};
+TimingState &timing() {
+    static TimingState t;
+    return t;
+}
+}  // namespace
+
+KTimer::KTimer(const char *n, hipStream_t st) : name(n), s(st) {
+    if (!timing().on) return;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        e0 = e1 = nullptr;
+        return;
+    }
+    (void)hipEventRecord(e0, s);
+}
+
+KTimer::~KTimer() {
+    if (!e0) return;
+    (void)hipEventRecord(e1, s);
+    std::lock_guard<std::mutex> lk(timing().mu);
+    timing().pending.emplace_back(name, e0, e1);
+}
+
+void timing_collect() {
+    std::lock_guard<std::mutex> lk(timing().mu);
+    for (auto &p : timing().pending) {
+        float ms = 0.f;
+        (void)hipEventSynchronize(std::get<2>(p));
+        if (hipEventElapsedTime(&ms, std::get<1>(p), std::get<2>(p)) == hipSuccess) {
+            auto &a = timing().acc[std::get<0>(p)];
+            a.first++;
+            a.second += ms;
+        }
+        (void)hipEventDestroy(std::get<1>(p));
+        (void)hipEventDestroy(std::get<2>(p));
+    }
+    timing().pending.clear();
+}
+
+void timing_set(bool on) { timing().on = on; }
+void timing_clear() {
+    std::lock_guard<std::mutex> lk(timing().mu);
+    timing().acc.clear();
+}
+void timing_read(const char *name, long long *n, double *ms) {
+    std::lock_guard<std::mutex> lk(timing().mu);
+    auto it = timing().acc.find(name);
+    if (n) *n = it == timing().acc.end() ? 0 : it->second.first;
+    if (ms) *ms = it == timing().acc.end() ? 0.0 : it->second.second;
 }
 
 // ------------------------------------------------------------------ device
@@ -159,6 +218,7 @@ void launch_unpack(bool fp64, const double2 *data, long long n, void *grid, hipS
 
 void Grid::splines(hipStream_t s) {
     SplineTables *t = tables_for(N, s);
+    KTimer kt("splines", s);
     launch_splines(fp64(), dev.p, N, (const double *)t->tab.p, (const long long *)t->off.p, s);
     check_hip(hipGetLastError(), "splines launch");
 }
@@ -185,10 +245,8 @@ void Grid::upload(hipStream_t s) {
 
 // ------------------------------------------------------------------ PDB builder
 void launch_build_atoms(bool fp64, bool phase64, const void *atoms, const void *atoms64, const int4 *tiles,
-                        int ntiles, const double *ff, int nion, double step, long long nhalf, void *grid,
-                        double scale, hipStream_t s);
-void launch_add_boxes(bool fp64, const int4 *dims, const float4 *com, int nbox, double vox, double contrast,
-                      double step, long long nhalf, void *grid, double scale, hipStream_t s);
+                        int ntiles, const double *ff, int nion, const int4 *bdims, double vox, double contrast,
+                        double step, long long nhalf, void *grid, double scale, hipStream_t s);
 void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s);
 
 // Atomic Form Factor.cpp:11-27 (cube-mean radius), 91-115 (dummy-atom
@@ -229,83 +287,128 @@ void form_factor_table(const dplus_pdb_desc *d, const std::vector<int> &counts, 
     }
 }
 
-void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s) {
+Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
     if (d->natoms < 0 || d->nion <= 0 || d->nion > 255) throw Error(kErrInvalidArgs, "bad atom/ion counts");
+    if (d->nbox < 0 || (d->nbox > 0 && (!d->box_dims || !d->box_com || !(d->vox_step > 0))))
+        throw Error(kErrInvalidArgs, "bad solvent boxes");
     for (int a = 0; a < d->natoms; a++)
         if (d->ion[a] < 0 || d->ion[a] >= d->nion) throw Error(kErrInvalidArgs, "ion index out of range");
-    // sort atoms by ion (ExtractRelevantCoeffs, PDBReaderLib.cpp:1443-1509) and tile them
+    const bool atoms_on = (d->flags & (DPLUS_CALC_ATOMIC_FORMFACTORS | DPLUS_CALC_DUMMY_SOLVENT)) && d->natoms > 0;
+    const bool boxes_on = d->nbox > 0 && d->box_contrast != 0.0;
+    // atoms sorted by ion (ExtractRelevantCoeffs, PDBReaderLib.cpp:1443-1509), then
+    // boxes sorted by shape (SortSolventBoxEntry, PDBAmplitude.cpp:161-188)
     std::vector<int> order(d->natoms);
     for (int a = 0; a < d->natoms; a++) order[a] = a;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d->ion[x] < d->ion[y]; });
-    std::vector<float4> at32(std::max(1, d->natoms));
-    std::vector<double4> at64(std::max(1, d->natoms));
+    std::vector<int> border(boxes_on ? d->nbox : 0);
+    for (size_t b = 0; b < border.size(); b++) border[b] = (int)b;
+    auto bkey = [&](int b) {
+        return std::make_tuple(d->box_dims[3 * b], d->box_dims[3 * b + 1], d->box_dims[3 * b + 2], d->box_com[3 * b],
+                               d->box_com[3 * b + 1], d->box_com[3 * b + 2]);
+    };
+    std::sort(border.begin(), border.end(), [&](int x, int y) { return bkey(x) < bkey(y); });
+    const size_t nsrc = (atoms_on ? d->natoms : 0) + border.size();
+    std::vector<float4> at32(std::max<size_t>(1, nsrc));
+    std::vector<double4> at64(std::max<size_t>(1, nsrc));
     std::vector<int> counts(d->nion, 0);
-    for (int k = 0; k < d->natoms; k++) {
-        const int a = order[k];
-        const double *p = d->xyz + 3 * a;
-        at32[k] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.f);
-        at64[k] = make_double4(p[0], p[1], p[2], 0.0);
-        counts[d->ion[a]]++;
-    }
-    std::vector<int4> tiles;
-    {
-        int k = 0;
-        while (k < d->natoms) {
-            const int ion = d->ion[order[k]];
-            int e = k;
-            while (e < d->natoms && d->ion[order[e]] == ion && e - k < 256) e++;
-            tiles.push_back(make_int4(k, e - k, ion, 0));
-            k = e;
+    for (int a = 0; a < d->natoms; a++) counts[d->ion[a]]++;
+    std::vector<int4> tiles, bdims;
+    size_t k = 0;
+    if (atoms_on) {
+        for (int i = 0; i < d->natoms; i++) {
+            const double *p = d->xyz + 3 * order[i];
+            at32[k] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.f);
+            at64[k] = make_double4(p[0], p[1], p[2], 0.0);
+            k++;
+        }
+        int i = 0;
+        while (i < d->natoms) {
+            const int ion = d->ion[order[i]];
+            int e = i;
+            while (e < d->natoms && d->ion[order[e]] == ion && e - i < 256) e++;
+            tiles.push_back(make_int4(i, e - i, ion, 0));
+            i = e;
         }
     }
-    std::vector<double> ff;
-    form_factor_table(d, counts, g.N, g.step, ff);
-    // origin: sum f~(0) over atoms + contrast * total box volume
-    double origin = 0.0;
-    if (d->flags & (DPLUS_CALC_ATOMIC_FORMFACTORS | DPLUS_CALC_DUMMY_SOLVENT))
-        for (int i = 0; i < d->nion; i++) origin += counts[i] * ff[i];
-    DevBuf b32, b64, bt, bff, bdims, bcom;
+    double box_volume = 0.0;
+    if (boxes_on) {
+        const size_t base = k;
+        for (int b : border) {
+            const double *p = d->box_com + 3 * b;
+            at32[k] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.f);
+            at64[k] = make_double4(p[0], p[1], p[2], 0.0);
+            box_volume += (double)d->box_dims[3 * b] * d->box_dims[3 * b + 1] * d->box_dims[3 * b + 2];
+            k++;
+        }
+        box_volume *= d->vox_step * d->vox_step * d->vox_step;
+        size_t i = 0;
+        while (i < border.size()) {
+            const int *di = d->box_dims + 3 * border[i];
+            bdims.push_back(make_int4(di[0], di[1], di[2], 0));
+            size_t e = i;
+            while (e < border.size() && e - i < 256) {
+                const int *de = d->box_dims + 3 * border[e];
+                if (de[0] != di[0] || de[1] != di[1] || de[2] != di[2]) break;
+                e++;
+            }
+            tiles.push_back(make_int4((int)(base + i), (int)(e - i), (int)bdims.size() - 1, 1));
+            i = e;
+        }
+    }
+    if (bdims.empty()) bdims.push_back(make_int4(0, 0, 0, 0));
+    desc = *d;
+    desc.xyz = nullptr;
+    desc.ion = nullptr;
+    desc.box_dims = nullptr;
+    desc.box_com = nullptr;
+    coef9.assign(d->coef9, d->coef9 + 9 * d->nion);
+    rad.assign(d->rad, d->rad + d->nion);
+    desc.coef9 = coef9.data();
+    desc.rad = rad.data();
+    ion_counts = counts;
+    this->box_volume = box_volume;
+    this->atoms_on = atoms_on;
+    this->boxes_on = boxes_on;
+    ntiles = (int)tiles.size();
+    nsources = (long long)nsrc;
     b32.upload(at32.data(), at32.size() * sizeof(float4), s);
     b64.upload(at64.data(), at64.size() * sizeof(double4), s);
     bt.upload(tiles.data(), std::max<size_t>(1, tiles.size()) * sizeof(int4), s);
+    bd.upload(bdims.data(), bdims.size() * sizeof(int4), s);
+    check_hip(hipStreamSynchronize(s), "structure upload");
+}
+
+void Structure::build(Grid &g, bool host_copy, hipStream_t s) {
+    std::vector<double> ff;
+    form_factor_table(&desc, ion_counts, g.N, g.step, ff);
+    // origin: sum of f~(0) over atoms + contrast * total box volume
+    double origin = 0.0;
+    if (atoms_on)
+        for (int i = 0; i < desc.nion; i++) origin += ion_counts[i] * ff[i];
+    if (boxes_on) origin += desc.box_contrast * box_volume;
     bff.upload(ff.data(), ff.size() * sizeof(double), s);
     const long long nhalf = half_base(g.N + 1);
-    const bool atoms_on = (d->flags & (DPLUS_CALC_ATOMIC_FORMFACTORS | DPLUS_CALC_DUMMY_SOLVENT)) && d->natoms > 0;
     const bool phase64 = getenv("DPLUS_PHASE64") != nullptr;
-    launch_build_atoms(g.fp64(), phase64, b32.p, b64.p, (const int4 *)bt.p, atoms_on ? (int)tiles.size() : 0,
-                       (const double *)bff.p, d->nion, g.step, nhalf, g.dev.p, d->scale, s);
-    check_hip(hipGetLastError(), "build_atoms launch");
-    if (d->nbox > 0 && d->box_contrast != 0.0) {
-        // sort boxes by their extents (SortSolventBoxEntry, PDBAmplitude.cpp:161-188)
-        std::vector<int> bo(d->nbox);
-        for (int b = 0; b < d->nbox; b++) bo[b] = b;
-        auto key = [&](int b) {
-            return std::make_tuple(d->box_dims[3 * b], d->box_dims[3 * b + 1], d->box_dims[3 * b + 2],
-                                   d->box_com[3 * b], d->box_com[3 * b + 1], d->box_com[3 * b + 2]);
-        };
-        std::sort(bo.begin(), bo.end(), [&](int x, int y) { return key(x) < key(y); });
-        std::vector<int4> dims(d->nbox);
-        std::vector<float4> com(d->nbox);
-        double vol = 0.0;
-        for (int k = 0; k < d->nbox; k++) {
-            const int b = bo[k];
-            dims[k] = make_int4(d->box_dims[3 * b], d->box_dims[3 * b + 1], d->box_dims[3 * b + 2], 0);
-            com[k] = make_float4((float)d->box_com[3 * b], (float)d->box_com[3 * b + 1], (float)d->box_com[3 * b + 2], 0.f);
-            vol += (double)dims[k].x * dims[k].y * dims[k].z * d->vox_step * d->vox_step * d->vox_step;
-        }
-        origin += d->box_contrast * vol;
-        bdims.upload(dims.data(), dims.size() * sizeof(int4), s);
-        bcom.upload(com.data(), com.size() * sizeof(float4), s);
-        launch_add_boxes(g.fp64(), (const int4 *)bdims.p, (const float4 *)bcom.p, d->nbox, d->vox_step,
-                         d->box_contrast, g.step, nhalf, g.dev.p, d->scale, s);
-        check_hip(hipGetLastError(), "add_boxes launch");
+    {
+        KTimer kt("build", s);
+        launch_build_atoms(g.fp64(), phase64, b32.p, b64.p, (const int4 *)bt.p, ntiles, (const double *)bff.p,
+                           desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast, g.step, nhalf, g.dev.p,
+                           desc.scale, s);
+        check_hip(hipGetLastError(), "build_atoms launch");
     }
-    launch_set_origin(g.fp64(), g.dev.p, origin * d->scale, s);
+    launch_set_origin(g.fp64(), g.dev.p, origin * desc.scale, s);
     g.splines(s);
+    last_terms = (double)nhalf * (double)nsources;
     if (host_copy)
         g.download(s);
     else
         check_hip(hipStreamSynchronize(s), "pdb build");
+    timing_collect();
+}
+
+void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s) {
+    Structure st(d, s);
+    st.build(g, host_copy, s);
 }
 
 // ------------------------------------------------------------------ orientation averages
@@ -423,9 +526,12 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
             const int nblk = (int)nblk_ll;
             bpart.ensure((size_t)ng * nblk * mstride * sizeof(double));
             bout.ensure((size_t)ng * mstride * sizeof(double));
-            launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
-                      (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
-                      (const double *)bq.p, D.step, p->seed, 0, nblk, (double *)bpart.p, (double *)bout.p, s);
+            {
+                KTimer kt("mc", s);
+                launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                          (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
+                          (const double *)bq.p, D.step, p->seed, 0, nblk, (double *)bpart.p, (double *)bout.p, s);
+            }
             check_hip(hipGetLastError(), "mc launch");
             std::vector<double> r((size_t)ng * mstride);
             check_hip(hipMemcpyAsync(r.data(), bout.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
@@ -444,10 +550,13 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
             std::vector<double> r((size_t)ng * mstride);
             for (long long k = 0; k < nchunks && !active.empty(); k++) {
                 bact.upload(active.data(), active.size() * sizeof(int), s);
-                launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
-                          (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, (int)active.size(),
-                          mstride, (const double *)bq.p, D.step, p->seed, k * chunk, nblk, (double *)bpart.p,
-                          (double *)bout.p, s);
+                {
+                    KTimer kt("mc", s);
+                    launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                              (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
+                              (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k * chunk, nblk,
+                              (double *)bpart.p, (double *)bout.p, s);
+                }
                 check_hip(hipGetLastError(), "mc launch");
                 check_hip(hipMemcpyAsync(r.data(), bout.p, active.size() * mstride * sizeof(double),
                                          hipMemcpyDeviceToHost, s), "D2H");
@@ -493,10 +602,13 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
         int used = 0;
         for (int k = 0; k < iters && !active.empty(); k++) {
             if (k > 0) bact.upload(active.data(), active.size() * sizeof(int), s);
-            launch_vegas_iter(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
-                              (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
-                              (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k, ppb,
-                              (double *)bb.p, (double *)bI.p, (double *)bh.p, nq, s);
+            {
+                KTimer kt("vegas", s);
+                launch_vegas_iter(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                                  (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
+                                  (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k, ppb,
+                                  (double *)bb.p, (double *)bI.p, (double *)bh.p, nq, s);
+            }
             check_hip(hipGetLastError(), "vegas launch");
             used = k + 1;
             if (p->eps > 0.0) {
@@ -762,6 +874,7 @@ int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const
         DockSet D(nleaf, leaves, s);
         if (st) memset(st, 0, sizeof *st);
         orientation_average(D, nq, q, p, intensity, st, s);
+        timing_collect();
         if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     })
 }
@@ -775,6 +888,55 @@ int dplus_dock_grid(dplus_grid *outh, int nleaf, const dplus_leaf *leaves, doubl
         DockSet D(nleaf, leaves, s);
         if (D.fp64 != out->fp64()) throw Error(kErrInvalidArgs, "output precision differs from the leaves'");
         dock_grid(D, *out, scale, s);
+    })
+}
+
+int dplus_structure_create(const dplus_pdb_desc *d, dplus_structure **out) {
+    API_GUARD({
+        if (!d || !out) throw Error(kErrInvalidArgs, "null argument");
+        Device::get().require();
+        *out = (dplus_structure *)new Structure(d, Device::get().stream);
+    })
+}
+
+void dplus_structure_destroy(dplus_structure *s) { delete (Structure *)s; }
+
+int dplus_structure_build_grid(dplus_structure *sh, dplus_grid *gh, int host_copy) {
+    API_GUARD({
+        Structure *st = (Structure *)sh;
+        Grid *g = (Grid *)gh;
+        if (!st || !g) throw Error(kErrInvalidArgs, "null argument");
+        Device::get().require();
+        st->build(*g, host_copy != 0, Device::get().stream);
+    })
+}
+
+int dplus_timing_enable(int on) {
+    API_GUARD({
+        timing_collect();
+        timing_set(on != 0);
+    })
+}
+
+int dplus_timing_get(const char *kernel, long long *launches, double *ms) {
+    API_GUARD({
+        if (!kernel) throw Error(kErrInvalidArgs, "null kernel name");
+        timing_collect();
+        timing_read(kernel, launches, ms);
+    })
+}
+
+int dplus_timing_reset(void) {
+    API_GUARD({
+        timing_collect();
+        timing_clear();
+    })
+}
+
+int dplus_microbench_valu(double *fma32, double *sin32, double *cos32, double *fma64) {
+    API_GUARD({
+        Device::get().require();
+        microbench_valu(Device::get().stream, fma32, sin32, cos32, fma64);
     })
 }
 

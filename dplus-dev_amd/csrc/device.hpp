@@ -5,7 +5,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <stdexcept>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -89,6 +92,51 @@ struct DockSet {
     bool fp64 = false;
     DevBuf grids, geos, rotb, trb;
     DockSet(int nleaf, const dplus_leaf *leaves, hipStream_t s);
+};
+
+// run fn(i) for i in [0, n) on the host's cores (dynamic chunks)
+template <typename F>
+void parallel_for(long long n, F fn) {
+    unsigned nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    if (n < 2 || nt == 1) {
+        for (long long i = 0; i < n; i++) fn(i);
+        return;
+    }
+    std::atomic<long long> next(0);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; t++)
+        th.emplace_back([&]() {
+            for (long long i; (i = next.fetch_add(1)) < n;) fn(i);
+        });
+    for (auto &x : th) x.join();
+}
+
+// HIP-event timing of one kernel family on a stream (active when enabled)
+struct KTimer {
+    const char *name;
+    hipStream_t s;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    KTimer(const char *n, hipStream_t st);
+    ~KTimer();
+};
+void timing_collect();
+void timing_set(bool on);
+void timing_clear();
+void timing_read(const char *name, long long *n, double *ms);
+void microbench_valu(hipStream_t s, double *fma32, double *sin32, double *cos32, double *fma64);
+
+// atoms and solvent boxes of one PDB leaf, sorted, tiled and resident in HBM
+struct Structure {
+    dplus_pdb_desc desc{};
+    std::vector<double> coef9, rad;
+    std::vector<int> ion_counts;
+    double box_volume = 0.0, last_terms = 0.0;
+    bool atoms_on = false, boxes_on = false;
+    int ntiles = 0;
+    long long nsources = 0;
+    DevBuf b32, b64, bt, bd, bff;
+    Structure(const dplus_pdb_desc *d, hipStream_t s);
+    void build(Grid &g, bool host_copy, hipStream_t s);
 };
 
 std::vector<GroupDev> make_groups(const double *q, int nq, double step);

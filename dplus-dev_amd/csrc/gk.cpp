@@ -124,6 +124,7 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
     auto evaluate = [&](size_t npts) {
         bq.upload(pts.data(), npts * 3 * sizeof(double), s);
         bo.ensure(npts * sizeof(double));
+        KTimer kt("gk", s);
         launch_docked_cart(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
                            (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (int)npts,
                            (const double *)bq.p, (double *)bo.p, s);

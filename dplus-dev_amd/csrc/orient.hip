@@ -644,6 +644,7 @@ __global__ void dock_grid_kernel(const T4 *const *__restrict__ grids, const Leaf
 
 void dock_grid(const DockSet &D, Grid &out, double scale, hipStream_t s) {
     const unsigned nb = (unsigned)((out.P + 255) / 256);
+    KTimer kt("dock", s);
     if (D.fp64)
         dock_grid_kernel<double, double4><<<nb, 256, 0, s>>>((const double4 *const *)D.grids.p,
                                                               (const LeafGeo *)D.geos.p, (const RotDev *)D.rotb.p,

@@ -82,6 +82,15 @@ SIGNATURES = {
     "dplus_dock_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_double]),
     "dplus_synth_walk": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
+    "dplus_hydration_boxes": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, _I, _I, _D]),
+    "dplus_structure_create": (ctypes.c_int, [ctypes.POINTER(PdbDesc), ctypes.POINTER(ctypes.c_void_p)]),
+    "dplus_structure_destroy": (None, [ctypes.c_void_p]),
+    "dplus_structure_build_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "dplus_timing_enable": (ctypes.c_int, [ctypes.c_int]),
+    "dplus_timing_get": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong), _D]),
+    "dplus_timing_reset": (ctypes.c_int, []),
+    "dplus_microbench_valu": (ctypes.c_int, [_D, _D, _D, _D]),
 }
 
 _lib = None
@@ -133,3 +142,25 @@ def device_info():
     buf = ctypes.create_string_buffer(512)
     check(lib().dplus_device_info(buf, 512))
     return json.loads(buf.value.decode())
+
+
+def timing_enable(on=True):
+    check(lib().dplus_timing_enable(1 if on else 0))
+
+
+def timing_reset():
+    check(lib().dplus_timing_reset())
+
+
+def timing_get(kernel):
+    """(launches, total ms) of a kernel family measured with HIP events on the library stream."""
+    n, ms = ctypes.c_longlong(), ctypes.c_double()
+    check(lib().dplus_timing_get(kernel.encode(), ctypes.byref(n), ctypes.byref(ms)))
+    return n.value, ms.value
+
+
+def microbench_valu():
+    """Device-wide ops/s of v_fma_f32, v_sin_f32, v_cos_f32, v_fma_f64."""
+    r = [ctypes.c_double() for _ in range(4)]
+    check(lib().dplus_microbench_valu(*[ctypes.byref(x) for x in r]))
+    return dict(zip(("fma32", "sin32", "cos32", "fma64"), (x.value for x in r)))

@@ -84,6 +84,43 @@ class Grid:
             pass
 
 
+def _pdb_desc(xyz, ion, coef9, rad, sol_ed, c1, flags, scale, box_dims, box_com, vox_step, box_contrast, keep):
+    xyz, ion = L.f64(xyz).reshape(-1, 3), L.i32(ion)
+    coef9, rad = L.f64(coef9).reshape(-1, 9), L.f64(rad)
+    if box_dims is None or len(box_dims) == 0:
+        bd, bc, nb = np.zeros((1, 3), np.int32), np.zeros((1, 3)), 0
+    else:
+        bd, bc, nb = L.i32(box_dims).reshape(-1, 3), L.f64(box_com).reshape(-1, 3), len(box_dims)
+    keep += [xyz, ion, coef9, rad, bd, bc]
+    return L.PdbDesc(len(ion), L.dp(xyz), L.ip(ion), len(coef9), L.dp(coef9), L.dp(rad), sol_ed, c1, flags, scale,
+                     nb, L.ip(bd), L.dp(bc), vox_step, box_contrast)
+
+
+class Structure:
+    """Atoms and solvent boxes of a PDB leaf resident in HBM (sorted and tiled once)."""
+
+    def __init__(self, xyz, ion, coef9, rad, sol_ed=0.0, c1=1.0, flags=L.CALC_ATOMIC_FORMFACTORS, scale=1.0,
+                 box_dims=None, box_com=None, vox_step=0.0, box_contrast=0.0):
+        keep = []
+        d = _pdb_desc(xyz, ion, coef9, rad, sol_ed, c1, flags, scale, box_dims, box_com, vox_step, box_contrast, keep)
+        h = ctypes.c_void_p()
+        L.check(L.lib().dplus_structure_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.natoms = len(L.i32(ion))
+        self.nboxes = 0 if box_dims is None else len(box_dims)
+
+    def build(self, grid, host_copy=False):
+        L.check(L.lib().dplus_structure_build_grid(self._h, grid.handle, 1 if host_copy else 0))
+
+    def __del__(self):
+        try:
+            if self._h:
+                L.lib().dplus_structure_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
 def index_from_indices(qi, ti, pi):
     return int(L.lib().dplus_grid_index_from_indices(qi, ti, pi))
 
@@ -125,6 +162,18 @@ def orientation_average(dock, q, method=L.OA_MC, seed=1, max_dirs=1 << 20, chunk
 
 def dock_grid(out, dock, scale=1.0):
     L.check(L.lib().dplus_dock_grid(out.handle, dock.n, dock.arr, float(scale)))
+
+
+def hydration_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05):
+    """Hydration-layer boxes (dims [n,3] voxels, centres [n,3] nm) around atoms of radius rad."""
+    xyz, rad = L.f64(xyz).reshape(-1, 3), L.f64(rad)
+    n = ctypes.c_int()
+    L.check(L.lib().dplus_hydration_boxes(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, 0,
+                                          ctypes.byref(n), None, None))
+    dims, com = np.zeros((max(1, n.value), 3), np.int32), np.zeros((max(1, n.value), 3))
+    L.check(L.lib().dplus_hydration_boxes(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, n.value,
+                                          ctypes.byref(n), L.ip(dims), L.dp(com)))
+    return dims[:n.value], com[:n.value]
 
 
 def synth_walk(seed, n, step=0.38, min_sep=0.12, axes=(3.0, 3.0, 3.0)):

@@ -104,6 +104,26 @@ typedef struct {
  * CalculateSplines; host_copy != 0 also fills the host mirror. */
 int dplus_pdb_build_grid(dplus_grid *g, const dplus_pdb_desc *d, int host_copy);
 
+/* Device-resident copy of a PDB leaf's atoms and solvent boxes, sorted and
+ * tiled once (the state PDBAmplitude::PreCalculate prepares,
+ * PDBAmplitude.cpp:1199-1336), so repeated grid builds read HBM only. */
+typedef struct dplus_structure dplus_structure;
+int dplus_structure_create(const dplus_pdb_desc *d, dplus_structure **out);
+void dplus_structure_destroy(dplus_structure *s);
+/* build the amplitude grid of a resident structure (+ CalculateSplines) */
+int dplus_structure_build_grid(dplus_structure *s, dplus_grid *g, int host_copy);
+
+/* ------------------------------------------------------------------ measurement */
+/* Per-kernel HIP-event timing on the library's stream: enable/disable, read the
+ * launch count and summed duration (ms) of a kernel family ("build", "splines",
+ * "mc", "vegas", "gk", "dock"), reset. */
+int dplus_timing_enable(int on);
+int dplus_timing_get(const char *kernel, long long *launches, double *ms);
+int dplus_timing_reset(void);
+/* Throughput microbenchmark of gfx950 VALU ops: ops/s of v_fma_f32, v_sin_f32,
+ * v_cos_f32 and v_fma_f64 over the whole device (for the transcendental weight). */
+int dplus_microbench_valu(double *fma32, double *sin32, double *cos32, double *fma64);
+
 /* ------------------------------------------------------------------ orientation average */
 /* A leaf of the hybrid regime (Amplitude.cpp:2028-2303): a grid seen under nrot
  * rotations rot[4r..4r+3] = (alpha, beta, gamma [rad], scale), rotation r placed
@@ -147,6 +167,15 @@ int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const
  * out(q) = scale * sum over leaves/rotations/translations of A_leaf(R^T q) e^{i q.t},
  * evaluated at every node of out, then CalculateSplines. */
 int dplus_dock_grid(dplus_grid *out, int nleaf, const dplus_leaf *leaves, double scale);
+
+/* ------------------------------------------------------------------ hydration layer */
+/* Voxelised hydration ("outer solvent") layer of PDBAmplitude::CalculateSolventSpace
+ * (Backend/Backend/VoxelizedSolvent.cpp:4-194, box reduction :263-363): voxels of
+ * bulk solvent within `thickness` of the probe shell (atom radius + probe), as
+ * boxes (dims in voxels, centres in nm).  Writes up to `capacity` boxes and sets
+ * *nbox to the number found; call again with capacity >= *nbox when it is larger. */
+int dplus_hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness,
+                          double vox, int capacity, int *nbox, int *dims, double *com);
 
 /* ------------------------------------------------------------------ synthetic inputs */
 /* Seeded self-avoiding random walk (step, minimum separation in nm) confined to

@@ -208,3 +208,53 @@ def test_friedel_and_debye_at_larger_size(dplus, oracle):
     err = np.abs(I - deb) / deb
     say(f"  max rel err vs Debye {err.max():.3e} median {np.median(err):.3e}")
     assert np.median(err) < 0.02 and err.max() < 0.2
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_hydration_boxes_match_oracle(dplus, oracle, prec):
+    """Atoms plus a hydration layer of solvent boxes (the builder's box tiles)
+    against the oracle's direct box sums, through the resident-structure API."""
+    say(f"[case] atoms + hydration boxes precision={prec}")
+    xyz, ion, coef9, rad = structure(400, 5, radius=1.2)
+    dims, com = dplus.hydration_boxes(xyz, rad[ion], probe=0.14, thickness=0.3, vox=0.1)
+    assert len(dims) > 10
+    gs, qmax, contrast = 24, 5.0, 33.3
+    st = dplus.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=3, box_dims=dims, box_com=com, vox_step=0.1,
+                         box_contrast=contrast)
+    g = dplus.Grid(gs, qmax, prec)
+    st.build(g, host_copy=True)
+    ions = oracle_ions(oracle, coef9, rad, ion, sol=333.0)
+    ref = oracle.grid_fill_pdb(gs, qmax, xyz, ion, ions, 1.0, dims, com, 0.1, contrast)
+    assert relmax(g.data, ref) < (1e-10 if prec == 0 else 2e-5)
+
+
+def test_kernel_timing_and_microbench(dplus):
+    say("[case] timing hooks and VALU microbenchmark")
+    mb = dplus.microbench_valu()
+    say("  ", mb)
+    assert all(v > 1e12 for v in mb.values())
+    xyz, ion, coef9, rad = structure(200, 2)
+    st = dplus.Structure(xyz, ion, coef9, rad)
+    g = dplus.Grid(30, 4.0, 1)
+    dplus.timing_reset()
+    dplus.timing_enable(True)
+    st.build(g)
+    st.build(g)
+    dplus.timing_enable(False)
+    n, ms = dplus.timing_get("build")
+    assert n == 2 and ms > 0
+
+
+def test_rebuild_is_idempotent(dplus):
+    """Building the same grid twice gives the same values (regression: the
+    origin node was accumulated across builds)."""
+    say("[case] rebuild idempotence")
+    xyz, ion, coef9, rad = structure(150, 4)
+    st = dplus.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=3)
+    for prec in (0, 1):
+        g = dplus.Grid(20, 4.0, prec)
+        st.build(g, host_copy=True)
+        first = g.data.copy()
+        st.build(g, host_copy=True)
+        st.build(g, host_copy=True)
+        assert np.array_equal(g.data, first)
