@@ -106,6 +106,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="one mixed builder launch plus a 256 MiB device copy (counter calibration), for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (see DESIGN.md for the "
@@ -122,6 +124,13 @@ def main():
     W = make_workload(D, S)
     st = resident(D, W)
     g32 = D.Grid(W["gs"], W["qmax"], D.PRECISION_MIXED)
+    if args.pmc_probe:
+        st.build(g32)
+        nbytes = 256 * 2 ** 20
+        ms = D.copy_probe(nbytes)
+        print(json.dumps({"probe": "build_atoms_mixed + copy_probe_kernel", "copy_bytes": nbytes,
+                          "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        return
     g64 = D.Grid(W["gs"], W["qmax"], D.PRECISION_FP64)
 
     mb = D.microbench_valu()
@@ -159,7 +168,6 @@ def main():
     log(f"[bench] fp64: {np.mean(lat64):.3f} s/curve; max rel I(q) mixed vs fp64 = {err:.2e}")
 
     # ---- roofline of the dominant kernel (the amplitude builder, VALU-bound)
-    from oracle import total_points  # noqa: F401  (geometry only)
     N = W["gs"] // 2 + 3
     nhalf = 3 * (N + 1) ** 2 * N
     sources = len(W["xyz"]) + len(W["dims"])
@@ -167,6 +175,12 @@ def main():
     flop_eq = 8.0 + 2.0 * w_t
     t_launch = ms_build / max(nb, 1) * 1e-3
     achieved = terms * flop_eq / t_launch / 1e12
+    # HBM bytes per builder launch from the two rocprofv3 --pmc passes of this workload
+    # (FETCH_SIZE, WRITE_SIZE, each scaled by a known-size copy), kept in profiles/
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", "build_atoms_mixed_c2_pmc.json")
+    if os.path.exists(pmc):
+        traffic = json.load(open(pmc)).get("traffic_bytes")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -199,7 +213,7 @@ def main():
         "speedup_vs_fp64": value / fp64_value,
         "max_rel_err_vs_fp64": err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "build_atoms_mixed", "launch_ms": t_launch * 1e3,
                      "terms_per_launch": terms, "flop_eq_per_term": flop_eq,
                      "transcendental_weight": w_t},

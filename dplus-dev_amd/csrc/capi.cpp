@@ -418,8 +418,8 @@ void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot
                hipStream_t s);
 void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
                        const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
-                       double step, uint64_t seed, int iter, int ppb, double *bounds, double *I, double *hist,
-                       int nq, hipStream_t s);
+                       double step, uint64_t seed, int iter, int ppb, double *bounds, double *binsum,
+                       double *binvar, double *hist, int nq, hipStream_t s);
 void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
                         const double4 *trans, int n, const double *qxyz, double *out, hipStream_t s);
 
@@ -585,16 +585,17 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
     } else if (p->method == DPLUS_OA_VEGAS) {
         const int iters = std::max(1, p->vegas_iterations);
         const int ppb = std::max(1, p->vegas_points_per_bin);
-        const int npts = 1024 * ppb;
+        if (ppb > 256) throw Error(kErrInvalidArgs, "VEGAS points per bin must be <= 256");
         std::vector<double> bounds((size_t)ng * 66);
         for (int g = 0; g < ng; g++)
             for (int i = 0; i <= 32; i++) {
                 bounds[(size_t)g * 66 + i] = (double)i / 32;
                 bounds[(size_t)g * 66 + 33 + i] = (double)i / 32;
             }
-        DevBuf bb, bI, bh;
+        DevBuf bb, bsum, bvar, bh;
         bb.upload(bounds.data(), bounds.size() * sizeof(double), s);
-        bI.ensure((size_t)ng * npts * mstride * sizeof(double));
+        bsum.ensure((size_t)ng * mstride * 1024 * sizeof(double));
+        bvar.ensure((size_t)ng * 1024 * sizeof(double));
         bh.ensure((size_t)iters * nq * sizeof(double));
         check_hip(hipMemsetAsync(bh.p, 0, (size_t)iters * nq * sizeof(double), s), "memset");
         std::vector<double> hist((size_t)iters * nq, 0.0);
@@ -607,7 +608,7 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                 launch_vegas_iter(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
                                   (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
                                   (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k, ppb,
-                                  (double *)bb.p, (double *)bI.p, (double *)bh.p, nq, s);
+                                  (double *)bb.p, (double *)bsum.p, (double *)bvar.p, (double *)bh.p, nq, s);
             }
             check_hip(hipGetLastError(), "vegas launch");
             used = k + 1;
@@ -937,6 +938,14 @@ int dplus_microbench_valu(double *fma32, double *sin32, double *cos32, double *f
     API_GUARD({
         Device::get().require();
         microbench_valu(Device::get().stream, fma32, sin32, cos32, fma64);
+    })
+}
+
+int dplus_copy_probe(long long bytes, double *ms) {
+    API_GUARD({
+        if (bytes <= 0 || bytes % 16) throw Error(kErrInvalidArgs, "bytes must be a positive multiple of 16");
+        Device::get().require();
+        copy_probe(Device::get().stream, bytes, ms);
     })
 }
 

@@ -124,6 +124,7 @@ void timing_set(bool on);
 void timing_clear();
 void timing_read(const char *name, long long *n, double *ms);
 void microbench_valu(hipStream_t s, double *fma32, double *sin32, double *cos32, double *fma64);
+void copy_probe(hipStream_t s, long long bytes, double *ms);
 
 // atoms and solvent boxes of one PDB leaf, sorted, tiled and resident in HBM
 struct Structure {

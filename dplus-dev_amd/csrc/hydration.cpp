@@ -7,15 +7,13 @@
 // probe shell as the hydration layer, and reduce the layer to boxes
 // (ReduceSolventSpaceToIrregularBoxes, :263-363).  Distances are measured
 // from voxel centres to atom surfaces (|c - r_atom| - radius).  The box
-// decomposition is greedy (z runs, then merged along y, then along x); any
-// decomposition gives the same Fourier transform.
+// decomposition is a greedy 3-D meshing (grow z, then y, then x); any
+// decomposition gives the same Fourier transform, fewer boxes cost less.
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <deque>
-#include <map>
-#include <tuple>
 #include <vector>
 
 #include "device.hpp"
@@ -109,59 +107,47 @@ void hydration_boxes(int n, const double *xyz, const double *rad, double probe, 
     // hydration layer: bulk voxels within the solvation thickness of the probe shell
     const float layer_max = (float)(probe + thickness);
     auto in_layer = [&](long long v) { return bulk[v] && dist[v] <= layer_max; };
-    // greedy boxes: runs along z, merged along y, then along x
-    struct Run {
-        long long j0, nj, k0, nk;
-    };
-    std::vector<std::vector<Run>> plane_runs(dim[0]);
-    parallel_for(dim[0], [&](long long i) {
-        std::vector<Run> runs;
-        std::map<std::pair<long long, long long>, std::pair<size_t, long long>> active;
-        for (long long j = 0; j < dim[1]; j++) {
-            std::map<std::pair<long long, long long>, std::pair<size_t, long long>> next;
-            long long k = 0;
-            while (k < dim[2]) {
-                if (!in_layer(vid(i, j, k))) {
-                    k++;
-                    continue;
-                }
-                long long e = k;
-                while (e < dim[2] && in_layer(vid(i, j, e))) e++;
-                auto key = std::make_pair(k, e - k);
-                auto it = active.find(key);
-                if (it != active.end() && it->second.second == j - 1) {
-                    runs[it->second.first].nj++;
-                    next[key] = {it->second.first, j};
-                } else {
-                    runs.push_back({j, 1, k, e - k});
-                    next[key] = {runs.size() - 1, j};
-                }
-                k = e;
-            }
-            active.swap(next);
-        }
-        plane_runs[i].swap(runs);
-    });
+    // greedy 3-D meshing: from each unassigned layer voxel (in x, y, z order)
+    // grow a box along z, then y, then x while every covered voxel is an
+    // unassigned layer voxel; cover those voxels and continue
     struct Box {
         long long i0, ni, j0, nj, k0, nk;
     };
     std::vector<Box> boxes;
-    std::map<std::tuple<long long, long long, long long, long long>, size_t> activeb;
-    for (long long i = 0; i < dim[0]; i++) {
-        std::map<std::tuple<long long, long long, long long, long long>, size_t> next;
-        for (const Run &r : plane_runs[i]) {
-            auto key = std::make_tuple(r.j0, r.nj, r.k0, r.nk);
-            auto it = activeb.find(key);
-            if (it != activeb.end() && boxes[it->second].i0 + boxes[it->second].ni == i) {
-                boxes[it->second].ni++;
-                next[key] = it->second;
-            } else {
-                boxes.push_back({i, 1, r.j0, r.nj, r.k0, r.nk});
-                next[key] = boxes.size() - 1;
+    std::vector<uint8_t> todo(nv, 0);
+    parallel_for(dim[0], [&](long long i) {
+        for (long long j = 0; j < dim[1]; j++)
+            for (long long k = 0; k < dim[2]; k++) {
+                const long long v = vid(i, j, k);
+                todo[v] = in_layer(v) ? 1 : 0;
             }
-        }
-        activeb.swap(next);
-    }
+    });
+    for (long long i = 0; i < dim[0]; i++)
+        for (long long j = 0; j < dim[1]; j++)
+            for (long long k = 0; k < dim[2]; k++) {
+                if (!todo[vid(i, j, k)]) continue;
+                long long nk = 1;
+                while (k + nk < dim[2] && todo[vid(i, j, k + nk)]) nk++;
+                long long nj = 1;
+                while (j + nj < dim[1]) {
+                    bool ok = true;
+                    for (long long c = 0; c < nk && ok; c++) ok = todo[vid(i, j + nj, k + c)] != 0;
+                    if (!ok) break;
+                    nj++;
+                }
+                long long ni = 1;
+                while (i + ni < dim[0]) {
+                    bool ok = true;
+                    for (long long b2 = 0; b2 < nj && ok; b2++)
+                        for (long long c = 0; c < nk && ok; c++) ok = todo[vid(i + ni, j + b2, k + c)] != 0;
+                    if (!ok) break;
+                    ni++;
+                }
+                for (long long a2 = 0; a2 < ni; a2++)
+                    for (long long b2 = 0; b2 < nj; b2++)
+                        for (long long c = 0; c < nk; c++) todo[vid(i + a2, j + b2, k + c)] = 0;
+                boxes.push_back({i, ni, j, nj, k, nk});
+            }
     dims.resize(3 * boxes.size());
     com.resize(3 * boxes.size());
     for (size_t b = 0; b < boxes.size(); b++) {

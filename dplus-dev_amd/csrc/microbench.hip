@@ -72,3 +72,36 @@ void microbench_valu(hipStream_t s, double *fma32, double *sin32, double *cos32,
 }
 
 }  // namespace dplus
+
+namespace dplus {
+
+__global__ __launch_bounds__(256) void copy_probe_kernel(const float4 *__restrict__ a, float4 *__restrict__ b,
+                                                         long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+// plain device-to-device copy of `bytes` (multiple of 16): HBM bandwidth probe and
+// the known-size reference for hardware byte counters
+void copy_probe(hipStream_t s, long long bytes, double *ms_out) {
+    DevBuf a, b;
+    a.ensure((size_t)bytes);
+    b.ensure((size_t)bytes);
+    check_hip(hipMemsetAsync(a.p, 1, (size_t)bytes, s), "memset");
+    const long long n = bytes / 16;
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), "event");
+    check_hip(hipEventCreate(&e1), "event");
+    check_hip(hipEventRecord(e0, s), "record");
+    copy_probe_kernel<<<Device::get().cus * 8, 256, 0, s>>>((const float4 *)a.p, (float4 *)b.p, n);
+    check_hip(hipGetLastError(), "copy launch");
+    check_hip(hipEventRecord(e1, s), "record");
+    check_hip(hipEventSynchronize(e1), "sync");
+    float ms = 0.f;
+    check_hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    if (ms_out) *ms_out = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+}  // namespace dplus

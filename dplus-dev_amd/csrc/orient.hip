@@ -145,14 +145,14 @@ __device__ __forceinline__ void phase_sincos<double>(double qd, double &s, doubl
 constexpr int kMaxM = 8;  // q-values of one shell group handled per pass
 
 
-// accumulate the docked amplitude along unit direction qv for q-values qs[0..m)
-template <typename T, typename T4>
+// accumulate the docked amplitude along unit direction qv for q-values qs[0..m), m <= M
+template <typename T, typename T4, int M = kMaxM>
 __device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ grids,
                                                  const RotDev *__restrict__ rots, int nrot,
                                                  const double4 *__restrict__ trans, int layer,
                                                  double step, const double qv[3],
                                                  const double *__restrict__ qs, int m,
-                                                 cplx<T> acc[kMaxM]) {
+                                                 cplx<T> acc[M]) {
     for (int r = 0; r < nrot; r++) {
         const RotDev rd = rots[r];
         double rv[3];
@@ -167,7 +167,7 @@ __device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ g
         const bool trivial = (rd.nt == 1) && trans[rd.t0].x == 0.0 && trans[rd.t0].y == 0.0 &&
                              trans[rd.t0].z == 0.0;
 #pragma unroll
-        for (int j = 0; j < kMaxM; j++) {
+        for (int j = 0; j < M; j++) {
             if (j < m) {
                 const T t = (T)((qs[j] - layer * step) / step);
                 cplx<T> a = hermite<T>(amps[1], amps[2], d1, d2, t);
@@ -261,7 +261,14 @@ __global__ void mc_reduce(const GroupDev *__restrict__ groups, const int *__rest
 // ------------------------------------------------------------------ VEGAS
 constexpr int kVB = 32;  // bins per axis (GPUHybridCalc.cu:30-31)
 
-// bounds[g*66 + 0..32] theta-axis boundaries in u = (1-cos)/2, [33..65] phi-axis in phi/2pi
+constexpr int kVM = 16;        // q-values per pass in the VEGAS kernels
+constexpr int kVPts = 4;       // points per lane and bin: points per bin <= 256
+
+// bounds[g*66 + 0..32] theta-axis boundaries in u = (1-cos)/2, [33..65] phi-axis in phi/2pi.
+// One wave per (group, bin): the wave evaluates the bin's ppb points, and writes
+// binsum[(gi * mstride + j) * 1024 + bin] = sum over the bin's points of I_j and
+// binvar[gi * 1024 + bin] = ppb * sum_p (vol * S_p - mean)^2 with S_p = sum_j I_j(p)
+// (HybridVEGASBinReduceToIKernel / HybridVEGASBinReduceKernel, HybridOA.cu:300-449).
 template <typename T, typename T4>
 __global__ __launch_bounds__(256) void vegas_sample(const T4 *const *__restrict__ grids,
                                                     const RotDev *__restrict__ rots, int nrot,
@@ -271,34 +278,73 @@ __global__ __launch_bounds__(256) void vegas_sample(const T4 *const *__restrict_
                                                     const double *__restrict__ qs, double step,
                                                     uint64_t seed, int iter, int ppb,
                                                     const double *__restrict__ bounds,
-                                                    double *__restrict__ I) {
+                                                    double *__restrict__ binsum,
+                                                    double *__restrict__ binvar) {
     const int gi = blockIdx.y;
     const int g = active[gi];
     const GroupDev G = groups[g];
-    const int npts = kVB * kVB * ppb;
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= npts) return;
-    const int bin = idx / ppb, ti = bin / kVB, pj = bin % kVB;
+    const int lane = threadIdx.x & 63;
+    const int bin = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (bin >= kVB * kVB) return;
+    const int ti = bin / kVB, pj = bin % kVB;
     const double *bt = bounds + (size_t)g * 66, *bp = bt + 33;
-    double u, v;
-    uniform_pair(seed, (uint32_t)idx, (uint32_t)iter, (uint32_t)G.layer, kStreamVegas, u, v);
-    const double th = acos(1. - 2. * (u * (bt[ti + 1] - bt[ti]) + bt[ti]));
-    const double ph = (v * (bp[pj + 1] - bp[pj]) + bp[pj]) * 2. * kPi;
-    double st, ct, sp, cp;
-    sincos(th, &st, &ct);
-    sincos(ph, &sp, &cp);
-    const double qv[3] = {cp * st, sp * st, ct};
-    double *o = I + ((size_t)gi * npts + idx) * mstride;
-    for (int jb = 0; jb < G.m; jb += kMaxM) {
-        const int m = min(kMaxM, G.m - jb);
-        cplx<T> acc[kMaxM];
+    const double vol = (bt[ti + 1] - bt[ti]) * (bp[pj + 1] - bp[pj]);
+    double qv[kVPts][3];
+    double so[kVPts];
 #pragma unroll
-        for (int j = 0; j < kMaxM; j++) acc[j] = {(T)0, (T)0};
-        docked_amplitude<T, T4>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, acc);
-#pragma unroll
-        for (int j = 0; j < kMaxM; j++)
-            if (j < m) o[jb + j] = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
+    for (int k = 0; k < kVPts; k++) {
+        so[k] = 0.0;
+        const int p = lane + 64 * k;
+        double u = 0.5, v = 0.5;
+        if (p < ppb) uniform_pair(seed, (uint32_t)(bin * ppb + p), (uint32_t)iter, (uint32_t)G.layer, kStreamVegas, u, v);
+        const double th = acos(1. - 2. * (u * (bt[ti + 1] - bt[ti]) + bt[ti]));
+        const double ph = (v * (bp[pj + 1] - bp[pj]) + bp[pj]) * 2. * kPi;
+        double st, ct, sp, cp;
+        sincos(th, &st, &ct);
+        sincos(ph, &sp, &cp);
+        qv[k][0] = cp * st;
+        qv[k][1] = sp * st;
+        qv[k][2] = ct;
     }
+    for (int jb = 0; jb < G.m; jb += kVM) {
+        const int m = min(kVM, G.m - jb);
+        double e[kVM];
+#pragma unroll
+        for (int j = 0; j < kVM; j++) e[j] = 0.0;
+#pragma unroll
+        for (int k = 0; k < kVPts; k++) {
+            if (lane + 64 * k >= ppb) continue;
+            cplx<T> acc[kVM];
+#pragma unroll
+            for (int j = 0; j < kVM; j++) acc[j] = {(T)0, (T)0};
+            docked_amplitude<T, T4, kVM>(grids, rots, nrot, trans, G.layer, step, qv[k], qs + G.q0 + jb, m, acc);
+#pragma unroll
+            for (int j = 0; j < kVM; j++) {
+                const double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
+                if (j < m) {
+                    e[j] += I;
+                    so[k] += I;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kVM; j++) {
+            const double t = wave_sum(e[j]);
+            if (lane == 0 && j < m) binsum[((size_t)gi * mstride + jb + j) * (kVB * kVB) + bin] = t;
+        }
+    }
+    // variance of vol * S over the bin's points, two-pass (equal to the reference's Welford sum)
+    double sx = 0.0;
+#pragma unroll
+    for (int k = 0; k < kVPts; k++)
+        if (lane + 64 * k < ppb) sx += vol * so[k];
+    const double mean = wave_sum(sx) / (double)ppb;
+    double sq = 0.0;
+#pragma unroll
+    for (int k = 0; k < kVPts; k++)
+        if (lane + 64 * k < ppb) sq += (vol * so[k] - mean) * (vol * so[k] - mean);
+    sq = wave_sum(sq);
+    if (lane == 0) binvar[(size_t)gi * (kVB * kVB) + bin] = sq * ppb;
 }
 
 // serial damped re-binning of one axis (HybridOA.cu:181-263); boundaries the
@@ -338,55 +384,36 @@ __device__ void vegas_resize(const double *var, double *b) {
     for (int i = 0; i < kVB - 1; i++) b[1 + i] = nb[i];
 }
 
-// one block (1024 threads = the 32x32 bins) per active group: integral
-// estimate of the iteration, per-bin variance marginals, re-binning and the
-// running mean (HybridVEGASBinReduceToIKernel, HybridVEGASBinReduceKernel,
-// HybridVEGASResizeGridKernel; GPUHybridCalc.cu:654-701).
+// one block (1024 threads = the 32x32 bins) per active group: the iteration's
+// integral sum_bins vol * mean_bin(I_j), the theta/phi marginals of the bin
+// variances, re-binning, and the running mean over iterations
+// (HybridVEGASResizeGridKernel; GPUHybridCalc.cu:654-701).
 // hist[iter * nq + q] holds the running mean after iteration iter.
 __global__ __launch_bounds__(1024) void vegas_reduce(const GroupDev *__restrict__ groups,
                                                      const int *__restrict__ active, int mstride,
-                                                     int ppb, int iter, const double *__restrict__ I,
+                                                     int ppb, int iter, const double *__restrict__ binsum,
+                                                     const double *__restrict__ binvar,
                                                      double *__restrict__ bounds,
                                                      double *__restrict__ hist, int nq) {
     const int gi = blockIdx.x;
     const int g = active[gi];
     const GroupDev G = groups[g];
-    const int npts = kVB * kVB * ppb;
     const int bin = threadIdx.x, ti = bin / kVB, pj = bin % kVB;
     double *bt = bounds + (size_t)g * 66, *bp = bt + 33;
     const double vol = (bt[ti + 1] - bt[ti]) * (bp[pj + 1] - bp[pj]);
-    const double *Ib = I + ((size_t)gi * npts + (size_t)bin * ppb) * mstride;
     __shared__ double sv[1024];
-    __shared__ double se[16][kMaxM];
+    __shared__ double se[16];
     __shared__ double marg[2][kVB];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // per-bin variance of vol * sum_q I over the bin's points (Welford)
-    double mean = 0., ssq = 0.;
-    for (int p = 0; p < ppb; p++) {
-        double so = 0.;
-        for (int j = 0; j < G.m; j++) so += Ib[(size_t)p * mstride + j];
-        const double diff = vol * so - mean;
-        mean += diff / (double)(p + 1);
-        ssq += diff * diff * (double)p / (double)(p + 1);
-    }
-    sv[bin] = ssq * ppb;
-    // integral: sum over bins of vol * (bin mean), q-batch by q-batch
-    for (int jb = 0; jb < G.m; jb += kMaxM) {
-        const int m = min(kMaxM, G.m - jb);
-#pragma unroll
-        for (int j = 0; j < kMaxM; j++) {
-            double e = 0.0;
-            if (j < m)
-                for (int p = 0; p < ppb; p++) e += Ib[(size_t)p * mstride + jb + j];
-            double c = wave_sum(e * vol / (double)ppb);
-            if (lane == 0) se[w][j] = c;
-        }
+    sv[bin] = binvar[(size_t)gi * 1024 + bin];
+    for (int j = 0; j < G.m; j++) {
+        const double c = wave_sum(binsum[((size_t)gi * mstride + j) * 1024 + bin] * vol / (double)ppb);
+        if (lane == 0) se[w] = c;
         __syncthreads();
-        if (threadIdx.x < m) {
-            const int j = threadIdx.x;
+        if (threadIdx.x == 0) {
             double e = 0.0;
-            for (int k = 0; k < 16; k++) e += se[k][j];
-            const int q = G.q0 + jb + j;
+            for (int k = 0; k < 16; k++) e += se[k];
+            const int q = G.q0 + j;
             if (iter > 0) e = (hist[(size_t)(iter - 1) * nq + q] * (double)iter + e) / (double)(iter + 1);
             hist[(size_t)iter * nq + q] = e;
         }
@@ -528,18 +555,17 @@ void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot
 void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, int nrot,
                        const double4 *trans, const GroupDev *groups, const int *active, int nactive,
                        int mstride, const double *qs, double step, uint64_t seed, int iter, int ppb,
-                       double *bounds, double *I, double *hist, int nq, hipStream_t s) {
-    const int npts = kVB * kVB * ppb;
-    dim3 grid((npts + 255) / 256, nactive);
+                       double *bounds, double *binsum, double *binvar, double *hist, int nq, hipStream_t s) {
+    dim3 grid(kVB * kVB / 4, nactive);
     if (fp64)
         vegas_sample<double, double4><<<grid, 256, 0, s>>>((const double4 *const *)grids, rots, nrot,
                                                             trans, groups, active, mstride, qs, step,
-                                                            seed, iter, ppb, bounds, I);
+                                                            seed, iter, ppb, bounds, binsum, binvar);
     else
         vegas_sample<float, float4><<<grid, 256, 0, s>>>((const float4 *const *)grids, rots, nrot, trans,
                                                           groups, active, mstride, qs, step, seed, iter,
-                                                          ppb, bounds, I);
-    vegas_reduce<<<nactive, 1024, 0, s>>>(groups, active, mstride, ppb, iter, I, bounds, hist, nq);
+                                                          ppb, bounds, binsum, binvar);
+    vegas_reduce<<<nactive, 1024, 0, s>>>(groups, active, mstride, ppb, iter, binsum, binvar, bounds, hist, nq);
 }
 
 void launch_sphr_batch(bool fp64, const void *g, int N, double step, int n, const double *q,

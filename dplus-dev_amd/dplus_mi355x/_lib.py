@@ -91,6 +91,7 @@ SIGNATURES = {
     "dplus_timing_get": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong), _D]),
     "dplus_timing_reset": (ctypes.c_int, []),
     "dplus_microbench_valu": (ctypes.c_int, [_D, _D, _D, _D]),
+    "dplus_copy_probe": (ctypes.c_int, [ctypes.c_longlong, _D]),
 }
 
 _lib = None
@@ -164,3 +165,10 @@ def microbench_valu():
     r = [ctypes.c_double() for _ in range(4)]
     check(lib().dplus_microbench_valu(*[ctypes.byref(x) for x in r]))
     return dict(zip(("fma32", "sin32", "cos32", "fma64"), (x.value for x in r)))
+
+
+def copy_probe(nbytes):
+    """Time a plain device copy of nbytes (ms); the known-size kernel for byte counters."""
+    ms = ctypes.c_double()
+    check(lib().dplus_copy_probe(int(nbytes), ctypes.byref(ms)))
+    return ms.value

@@ -123,6 +123,9 @@ int dplus_timing_reset(void);
 /* Throughput microbenchmark of gfx950 VALU ops: ops/s of v_fma_f32, v_sin_f32,
  * v_cos_f32 and v_fma_f64 over the whole device (for the transcendental weight). */
 int dplus_microbench_valu(double *fma32, double *sin32, double *cos32, double *fma64);
+/* A plain device copy of `bytes` (multiple of 16) timed in ms: HBM bandwidth probe and the
+ * known-size kernel hardware byte counters are calibrated against. */
+int dplus_copy_probe(long long bytes, double *ms);
 
 /* ------------------------------------------------------------------ orientation average */
 /* A leaf of the hybrid regime (Amplitude.cpp:2028-2303): a grid seen under nrot
