@@ -423,6 +423,54 @@ void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, 
 void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
                         const double4 *trans, int n, const double *qxyz, double *out, hipStream_t s);
 
+// R^T with R = Rx(a) Ry(b) Rz(c) (EulerD, Amplitude.h:557-573; HybridOA.cu:60-63)
+void euler_rt(double a, double b, double c, double m[9]) {
+    const double s1 = std::sin(a), c1 = std::cos(a), s2 = std::sin(b), c2 = std::cos(b), s3 = std::sin(c),
+                 c3 = std::cos(c);
+    const double rt[9] = {c2 * c3, c1 * s3 + c3 * s1 * s2, s1 * s3 - c1 * c3 * s2,
+                          -(c2 * s3), c1 * c3 - s1 * s2 * s3, c3 * s1 + c1 * s2 * s3,
+                          s2, -(c2 * s1), c1 * c2};
+    for (int i = 0; i < 9; i++) m[i] = rt[i];
+}
+
+// GridSymmetry::OrganizeParameters/PreCalculate (DefaultModels/Symmetries.cpp:29-97):
+// angles wrapped to [-pi, pi), validated, and the basis a = (da,0,0),
+// b = db (cos g, sin g, 0), c = dc (cos b, sin b cat, sin b sin(acos cat)),
+// cat = (cos a - cos b cos g) / (sin b sin g)
+LatticeDev space_filling_lattice(const double *d, const double *deg, const double *reps) {
+    double ang[3];
+    for (int k = 0; k < 3; k++) {
+        double a = deg[k] * kPi / 180.0;
+        while (a < -kPi) a += 2.0 * kPi;
+        while (a >= kPi) a -= 2.0 * kPi;
+        ang[k] = a;
+    }
+    const double al = ang[0], be = ang[1], ga = ang[2];
+    for (int k = 0; k < 3; k++)
+        if (ang[k] == 0.0 || std::fabs(std::fabs(ang[k]) - kPi) < 1e-6)
+            throw Error(kErrInvalidArgs, "Angles in a Space-filling Symmetry cannot be zero or 180. The result is a 2D "
+                                         "space. To create a 2D space, make the angle 90 degrees and one repetition in "
+                                         "that direction.");
+    if (std::fabs(al) + std::fabs(be) <= std::fabs(ga) || std::fabs(ga) + std::fabs(be) <= std::fabs(al) ||
+        std::fabs(al) + std::fabs(ga) <= std::fabs(be))
+        throw Error(kErrInvalidArgs, "In the Space-filling Symmetry, the sum of any two angles must be greater than the third.");
+    const double cat = (std::cos(al) - std::cos(be) * std::cos(ga)) / (std::sin(be) * std::sin(ga));
+    if (std::fabs(cat) > 1.0) throw Error(kErrInvalidArgs, "In the Space-filling Symmetry, the angles don't form a valid basis.");
+    LatticeDev L{};
+    L.v[0] = d[0];
+    L.v[3] = d[1] * std::cos(ga);
+    L.v[4] = d[1] * std::sin(ga);
+    L.v[6] = d[2] * std::cos(be);
+    L.v[7] = d[2] * std::sin(be) * cat;
+    L.v[8] = d[2] * std::sin(be) * std::sin(std::acos(cat));
+    for (int k = 0; k < 3; k++) {
+        if (!(reps[k] >= 1.0)) throw Error(kErrInvalidArgs, "space-filling repetitions must be at least 1");
+        L.n[k] = reps[k];
+    }
+    L.on = 1;
+    return L;
+}
+
 // shell groups (OAJacob.cu:234-249): consecutive q with L*step < q <= (L+1)*step
 std::vector<GroupDev> make_groups(const double *q, int nq, double step) {
     std::vector<GroupDev> gs;
@@ -450,28 +498,18 @@ static bool converged3(const std::vector<double> &hist, int k, int q0, int m, in
 
 DockSet::DockSet(int nleaf, const dplus_leaf *leaves, hipStream_t s) {
     if (nleaf <= 0) throw Error(kErrInvalidArgs, "no leaves");
-    std::vector<const void *> gp;
-    std::vector<LeafGeo> geo;
+    std::vector<Grid *> gl;
     std::vector<RotDev> rots;
     std::vector<double4> tr;
     for (int l = 0; l < nleaf; l++) {
         const dplus_leaf &L = leaves[l];
         if (!L.grid) throw Error(kErrInvalidArgs, "leaf without grid");
-        Grid *g = (Grid *)L.grid;
-        if (l == 0) {
-            grid_size = g->grid_size;
-            qmax = g->qmax;
-            step = g->step;
-            fp64 = g->fp64();
-        } else if (g->grid_size != grid_size || g->qmax != qmax || g->fp64() != fp64) {
-            throw Error(kErrInvalidArgs, "all leaves of one orientation average must share grid size, qmax and precision");
-        }
-        gp.push_back(g->dev.p);
-        geo.push_back({g->N, g->step});
+        gl.push_back((Grid *)L.grid);
         int t0 = 0;
         for (int r = 0; r < L.nrot; r++) {
             RotDev rd;
-            rd.ang = make_double4(L.rot[4 * r], L.rot[4 * r + 1], L.rot[4 * r + 2], L.rot[4 * r + 3]);
+            euler_rt(L.rot[4 * r], L.rot[4 * r + 1], L.rot[4 * r + 2], rd.m);
+            rd.scale = L.rot[4 * r + 3];
             rd.leaf = l;
             rd.t0 = (int)tr.size();
             rd.nt = L.ntrans[r];
@@ -482,6 +520,57 @@ DockSet::DockSet(int nleaf, const dplus_leaf *leaves, hipStream_t s) {
             t0 += rd.nt;
             rots.push_back(rd);
         }
+    }
+    finish(gl, rots, tr, s);
+}
+
+DockSet::DockSet(const std::vector<Placement> &pl, hipStream_t s) {
+    if (pl.empty()) throw Error(kErrInvalidArgs, "no placements");
+    std::vector<Grid *> gl;
+    std::vector<RotDev> rots;
+    std::vector<std::vector<double4>> per_rot;
+    for (const Placement &p : pl) {
+        if (!p.grid) throw Error(kErrInvalidArgs, "placement without grid");
+        int l = (int)(std::find(gl.begin(), gl.end(), p.grid) - gl.begin());
+        if (l == (int)gl.size()) gl.push_back(p.grid);
+        size_t r = 0;
+        for (; r < rots.size(); r++)
+            if (rots[r].leaf == l && rots[r].scale == p.scale && std::equal(p.rt, p.rt + 9, rots[r].m)) break;
+        if (r == rots.size()) {
+            RotDev rd;
+            std::copy(p.rt, p.rt + 9, rd.m);
+            rd.scale = p.scale;
+            rd.leaf = l;
+            rd.t0 = rd.nt = rd.pad = 0;
+            rots.push_back(rd);
+            per_rot.emplace_back();
+        }
+        per_rot[r].push_back(make_double4(p.t[0], p.t[1], p.t[2], 0.0));
+    }
+    std::vector<double4> tr;
+    for (size_t r = 0; r < rots.size(); r++) {
+        rots[r].t0 = (int)tr.size();
+        rots[r].nt = (int)per_rot[r].size();
+        tr.insert(tr.end(), per_rot[r].begin(), per_rot[r].end());
+    }
+    finish(gl, rots, tr, s);
+}
+
+void DockSet::finish(std::vector<Grid *> &gl, std::vector<RotDev> &rots, std::vector<double4> &tr, hipStream_t s) {
+    std::vector<const void *> gp;
+    std::vector<LeafGeo> geo;
+    for (size_t l = 0; l < gl.size(); l++) {
+        const Grid *g = gl[l];
+        if (l == 0) {
+            grid_size = g->grid_size;
+            qmax = g->qmax;
+            step = g->step;
+            fp64 = g->fp64();
+        } else if (g->grid_size != grid_size || g->qmax != qmax || g->fp64() != fp64) {
+            throw Error(kErrInvalidArgs, "all leaves of one orientation average must share grid size, qmax and precision");
+        }
+        gp.push_back(g->dev.p);
+        geo.push_back({g->N, g->step});
     }
     if (tr.empty()) tr.push_back(make_double4(0, 0, 0, 0));
     nrot = (int)rots.size();
@@ -889,6 +978,20 @@ int dplus_dock_grid(dplus_grid *outh, int nleaf, const dplus_leaf *leaves, doubl
         DockSet D(nleaf, leaves, s);
         if (D.fp64 != out->fp64()) throw Error(kErrInvalidArgs, "output precision differs from the leaves'");
         dock_grid(D, *out, scale, s);
+    })
+}
+
+int dplus_dock_grid_space_filling(dplus_grid *outh, int nleaf, const dplus_leaf *leaves, double scale,
+                                  const double *distances, const double *angles_deg, const double *reps) {
+    API_GUARD({
+        Grid *out = (Grid *)outh;
+        if (!out || !leaves || !distances || !angles_deg || !reps) throw Error(kErrInvalidArgs, "null argument");
+        Device::get().require();
+        hipStream_t s = Device::get().stream;
+        LatticeDev lat = space_filling_lattice(distances, angles_deg, reps);
+        DockSet D(nleaf, leaves, s);
+        if (D.fp64 != out->fp64()) throw Error(kErrInvalidArgs, "output precision differs from the leaves'");
+        dock_grid(D, *out, scale, s, &lat);
     })
 }
 

@@ -53,12 +53,17 @@ struct DevBuf {
     void upload(const void *src, size_t bytes, hipStream_t s);
 };
 
-// Rotation record of a docked leaf: (alpha, beta, gamma, scale), translations
-// [t0, t0+nt) of the translation table, leaf = index into the grid table.
+// Rotation record of a docked leaf: m = R^T row-major (R = Rx(a) Ry(b) Rz(c) for
+// Euler angles, Amplitude.h:557-573, or a composition of such), the amplitude
+// scale, translations [t0, t0+nt) of the translation table, leaf = grid index.
 struct RotDev {
-    double4 ang;
+    double m[9];
+    double scale;
     int leaf, t0, nt, pad;
 };
+
+// R^T of R = Rx(a) Ry(b) Rz(c), row-major
+void euler_rt(double a, double b, double c, double m[9]);
 
 // one shell group: q-values q0..q0+m-1 lie in (layer*step, (layer+1)*step]
 struct GroupDev {
@@ -85,6 +90,15 @@ struct Grid {
     void upload(hipStream_t s);
 };
 
+// one placed copy of a leaf: amplitude scale * A_grid(R^T q) e^{i q.t};
+// rt = R^T row-major
+struct Placement {
+    Grid *grid = nullptr;
+    double rt[9];
+    double t[3];
+    double scale = 1.0;
+};
+
 // leaves of one orientation average, flattened for the kernels
 struct DockSet {
     int grid_size = 0, nrot = 0;
@@ -92,6 +106,12 @@ struct DockSet {
     bool fp64 = false;
     DevBuf grids, geos, rotb, trb;
     DockSet(int nleaf, const dplus_leaf *leaves, hipStream_t s);
+    // placements sharing grid, rotation and scale become one rotation record
+    // with several translations
+    DockSet(const std::vector<Placement> &pl, hipStream_t s);
+
+   private:
+    void finish(std::vector<Grid *> &gl, std::vector<RotDev> &rots, std::vector<double4> &tr, hipStream_t s);
 };
 
 // run fn(i) for i in [0, n) on the host's cores (dynamic chunks)
@@ -146,7 +166,16 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                          dplus_oa_stats *st, hipStream_t s);
 void gk_average(const DockSet &D, int nq, const double *q, double eps, long long max_depth, double *out,
                 dplus_oa_stats *st, hipStream_t s);
-void dock_grid(const DockSet &D, Grid &out, double scale, hipStream_t s);
+// space-filling lattice of a docked grid: basis vectors v[3k..3k+2] (nm) and
+// repetitions n[k]; on = 0 for none
+struct LatticeDev {
+    double v[9];
+    double n[3];
+    int on;
+};
+void dock_grid(const DockSet &D, Grid &out, double scale, hipStream_t s, const LatticeDev *lattice = nullptr);
+// basis of a space-filling symmetry from distances (nm), angles (degrees) and repetitions
+LatticeDev space_filling_lattice(const double *distances, const double *angles_deg, const double *reps);
 void synth_walk(unsigned long long seed, int n, double step, double min_sep, double ax, double ay, double az,
                 double *xyz);
 

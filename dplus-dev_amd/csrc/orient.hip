@@ -116,14 +116,9 @@ __device__ __forceinline__ void group_planes(const T4 *__restrict__ g, int layer
 }
 
 // R^T q with R = Rx(a) Ry(b) Rz(c): HybridOA.cu:60-63
-__device__ __forceinline__ void rotate_t(const double4 r, const double q[3], double o[3]) {
-    double s1, c1, s2, c2, s3, c3;
-    sincos(r.x, &s1, &c1);
-    sincos(r.y, &s2, &c2);
-    sincos(r.z, &s3, &c3);
-    o[0] = (c2 * c3) * q[0] + (c1 * s3 + c3 * s1 * s2) * q[1] + (s1 * s3 - c1 * c3 * s2) * q[2];
-    o[1] = -(c2 * s3) * q[0] + (c1 * c3 - s1 * s2 * s3) * q[1] + (c3 * s1 + c1 * s2 * s3) * q[2];
-    o[2] = (s2)*q[0] - (c2 * s1) * q[1] + (c1 * c2) * q[2];
+__device__ __forceinline__ void rotate_t(const RotDev &r, const double q[3], double o[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; i++) o[i] = r.m[3 * i] * q[0] + r.m[3 * i + 1] * q[1] + r.m[3 * i + 2] * q[2];
 }
 
 // phase factor e^{i q d} for the docking translations: FP64 argument, reduced
@@ -156,14 +151,14 @@ __device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ g
     for (int r = 0; r < nrot; r++) {
         const RotDev rd = rots[r];
         double rv[3];
-        rotate_t(rd.ang, qv, rv);
+        rotate_t(rd, qv, rv);
         if (fabs(rv[2]) > 1.) rv[2] = (rv[2] > 0.) ? 1. : -1.;
         const double nth = acos(rv[2]);
         double nph = atan2(rv[1], rv[0]);
         if (nph < 0.0) nph += k2Pi;
         cplx<T> amps[4], d1, d2;
         group_planes<T, T4>(grids[rd.leaf], layer, nth, nph, amps, d1, d2);
-        const T sc = (T)rd.ang.w;
+        const T sc = (T)rd.scale;
         const bool trivial = (rd.nt == 1) && trans[rd.t0].x == 0.0 && trans[rd.t0].y == 0.0 &&
                              trans[rd.t0].z == 0.0;
 #pragma unroll
@@ -518,7 +513,7 @@ __global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const Lea
     for (int r = 0; r < nrot; r++) {
         const RotDev rd = rots[r];
         double rv[3];
-        rotate_t(rd.ang, q, rv);
+        rotate_t(rd, q, rv);
         const LeafGeo G = geo[rd.leaf];
         cplx<T> a = get_cart<T, T4>(grids[rd.leaf], G.N, G.step, rv[0], rv[1], rv[2]);
         double sc = 0.0, ss = 0.0;
@@ -529,8 +524,8 @@ __global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const Lea
             sc += c;
             ss += s;
         }
-        ax += ((double)a.x * sc - (double)a.y * ss) * rd.ang.w;
-        ay += ((double)a.y * sc + (double)a.x * ss) * rd.ang.w;
+        ax += ((double)a.x * sc - (double)a.y * ss) * rd.scale;
+        ay += ((double)a.y * sc + (double)a.x * ss) * rd.scale;
     }
     out[i] = ax * ax + ay * ay;
 }
@@ -621,10 +616,38 @@ __device__ __forceinline__ void node_indices(long long index, int &qi, long long
 
 // ManualSymmetry::calcAmplitude (Symmetries.cpp:697-716) at every node of the
 // output grid (Grid.cpp:760-826 node vectors), sub-amplitudes through GetCart
+// Space-filling lattice sum sum_{n<N} e^{i n d} = e^{i(N-1)d/2} sin(Nd/2)/sin(d/2)
+// (GridSymmetry::calcAmplitude, DefaultModels/Symmetries.cpp:97-136; GPU
+// CalcSpaceFillingKernelJcb, GPU/SymmetryGenJacob.cu:32-132).  N at d = 0 as the
+// reference does; at the other zeros of sin(d/2) (d = 2 pi k) the limit
+// N (-1)^{(N-1)k}, where the reference's expression is 0/0.
+__device__ inline void lattice_mul(double d, double n, double &re, double &im) {
+    if (fabs(d) < 100.0 * 2.220446049250313e-16 || n == 1.0) {
+        re *= n;
+        im *= n;
+        return;
+    }
+    const double h = 0.5 * d;
+    const double sh = sin(h);
+    double f;
+    if (fabs(sh) < 1e-12) {
+        const double k = rint(d / k2Pi);
+        f = (fmod(fabs((n - 1.0) * k), 2.0) == 1.0) ? -n : n;
+    } else {
+        f = sin(n * h) / sh;
+    }
+    double sp, cp;
+    sincos((n - 1.0) * h, &sp, &cp);
+    const double r = re * cp - im * sp, i = im * cp + re * sp;
+    re = r * f;
+    im = i * f;
+}
+
 template <typename T, typename T4>
 __global__ void dock_grid_kernel(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
                                  const RotDev *__restrict__ rots, int nrot, const double4 *__restrict__ trans,
-                                 T4 *__restrict__ out, double step_out, long long P, double scale) {
+                                 T4 *__restrict__ out, double step_out, long long P, double scale,
+                                 LatticeDev lat) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= P) return;
     int qi;
@@ -646,7 +669,7 @@ __global__ void dock_grid_kernel(const T4 *const *__restrict__ grids, const Leaf
     for (int r = 0; r < nrot; r++) {
         const RotDev rd = rots[r];
         double rv[3];
-        rotate_t(rd.ang, q, rv);
+        rotate_t(rd, q, rv);
         const LeafGeo G = geo[rd.leaf];
         cplx<T> a = get_cart<T, T4>(grids[rd.leaf], G.N, G.step, rv[0], rv[1], rv[2]);
         double sc = 0.0, ss = 0.0;
@@ -657,9 +680,12 @@ __global__ void dock_grid_kernel(const T4 *const *__restrict__ grids, const Leaf
             sc += c;
             ss += s;
         }
-        ax += ((double)a.x * sc - (double)a.y * ss) * rd.ang.w;
-        ay += ((double)a.y * sc + (double)a.x * ss) * rd.ang.w;
+        ax += ((double)a.x * sc - (double)a.y * ss) * rd.scale;
+        ay += ((double)a.y * sc + (double)a.x * ss) * rd.scale;
     }
+    if (lat.on)
+        for (int k = 0; k < 3; k++)
+            lattice_mul(q[0] * lat.v[3 * k] + q[1] * lat.v[3 * k + 1] + q[2] * lat.v[3 * k + 2], lat.n[k], ax, ay);
     T4 o;
     o.x = (T)(ax * scale);
     o.y = (T)(ay * scale);
@@ -668,19 +694,21 @@ __global__ void dock_grid_kernel(const T4 *const *__restrict__ grids, const Leaf
     out[idx] = o;
 }
 
-void dock_grid(const DockSet &D, Grid &out, double scale, hipStream_t s) {
+void dock_grid(const DockSet &D, Grid &out, double scale, hipStream_t s, const LatticeDev *lattice) {
     const unsigned nb = (unsigned)((out.P + 255) / 256);
+    LatticeDev lat{};
+    if (lattice) lat = *lattice;
     KTimer kt("dock", s);
     if (D.fp64)
         dock_grid_kernel<double, double4><<<nb, 256, 0, s>>>((const double4 *const *)D.grids.p,
                                                               (const LeafGeo *)D.geos.p, (const RotDev *)D.rotb.p,
                                                               D.nrot, (const double4 *)D.trb.p,
-                                                              (double4 *)out.dev.p, out.step, out.P, scale);
+                                                              (double4 *)out.dev.p, out.step, out.P, scale, lat);
     else
         dock_grid_kernel<float, float4><<<nb, 256, 0, s>>>((const float4 *const *)D.grids.p,
                                                             (const LeafGeo *)D.geos.p, (const RotDev *)D.rotb.p,
                                                             D.nrot, (const double4 *)D.trb.p, (float4 *)out.dev.p,
-                                                            out.step, out.P, scale);
+                                                            out.step, out.P, scale, lat);
     check_hip(hipGetLastError(), "dock_grid launch");
     out.splines(s);
     check_hip(hipStreamSynchronize(s), "dock_grid");

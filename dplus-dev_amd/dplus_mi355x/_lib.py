@@ -20,6 +20,7 @@ CALC_DUMMY_SOLVENT = 2
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
+_VP = ctypes.POINTER(ctypes.c_void_p)  # char ** outputs, released with dplus_free
 
 
 class DplusError(RuntimeError):
@@ -80,6 +81,8 @@ SIGNATURES = {
     "dplus_orientation_average": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_int, _D,
                                                  ctypes.POINTER(OAParams), _D, ctypes.POINTER(OAStats)]),
     "dplus_dock_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_double]),
+    "dplus_dock_grid_space_filling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Leaf),
+                                                     ctypes.c_double, _D, _D, _D]),
     "dplus_synth_walk": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
     "dplus_hydration_boxes": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double,
@@ -92,6 +95,22 @@ SIGNATURES = {
     "dplus_timing_reset": (ctypes.c_int, []),
     "dplus_microbench_valu": (ctypes.c_int, [_D, _D, _D, _D]),
     "dplus_copy_probe": (ctypes.c_int, [ctypes.c_longlong, _D]),
+    "dplus_amp_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _VP]),
+    "dplus_amp_write": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_char_p]),
+    "dplus_free": (None, [ctypes.c_void_p]),
+    "dplus_backend_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "dplus_backend_destroy": (None, [ctypes.c_void_p]),
+    "dplus_backend_check_capabilities": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "dplus_backend_get_all_model_metadata": (ctypes.c_int, [ctypes.c_void_p, _VP]),
+    "dplus_backend_initialize_cache": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "dplus_backend_start_generate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    "dplus_backend_get_job_status": (ctypes.c_int, [ctypes.c_void_p, _VP]),
+    "dplus_backend_get_generate_results": (ctypes.c_int, [ctypes.c_void_p, _VP]),
+    "dplus_backend_get_model_ptrs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint), ctypes.c_int, _I]),
+    "dplus_backend_get_amplitude": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, _VP,
+                                                   ctypes.POINTER(ctypes.c_longlong)]),
+    "dplus_backend_save_amplitude": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_char_p]),
+    "dplus_backend_stop": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None
@@ -121,6 +140,16 @@ def check(rc):
             raise DplusError(d.get("code", rc), d.get("message", msg))
         except ValueError:
             raise DplusError(rc, msg)
+
+
+def take_string(p, length=None):
+    """Bytes of a library-allocated buffer, then release it with dplus_free."""
+    if not p.value:
+        return b""
+    try:
+        return ctypes.string_at(p.value, length) if length is not None else ctypes.string_at(p.value)
+    finally:
+        lib().dplus_free(p.value)
 
 
 def dp(a):

@@ -15,6 +15,9 @@ class Grid:
     def __init__(self, grid_size, q_max, precision=L.PRECISION_MIXED):
         h = ctypes.c_void_p()
         L.check(L.lib().dplus_grid_create(int(grid_size), float(q_max), int(precision), ctypes.byref(h)))
+        self._init_from_handle(h)
+
+    def _init_from_handle(self, h):
         self._h = h
         gs, qm, st, n, pts, pr = (ctypes.c_int(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int(),
                                   ctypes.c_longlong(), ctypes.c_int())
@@ -162,6 +165,14 @@ def orientation_average(dock, q, method=L.OA_MC, seed=1, max_dirs=1 << 20, chunk
 
 def dock_grid(out, dock, scale=1.0):
     L.check(L.lib().dplus_dock_grid(out.handle, dock.n, dock.arr, float(scale)))
+
+
+def dock_grid_space_filling(out, dock, distances, angles_deg, reps, scale=1.0):
+    """GridSymmetry::calculateGrid: dock the leaves, times the lattice sum of the
+    basis built from distances (nm), angles (degrees) and repetitions."""
+    d, a, r = L.f64(distances), L.f64(angles_deg), L.f64(reps)
+    L.check(L.lib().dplus_dock_grid_space_filling(out.handle, dock.n, dock.arr, float(scale), L.dp(d), L.dp(a),
+                                                  L.dp(r)))
 
 
 def hydration_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05):

@@ -171,6 +171,17 @@ int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const
  * evaluated at every node of out, then CalculateSplines. */
 int dplus_dock_grid(dplus_grid *out, int nleaf, const dplus_leaf *leaves, double scale);
 
+/* ------------------------------------------------------------------ space-filling symmetry */
+/* GridSymmetry::calculateGrid (DefaultModels/Symmetries.cpp:348-431, GPU
+ * CalcSpaceFillingKernelJcb Backend/GPU/SymmetryGenJacob.cu:32-132):
+ * out(q) = scale * [sum over leaves/rotations/translations of A_leaf(R^T q) e^{i q.t}]
+ *          * prod_k sum_{n<N_k} e^{i n q.v_k},
+ * with the basis v_k built from distances[3] (nm), angles_deg[3] (alpha, beta,
+ * gamma) and reps[3] as GridSymmetry::PreCalculate does (Symmetries.cpp:59-97),
+ * including its argument checks (ERROR_INVALIDARGS). */
+int dplus_dock_grid_space_filling(dplus_grid *out, int nleaf, const dplus_leaf *leaves, double scale,
+                                  const double *distances, const double *angles_deg, const double *reps);
+
 /* ------------------------------------------------------------------ hydration layer */
 /* Voxelised hydration ("outer solvent") layer of PDBAmplitude::CalculateSolventSpace
  * (Backend/Backend/VoxelizedSolvent.cpp:4-194, box reduction :263-363): voxels of
@@ -179,6 +190,58 @@ int dplus_dock_grid(dplus_grid *out, int nleaf, const dplus_leaf *leaves, double
  * *nbox to the number found; call again with capacity >= *nbox when it is larger. */
 int dplus_hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness,
                           double vox, int capacity, int *nbox, int *dims, double *com);
+
+/* ------------------------------------------------------------------ amplitude files */
+/* AmpGridAmplitude::ReadAmplitudeFromFile (Backend/Backend/Amplitude.cpp:3724-3796):
+ * a ".amp" legacy stream (JacobianSphereGrid::ImportBinaryData, Grid.cpp:272-345)
+ * or an ".ampj" zip with stored members grid.dat / criticalinfo.json /
+ * header.json (PythonInterface/dplus/Amplitudes.py:705-750), uploaded to a new
+ * grid of the given precision and splined.  *header (optional) receives the
+ * file's header text; release it with dplus_free. */
+int dplus_amp_read(const char *path, int precision, dplus_grid **out, char **header);
+/* Amplitude::WriteAmplitudeToFile (Amplitude.cpp:3119-3190) for ".ampj" paths,
+ * JacobianSphereGrid::ExportBinaryData (Grid.cpp:847-894) for ".amp" paths. */
+int dplus_amp_write(dplus_grid *g, const char *path, const char *header);
+
+/* ------------------------------------------------------------------ backend */
+/* The job interface of PythonBackendWrapper (Backend/Backend/PythonBackendWrapper.h:15-46,
+ * bound by PythonInterface/dplus/wrappers/BackendWrapper.pxd), for the model
+ * types of the scattering hot path: "PDB" leaves, "AMP" leaves (amplitude
+ * files) and Manual Symmetry (",26") nodes, orientation averaged by Monte
+ * Carlo, VEGAS or adaptive Gauss-Kronrod.  Strings returned through char**
+ * are malloc'ed; release them with dplus_free. */
+typedef struct dplus_backend dplus_backend;
+void dplus_free(void *p);
+int dplus_backend_create(dplus_backend **out);
+void dplus_backend_destroy(dplus_backend *b);
+/* CheckCapabilities (PythonBackendWrapper.cpp:20-31): ERROR_NO_GPU (31) without a device */
+int dplus_backend_check_capabilities(dplus_backend *b, int check_tdr);
+/* GetAllModelMetadata (PythonBackendWrapper.cpp:33-47): the supported models, JSON */
+int dplus_backend_get_all_model_metadata(dplus_backend *b, char **out);
+/* InitializeCache(cacheDir) (PythonBackendWrapper.cpp:76-85) */
+int dplus_backend_initialize_cache(dplus_backend *b, const char *dir);
+/* StartGenerate(state, useGPU) (PythonBackendWrapper.cpp:87-106, BackendWrapper.cpp:313-372):
+ * state = CalculationInput args JSON {"state": {...}, "x": [q...], ...}.
+ * ERROR_ILLEGAL_JSON (9) for unparsable text, ERROR_INVALIDPARAMTREE (19) for a
+ * malformed tree, ERROR_JOBRUNNING (2) while a job runs; the job itself runs
+ * on a worker thread. */
+int dplus_backend_start_generate(dplus_backend *b, const char *state, int use_gpu);
+/* GetJobStatus (PythonBackendWrapper.cpp:130-143, Conversions.cpp:49-64):
+ * {"isRunning": bool, "progress": double, "code": int (-1 while running), "message": str} */
+int dplus_backend_get_job_status(dplus_backend *b, char **out);
+/* GetGenerateResults (PythonBackendWrapper.cpp:145-158, BackendWrapper.cpp:540-581):
+ * {"Headers": [{"ModelPtr": n, "Header": str}...], "Graph": [I(q)...]} */
+int dplus_backend_get_generate_results(dplus_backend *b, char **out);
+/* GetModelPtrs (PythonBackendWrapper.cpp:213-223): pointers of the last state's
+ * models; writes min(*n, capacity) entries and sets *n to the total */
+int dplus_backend_get_model_ptrs(dplus_backend *b, unsigned *ptrs, int capacity, int *n);
+/* GetAmplitude (PythonBackendWrapper.cpp:188-198): the model's grid as a legacy
+ * .amp byte stream of *len bytes */
+int dplus_backend_get_amplitude(dplus_backend *b, unsigned model_ptr, char **bytes, long long *len);
+/* SaveAmplitude (PythonBackendWrapper.cpp:176-186): .ampj (or .amp) file */
+int dplus_backend_save_amplitude(dplus_backend *b, unsigned model_ptr, const char *path);
+/* Stop (PythonBackendWrapper.cpp:225-235): the running job ends with ERROR_STOPPED (1) */
+int dplus_backend_stop(dplus_backend *b);
 
 /* ------------------------------------------------------------------ synthetic inputs */
 /* Seeded self-avoiding random walk (step, minimum separation in nm) confined to

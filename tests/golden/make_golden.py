@@ -1,6 +1,7 @@
 """Extract the reference's reviewer-test fixtures for the hot path into small
 JSON files under tests/golden/.  Reads data files only (PDB text, .state JSON
-and the <name>TestStandard.out mean/sigma curves) from the reference tree:
+and the <name>TestStandard.out mean/sigma curves) from the reference tree and
+stores them with the full state document:
 
   PythonInterface/tests/reviewer_tests/files_for_tests/generate/cpu/short/<name>/
 
@@ -59,6 +60,10 @@ def main():
             "atoms": read_pdb(os.path.join(d, pdb)),
             "q": q, "intensity": I, "sigma": s,
             "meta": meta,
+            # the whole state and PDB text, so the backend can be driven as the reference's tests do
+            "state": state,
+            "pdb_name": pdb,
+            "pdb_text": open(os.path.join(d, pdb)).read(),
         }
         with open(os.path.join(OUT, name + ".json"), "w") as f:
             json.dump(rec, f)

@@ -1,0 +1,322 @@
+"""The backend boundary on the GPU: states in the reference's JSON format run
+through dplus_backend_* (start_generate / job status / results, as
+EmbeddedLocalRunner.generate drives PythonBackendWrapper,
+PythonInterface/dplus/CalculationRunner.py:724-738) and are checked against
+  * the reference's own golden curves (reviewer-test fixtures), and
+  * the FP64 oracle on the same seeded inputs, with the model tree flattened
+    independently here in numpy (Euler X1Y2Z3 matrices, Amplitude.cpp:557-573,
+    composed as Flatten does, Amplitude.cpp:577-671).
+Tolerances: mixed precision 1e-4 relative, FP64 1e-6 relative on I(q)."""
+import copy
+import io
+import json
+import os
+import struct
+import sys
+import zipfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+WEIGHT = {"H": 1.008, "C": 12.011, "N": 14.007, "O": 15.999, "P": 30.97376, "S": 32.06}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def load(name):
+    return json.load(open(os.path.join(GOLDEN, name + ".json")))
+
+
+def write_pdb(path, els, xyz_nm):
+    with open(path, "w") as f:
+        for i, (e, p) in enumerate(zip(els, xyz_nm)):
+            x, y, z = 10.0 * np.asarray(p)
+            f.write(f"ATOM  {i + 1:5d}  {e:<3s} ALA A   1    {x:8.3f}{y:8.3f}{z:8.3f}  1.00  0.00          {e:>2s}\n")
+        f.write("END\n")
+
+
+def run(args):
+    from dplus_mi355x.backend import EmbeddedLocalRunner
+    r = EmbeddedLocalRunner()
+    return r, r.generate(args)
+
+
+def domain(gs, method="Monte Carlo (Mersenne Twister)", iters=8192, eps=0.0, use_grid=True, precision="mixed",
+           seed=3):
+    return {"GridSize": gs, "qMax": 5, "OrientationMethod": method, "OrientationIterations": iters,
+            "Convergence": eps, "UseGrid": use_grid, "Precision": precision, "Seed": seed}
+
+
+def pdb_model(path, ptr, extra=(1, 334, 1, 0.05, 0.14, 0.28, 0, 0, 0, 4), loc=(0, 0, 0, 0, 0, 0), use_grid=True,
+              centered=True):
+    return {"Type": "PDB", "ModelPtr": ptr, "Filename": path, "Centered": centered, "ExtraParameters": list(extra),
+            "Use_Grid": use_grid, "Location": dict(zip(("x", "y", "z", "alpha", "beta", "gamma"), loc))}
+
+
+def sym_model(kind, rows, children, ptr, scale=1.0, loc=(0, 0, 0, 0, 0, 0), use_grid=True):
+    return {"Type": kind, "ModelPtr": ptr, "Parameters": rows, "ExtraParameters": [scale], "Use_Grid": use_grid,
+            "Children": children, "Location": dict(zip(("x", "y", "z", "alpha", "beta", "gamma"), loc))}
+
+
+def state(dom, models, scale=1.0, constant=0.0, pop_size=1.0):
+    return {"DomainPreferences": dom,
+            "Domain": {"Scale": scale, "Constant": constant, "ModelPtr": 1,
+                       "Populations": [{"ModelPtr": 2, "PopulationSize": pop_size, "Models": models}]}}
+
+
+# ---------------------------------------------------------------- independent flattening
+def euler(a, b, c):
+    """R = Rx(a) Ry(b) Rz(c) (Amplitude.cpp:557-573)."""
+    ca, sa, cb, sb, cc, sc = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+    Rx = np.array([[1, 0, 0], [0, ca, -sa], [0, sa, ca]])
+    Ry = np.array([[cb, 0, sb], [0, 1, 0], [-sb, 0, cb]])
+    Rz = np.array([[cc, -sc, 0], [sc, cc, 0], [0, 0, 1]])
+    return Rx @ Ry @ Rz
+
+
+def angles(R):
+    """(a, b, c) with euler(a, b, c) == R."""
+    return np.arctan2(-R[1, 2], R[2, 2]), np.arcsin(np.clip(R[0, 2], -1, 1)), np.arctan2(-R[0, 1], R[0, 0])
+
+
+def lattice_points(rows):
+    (da, al, na), (db, be, nb), (dc, ga, nc) = rows
+    al, be, ga = np.radians([al, be, ga])
+    cat = (np.cos(al) - np.cos(be) * np.cos(ga)) / (np.sin(be) * np.sin(ga))
+    a = np.array([da, 0, 0])
+    b = db * np.array([np.cos(ga), np.sin(ga), 0])
+    c = dc * np.array([np.cos(be), np.sin(be) * cat, np.sin(be) * np.sin(np.arccos(cat))])
+    return [i * a + j * b + k * c for i in range(int(na)) for j in range(int(nb)) for k in range(int(nc))]
+
+
+def flatten(m, R=np.eye(3), t=np.zeros(3), scale=1.0, out=None):
+    """[(leaf model, R, t, scale)] of a model tree."""
+    out = [] if out is None else out
+    L = m["Location"]
+    Rn = R @ euler(L["alpha"], L["beta"], L["gamma"])
+    tn = R @ np.array([L["x"], L["y"], L["z"]]) + t
+    if m["Type"] == "PDB":
+        out.append((m, Rn, tn, scale))
+        return out
+    s = m["ExtraParameters"][0]
+    if m["Type"] == ",26":
+        subs = [(euler(*np.radians(r[3:6])), np.array(r[:3], float)) for r in m["Parameters"]]
+    else:
+        subs = [(np.eye(3), p) for p in lattice_points(m["Parameters"])]
+    for Rk, tk in subs:
+        for c in m["Children"]:
+            flatten(c, Rn @ Rk, Rn @ tk + tn, scale * s, out)
+    return out
+
+
+def oracle_leaf(O, els, xyz, gs, qmax, extra=(1, 334, 1, 0.05, 0.14, 0.28, 0, 0, 0, 4)):
+    """Centred atoms -> FP64 oracle grid and splines of one PDB leaf."""
+    import dplus_mi355x.elements as E
+    w = np.array([WEIGHT[e] for e in els])
+    xyz = np.asarray(xyz) - (w[:, None] * xyz).sum(0) / w.sum()
+    coef9, rad, ion, _ = E.ion_tables(els)
+    sol = extra[1]
+    flags = O.CALC_ATOMIC_FORMFACTORS | (O.CALC_DUMMY_SOLVENT if sol else 0)
+    ions = O.Ions(coef9, rad, np.bincount(ion, minlength=len(coef9)), sol, extra[2], flags)
+    data = O.grid_fill_pdb(gs, qmax, xyz, ion, ions, extra[0])
+    return data, O.grid_splines(gs, data)
+
+
+def docked_of(flat, leafdata, gs, qmax):
+    leaves = []
+    for m, R, t, s in flat:
+        data, co = leafdata[m["Filename"]]
+        leaves.append((gs, qmax, data, co, [list(angles(R)) + [s]], [1], [t]))
+    return leaves
+
+
+@pytest.fixture(scope="module")
+def subunit(tmp_path_factory):
+    import dplus_mi355x.synth as S
+    xyz, el = S.walk_structure(11, 80, (0.9, 0.9, 0.9))
+    els = [str(e) for e in el]
+    d = tmp_path_factory.mktemp("pdb")
+    path = str(d / "subunit.pdb")
+    xyz = np.round(np.asarray(xyz) * 10.0, 3) / 10.0  # as written in the PDB (0.001 Angstrom)
+    write_pdb(path, els, xyz)
+    return dict(path=path, els=els, xyz=xyz, dir=d)
+
+
+# ---------------------------------------------------------------- golden curves through the backend
+@pytest.mark.parametrize("name", [f"{e}_explicit_{s}" for e in "hcno" for s in ("Vacuum", "SS334")])
+def test_single_atom_state_matches_golden(name, tmp_path):
+    say(f"[case] backend golden {name}")
+    rec = load(name)
+    path = str(tmp_path / rec["pdb_name"])
+    open(path, "w").write(rec["pdb_text"])
+    st = copy.deepcopy(rec["state"])
+    st["Domain"]["Populations"][0]["Models"][0]["Filename"] = path
+    _, res = run({"state": st, "x": rec["q"]})
+    I, ref = np.array(res["Graph"]), np.array(rec["intensity"])
+    assert len(I) == len(ref)
+    assert np.max(np.abs(I - ref)) <= 2e-5 * ref.max()
+
+
+@pytest.mark.parametrize("name", ["points_explicit_Vacuum", "points_explicit_SS334"])
+def test_points_state_sigma_band(name, tmp_path, oracle):
+    """The reference's sigma-band rule (test_generate_results.py:149-177)."""
+    say(f"[case] backend golden {name}")
+    rec = load(name)
+    path = str(tmp_path / rec["pdb_name"])
+    open(path, "w").write(rec["pdb_text"])
+    st = copy.deepcopy(rec["state"])
+    st["Domain"]["Populations"][0]["Models"][0]["Filename"] = path
+    _, res = run({"state": st, "x": rec["q"]})
+    I, ref, sig = np.array(res["Graph"]), np.array(rec["intensity"]), np.array(rec["sigma"])
+    d = np.abs(I - ref)
+    m = d > 256 * np.spacing(np.abs(ref))
+    sig = np.where(sig > 0, sig, np.inf)
+    frac = [np.mean((d > k * sig) & m) for k in (1, 2, 3)]
+    assert frac[0] < 0.5 and frac[1] < 0.3 and frac[2] < 0.1, frac
+
+
+# ---------------------------------------------------------------- model trees against the oracle
+def _tree(kind, subunit, use_grid):
+    p = subunit["path"]
+    inner = sym_model(",26", [[0.3, -0.2, 0.5, 90, 0, 0]], [pdb_model(p, 12, loc=(0.1, 0, 0, 0.2, 0, 0))], 11,
+                      use_grid=use_grid)
+    if kind == ",26":
+        rows = [[0, 0, 0, 0, 0, 0], [2.0, 0, 0, 0, 0, 30], [0, 2.5, 1.0, 15, 40, 0], [1.5, 1.5, -1.0, 0, 90, 60]]
+    else:
+        rows = [[2.0, 80, 2], [2.2, 90, 2], [3.0, 75, 1]]
+    return sym_model(kind, rows, [inner], 10, scale=0.7, loc=(0.2, 0.1, -0.3, 0.1, 0.2, 0.3), use_grid=use_grid)
+
+
+@pytest.mark.parametrize("kind", [",26", ",25"])
+@pytest.mark.parametrize("use_grid", [False, True])
+@pytest.mark.parametrize("precision", ["mixed", "FP64"])
+def test_symmetry_tree_matches_oracle(kind, use_grid, precision, subunit, oracle):
+    say(f"[case] backend symmetry {kind} use_grid={use_grid} precision={precision}")
+    gs, q = 24, np.linspace(0.0, 4.0, 41)
+    q[0] = 0.01
+    top = _tree(kind, subunit, use_grid)
+    st = state(domain(gs, iters=4096, use_grid=use_grid, precision=precision), [top], scale=2.0, constant=0.5,
+               pop_size=3.0)
+    _, res = run({"state": st, "x": list(q)})
+    I = np.array(res["Graph"])
+    qmax = q[-1]
+    leaf = {subunit["path"]: oracle_leaf(oracle, subunit["els"], subunit["xyz"], gs, qmax)}
+    if not use_grid:
+        dock = oracle.Docked(docked_of(flatten(top), leaf, gs, qmax))
+        ref, _ = oracle.mc_oa(dock, q, 3, 4096, 4096, 0.0)
+    else:
+        # grid regime: the inner manual symmetry is docked into a grid, then the
+        # outer symmetry docks that grid (with the lattice for ',25'), then MC
+        inner = top["Children"][0]
+        fi = flatten(dict(inner, Location=dict(x=0, y=0, z=0, alpha=0, beta=0, gamma=0)))
+        g_in = oracle.dock_grid(gs, qmax, oracle.Docked(docked_of(fi, leaf, gs, qmax)), 1.0)
+        c_in = oracle.grid_splines(gs, g_in)
+        L = inner["Location"]
+        Rl = euler(L["alpha"], L["beta"], L["gamma"])
+        tl = np.array([L["x"], L["y"], L["z"]])
+        if kind == ",26":
+            subs = [(euler(*np.radians(r[3:6])), np.array(r[:3], float)) for r in top["Parameters"]]
+        else:
+            subs = [(np.eye(3), p) for p in lattice_points(top["Parameters"])]
+        leaves = [(gs, qmax, g_in, c_in, [list(angles(Rk @ Rl)) + [top["ExtraParameters"][0]]], [1], [Rk @ tl + tk])
+                  for Rk, tk in subs]
+        g_top = oracle.dock_grid(gs, qmax, oracle.Docked(leaves), 1.0)
+        c_top = oracle.grid_splines(gs, g_top)
+        T = top["Location"]
+        Rt = euler(T["alpha"], T["beta"], T["gamma"])
+        dock = oracle.Docked([(gs, qmax, g_top, c_top, [list(angles(Rt)) + [1.0]], [1], [[T["x"], T["y"], T["z"]]])])
+        ref, _ = oracle.mc_oa(dock, q, 3, 4096, 4096, 0.0)
+    ref = 2.0 * 3.0 * ref + 0.5
+    tol = 1e-6 if precision == "FP64" else 1e-4
+    err = float(np.max(np.abs(I - ref) / np.abs(ref)))
+    say(f"    max rel err {err:.2e}")
+    assert err < tol
+
+
+def test_vegas_and_gk_states_match_oracle(subunit, oracle):
+    say("[case] backend VEGAS and GK on a single PDB")
+    gs, q = 20, np.linspace(0.05, 4.0, 30)
+    leaf = oracle_leaf(oracle, subunit["els"], subunit["xyz"], gs, q[-1])
+    dock = oracle.Docked.single(gs, q[-1], *leaf)
+    m = [pdb_model(subunit["path"], 5)]
+    _, res = run({"state": state(domain(gs, "Adaptive (VEGAS) Monte Carlo", iters=3 * 4096, precision="FP64",
+                                        seed=9), m), "x": list(q)})
+    ref, _ = oracle.vegas_oa(dock, q, 9, 3, 4)
+    assert np.max(np.abs(np.array(res["Graph"]) - ref) / ref) < 1e-6
+    _, res = run({"state": state(domain(gs, "Adaptive Gauss Kronrod", iters=6, eps=1e-4, precision="FP64"), m),
+                  "x": list(q)})
+    ref, _ = oracle.gk_oa(dock, q, 1e-4, 6)
+    assert np.max(np.abs(np.array(res["Graph"]) - ref) / ref) < 1e-6
+
+
+# ---------------------------------------------------------------- amplitudes, headers, model pointers
+def parse_legacy_amp(b):
+    """JacobianSphereGrid legacy stream (Grid.cpp:847-894), read independently."""
+    assert b[:2] == b"#@"
+    (headlen,) = struct.unpack("<I", b[2:6])
+    f = io.BytesIO(b)
+    f.seek(headlen)
+    while f.read(1) == b"#":
+        f.readline()
+    f.seek(f.tell() - 1)
+    head = [f.readline() for _ in range(4)]
+    version, nbytes, actual, extras = (int(x) for x in head)
+    (step,) = struct.unpack("<d", f.read(8))
+    return version, nbytes, actual, extras, step, np.frombuffer(f.read(), dtype="<f8")
+
+
+def test_amplitude_files_and_amp_leaf(subunit, dplus):
+    say("[case] backend amplitude export, .ampj/.amp round trip and AMP leaf")
+    from dplus_mi355x.wrappers import amp_read
+    gs, q = 20, list(np.linspace(0.05, 4.0, 30))
+    m = [pdb_model(subunit["path"], 7)]
+    r, res = run({"state": state(domain(gs, precision="FP64"), m), "x": q})
+    assert [h["ModelPtr"] for h in res["Headers"]] == [7]
+    assert 7 in r.get_model_ptrs()
+    raw = r.get_amp(7)
+    version, nbytes, actual, extras, step, vals = parse_legacy_amp(raw)
+    assert (version, nbytes, actual, extras) == (13, 16, gs // 2 + 3, 3)
+    assert step == pytest.approx(q[-1] / (gs // 2), rel=1e-15)
+    path = str(subunit["dir"] / "model7.ampj")
+    r.save_amp(7, path)
+    z = zipfile.ZipFile(path)
+    dat = np.frombuffer(z.read("grid.dat"), dtype="<f8")
+    info = json.loads(z.read("criticalinfo.json"))
+    assert info["gridSize"] == gs and info["qmax"] == pytest.approx(q[-1])
+    assert np.array_equal(dat, vals)
+    g, _ = amp_read(path, dplus.PRECISION_FP64)
+    assert np.array_equal(g.get_data(), dat)
+    legacy = str(subunit["dir"] / "model7.amp")
+    open(legacy, "wb").write(raw)
+    g2, hdr = amp_read(legacy, dplus.PRECISION_FP64)
+    assert np.array_equal(g2.get_data(), dat) and "PDB" in hdr
+    # the saved grid as an AMP leaf reproduces the PDB state's curve exactly (FP64 storage)
+    amp = {"Type": "AMP", "ModelPtr": 8, "Filename": path, "ExtraParameters": [1.0], "Use_Grid": True,
+           "Location": dict(x=0, y=0, z=0, alpha=0, beta=0, gamma=0)}
+    _, res2 = run({"state": state(domain(gs, precision="FP64"), [amp]), "x": q})
+    assert np.array_equal(np.array(res2["Graph"]), np.array(res["Graph"]))
+
+
+def test_errors_surface_in_status(subunit, tmp_path):
+    say("[case] backend job errors")
+    from dplus_mi355x.backend import EmbeddedLocalRunner, BackendError
+    r = EmbeddedLocalRunner()
+    bad = [pdb_model(str(tmp_path / "missing.pdb"), 3)]
+    r.generate_async({"state": state(domain(20), bad), "x": [0.1, 1.0]})
+    st = r.wait()
+    assert st["isRunning"] is False and st["code"] == 6  # ERROR_FILENOTFOUND
+    with pytest.raises(BackendError):
+        r.wrapper.get_generate_results()
+    sf = [sym_model(",25", [[2, 0, 2], [2, 90, 2], [2, 90, 1]], [pdb_model(subunit["path"], 4)], 9)]
+    r.generate_async({"state": state(domain(20), sf), "x": [0.1, 1.0]})
+    st = r.wait()
+    assert st["code"] == 21 and "zero or 180" in st["message"]  # GridSymmetry::PreCalculate
+    with pytest.raises(BackendError) as e:
+        r.wrapper.start_generate(json.dumps({"state": state(domain(20), []), "x": [1.0]}), True)
+    assert e.value.error_code == 21  # no models (BackendWrapper.cpp:351-354)
