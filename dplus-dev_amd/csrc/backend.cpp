@@ -43,50 +43,6 @@ Grid *amp_read(const std::string &path, int precision, std::string &header, hipS
 
 namespace {
 
-// ------------------------------------------------------------------ PDB text
-struct AtomSet {
-    std::vector<double> xyz;  // nm
-    std::vector<int> element;
-};
-
-// ATOM/HETATM records of the first model: coordinates (columns 31-54, Angstrom
-// -> nm, PDBReaderLib.cpp:872-874) and the element symbol (columns 77-78, else
-// the first letter of the atom name)
-AtomSet read_pdb(const std::string &path) {
-    std::ifstream f(path);
-    if (!f) throw Error(kErrFileNotFound, "cannot open PDB file " + path);
-    AtomSet a;
-    std::string line;
-    while (std::getline(f, line)) {
-        if (line.compare(0, 6, "ENDMDL") == 0) break;
-        if (line.compare(0, 4, "ATOM") != 0 && line.compare(0, 6, "HETATM") != 0) continue;
-        if (line.size() < 54) continue;
-        double x = atof(line.substr(30, 8).c_str()), y = atof(line.substr(38, 8).c_str()),
-               z = atof(line.substr(46, 8).c_str());
-        std::string el;
-        if (line.size() >= 78) el = line.substr(76, 2);
-        el.erase(0, el.find_first_not_of(' '));
-        el.erase(el.find_last_not_of(" \r") + 1);
-        if (el.empty()) {
-            std::string name = line.substr(12, 4);
-            for (char c : name)
-                if (isalpha((unsigned char)c)) {
-                    el = std::string(1, c);
-                    break;
-                }
-        }
-        if (el.size() == 2) el[1] = (char)tolower(el[1]);
-        const int e = element_index(el);
-        if (e < 0) throw Error(kErrUnimplemented, "element \"" + el + "\" has no X-ray form factor in this build (" + path + ")");
-        a.xyz.push_back(x / 10.0);
-        a.xyz.push_back(y / 10.0);
-        a.xyz.push_back(z / 10.0);
-        a.element.push_back(e);
-    }
-    if (a.element.empty()) throw Error(kErrInvalidArgs, "no atoms in PDB file " + path);
-    return a;
-}
-
 // ------------------------------------------------------------------ model tree
 struct ModelNode {
     std::string type, name, filename;
@@ -292,7 +248,7 @@ struct Backend {
     std::map<unsigned, Grid *> model_grid;  // grids of this job's models (get/save amplitude)
     std::string cache_dir;
     // device-resident caches across jobs
-    std::map<std::string, AtomSet> pdb_cache;
+    std::map<std::string, PdbAtoms> pdb_cache;
     std::map<std::string, std::unique_ptr<Structure>> structures;
     std::map<std::string, std::unique_ptr<Grid>> leaf_grids;
     std::map<std::string, std::pair<std::unique_ptr<Grid>, std::string>> amp_grids;
@@ -321,13 +277,13 @@ struct Backend {
         if (sit == structures.end()) {
             auto pit = pdb_cache.find(fk);
             if (pit == pdb_cache.end()) pit = pdb_cache.emplace(fk, read_pdb(n.filename)).first;
-            const AtomSet &A = pit->second;
-            const int natoms = (int)A.element.size();
+            const PdbAtoms &A = pit->second;
+            const int natoms = (int)A.species.size();
             std::vector<double> xyz = A.xyz;
             if (n.centered) {  // moveCOMToOrigin (PDBReaderLib.cpp:271-292)
                 double c[3] = {0, 0, 0}, w = 0;
                 for (int a = 0; a < natoms; a++) {
-                    const double wt = kElements[A.element[a]].weight;
+                    const double wt = kElements[A.species[a]].weight;
                     for (int k = 0; k < 3; k++) c[k] += xyz[3 * a + k] * wt;
                     w += wt;
                 }
@@ -339,7 +295,7 @@ struct Backend {
             std::vector<double> coef9, rad;
             for (int e = 0; e < kNumElements; e++)
                 for (int a = 0; a < natoms; a++)
-                    if (A.element[a] == e) {
+                    if (A.species[a] == e) {
                         row[e] = (int)rad.size();
                         coef9.insert(coef9.end(), kElements[e].coef9, kElements[e].coef9 + 9);
                         rad.push_back(kElements[e].svg_radius);
@@ -348,8 +304,8 @@ struct Backend {
             std::vector<int> ion(natoms);
             std::vector<double> arad(natoms);
             for (int a = 0; a < natoms; a++) {
-                ion[a] = row[A.element[a]];
-                arad[a] = kElements[A.element[a]].svg_radius;
+                ion[a] = row[A.species[a]];
+                arad[a] = kElements[A.species[a]].svg_radius;
             }
             // ExtraParameters (PDBAmplitude.cpp:1205-1214)
             const double scale = p[0], sol_ed = p[1], c1 = p[2], vox = p[3], probe = p[4], thick = p[5], out_ed = p[6];

@@ -12,7 +12,8 @@ import os
 
 REF = "/root/reference/PythonInterface/tests/reviewer_tests/files_for_tests/generate/cpu/short"
 OUT = os.path.dirname(os.path.abspath(__file__))
-CASES = [f"{e}_explicit_{s}" for e in ("h", "c", "n", "o", "points") for s in ("Vacuum", "SS334")]
+CASES = ([f"{e}_explicit_{s}" for e in ("h", "c", "n", "o", "points") for s in ("Vacuum", "SS334")] +
+         [f"{g}_implicit_{s}" for g in ("ch", "ch2", "ch3", "nh", "nh2", "nh3", "oh") for s in ("Vacuum", "SS334")])
 
 
 def read_pdb(path):
@@ -70,5 +71,42 @@ def main():
         print(name, len(q), rec["atoms"][:2])
 
 
+GPU_REF = "/root/reference/PythonInterface/tests/reviewer_tests/files_for_tests/generate/gpu/short"
+SYM_CASES = [f"Man_Symm_Impl_No_Charges_wo_Hydr_{m}" for m in ("MC", "VEGAS", "Hybrid_MC", "Hybrid_VEGAS", "Ga_Kr")] + \
+            [f"Space_fill_Symm_Impl_No_Charges_wo_Hydr_{m}" for m in ("MC", "Hybrid_MC", "Hybrid_VEGAS", "Ga_Kr",
+                                                                       "Hybrid_Ga_Kr")]
+
+
+def symmetry_cases():
+    """Tubulin-dimer symmetry fixtures (manual and space-filling, grid and
+    hybrid, MC/VEGAS/GK): the state, the mean/sigma curve, and the one PDB they
+    share, gzip-compressed, under tests/golden/symmetry/."""
+    import gzip
+    import hashlib
+    out = os.path.join(OUT, "symmetry")
+    os.makedirs(out, exist_ok=True)
+    pdb_digest = None
+    for name in SYM_CASES:
+        d = os.path.join(GPU_REF, name)
+        state = json.load(open(os.path.join(d, name + ".state")))
+        pdb = [f for f in os.listdir(d) if f.endswith(".pdb")][0]
+        raw = open(os.path.join(d, pdb), "rb").read()
+        digest = hashlib.sha256(raw).hexdigest()
+        if pdb_digest is None:
+            with gzip.GzipFile(os.path.join(out, "tubulin_dimer_no_charges.pdb.gz"), "wb", mtime=0) as f:
+                f.write(raw)
+            pdb_digest = digest
+        assert digest == pdb_digest, name
+        std = [f for f in os.listdir(d) if f.startswith(name + "TestStandard")][0]
+        q, I, s, meta = read_standard(os.path.join(d, std))
+        rec = {"name": name, "source": f"reviewer_tests/files_for_tests/generate/gpu/short/{name}",
+               "pdb": "tubulin_dimer_no_charges.pdb.gz", "pdb_sha256": digest, "state": state,
+               "q": q, "intensity": I, "sigma": s, "meta": meta}
+        with open(os.path.join(out, name + ".json"), "w") as f:
+            json.dump(rec, f)
+        print(name, len(q))
+
+
 if __name__ == "__main__":
     main()
+    symmetry_cases()

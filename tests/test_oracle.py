@@ -199,6 +199,23 @@ def test_single_atom_fixture(name):
     assert np.max(np.abs(I - ref)) <= 2e-5 * np.max(ref)
 
 
+@pytest.mark.parametrize("name", [f"{g}_implicit_{s}" for g in ("ch", "ch2", "ch3", "nh", "nh2", "nh3", "oh")
+                                  for s in ("Vacuum", "SS334")])
+def test_implicit_group_fixture(name):
+    """One heavy atom in a residue without hydrogens: the reference relabels it
+    as an implicit-hydrogen group (PDBReaderLib.cpp:1533-1758) whose fitted
+    form factor and dummy-atom radius (:2479-2486, :229-264) give I(q) = f~_group(q)^2."""
+    rec = _load(name)
+    rec = dict(rec, atoms=[[name.split("_")[0].upper()] + list(a[1:]) for a in rec["atoms"]])
+    ions, ion, xyz, scale = _ions_for(rec)
+    gs, qmax = rec["domain"]["GridSize"], rec["domain"]["qMax"]
+    data = O.grid_fill_pdb(gs, qmax, xyz, ion, ions, scale)
+    co = O.grid_splines(gs, data)
+    I, _ = O.mc_oa(O.Docked.single(gs, qmax, data, co), np.array(rec["q"]), seed=1, max_dirs=256, chunk=256)
+    ref = np.array(rec["intensity"])
+    assert np.max(np.abs(I - ref)) <= 2e-5 * np.max(ref)
+
+
 @pytest.mark.parametrize("name", ["points_explicit_Vacuum", "points_explicit_SS334"])
 def test_points_fixture_sigma_band(name):
     """Six atoms: the reference curve is a mean of 10 MC runs with its sigma.
