@@ -35,7 +35,7 @@
 namespace dplus {
 
 void hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness, double vox,
-                     std::vector<int> &dims, std::vector<double> &com);
+                     bool fill_holes, std::vector<int> &dims, std::vector<double> &com);
 void euler_rt(double a, double b, double c, double m[9]);
 std::string amp_legacy_bytes(Grid &g, const std::string &header, hipStream_t s);
 void amp_write(Grid &g, const std::string &path, const std::string &title, hipStream_t s);
@@ -279,17 +279,24 @@ struct Backend {
             if (pit == pdb_cache.end()) pit = pdb_cache.emplace(fk, read_pdb(n.filename)).first;
             const PdbAtoms &A = pit->second;
             const int natoms = (int)A.species.size();
-            std::vector<double> xyz = A.xyz;
-            if (n.centered) {  // moveCOMToOrigin (PDBReaderLib.cpp:271-292)
-                double c[3] = {0, 0, 0}, w = 0;
+            // the reference's PDB reader holds coordinates and weights in float
+            // (PDBReaderOb<float>) and centres in float, in file order
+            // (moveCOMToOrigin, PDBReaderLib.cpp:271-292); the same arithmetic here
+            // keeps the voxelised layer on the reference's voxels
+            std::vector<float> xf(3 * (size_t)natoms);
+            for (size_t i = 0; i < xf.size(); i++) xf[i] = (float)A.xyz[i];
+            if (n.centered) {
+                float c[3] = {0.f, 0.f, 0.f}, w = 0.f;
                 for (int a = 0; a < natoms; a++) {
-                    const double wt = kElements[A.species[a]].weight;
-                    for (int k = 0; k < 3; k++) c[k] += xyz[3 * a + k] * wt;
+                    const float wt = (float)kElements[A.species[a]].weight;
+                    for (int k = 0; k < 3; k++) c[k] += xf[3 * a + k] * wt;
                     w += wt;
                 }
+                for (int k = 0; k < 3; k++) c[k] /= w;
                 for (int a = 0; a < natoms; a++)
-                    for (int k = 0; k < 3; k++) xyz[3 * a + k] -= c[k] / w;
+                    for (int k = 0; k < 3; k++) xf[3 * a + k] -= c[k];
             }
+            std::vector<double> xyz(xf.begin(), xf.end());
             // ion table: elements present, in table order (sorted ion indices)
             std::vector<int> row(kNumElements, -1);
             std::vector<double> coef9, rad;
@@ -311,9 +318,11 @@ struct Backend {
             const double scale = p[0], sol_ed = p[1], c1 = p[2], vox = p[3], probe = p[4], thick = p[5], out_ed = p[6];
             const bool solvent_only = std::fabs(p[8] - 1.0) < 1e-6;
             const int rad_type = (int)(p[9] + 0.1);
-            if (sol_ed != 0.0 && (rad_type == 1 || rad_type == 2 || rad_type == 3 || rad_type == 5))
-                throw Error(kErrUnimplemented, "voxelised excluded volume (radius types 1,2,3,5) is not on this path; "
-                                               "use radius type 4 (dummy atoms)");
+            if ((sol_ed != 0.0 || out_ed != 0.0) && (rad_type == 1 || rad_type == 2 || rad_type == 3 || rad_type == 5))
+                throw Error(kErrUnimplemented, "voxelised excluded volume and van der Waals / empirical / calculated "
+                                               "radii (radius types 1, 2, 3, 5) are not on this path; use radius type "
+                                               "4 (dummy atoms)");
+            const bool fill_holes = std::fabs(p[7] - 1.0) < 1e-6;
             dplus_pdb_desc d{};
             d.natoms = natoms;
             d.xyz = xyz.data();
@@ -329,7 +338,7 @@ struct Backend {
             std::vector<int> bdims;
             std::vector<double> bcom;
             if (out_ed != 0.0 && rad_type >= 1 && rad_type <= 5) {
-                hydration_boxes(natoms, xyz.data(), arad.data(), probe, thick, vox, bdims, bcom);
+                hydration_boxes(natoms, xyz.data(), arad.data(), probe, thick, vox, fill_holes, bdims, bcom);
                 d.nbox = (int)(bdims.size() / 3);
                 d.box_dims = bdims.data();
                 d.box_com = bcom.data();

@@ -1,14 +1,34 @@
-// hydration.cpp -- voxelised hydration layer reduced to solvent boxes.
+// hydration.cpp -- voxelised hydration ("outer solvent") layer reduced to boxes.
 //
-// Follows the steps of PDBAmplitude::CalculateSolventSpace
-// (Backend/Backend/VoxelizedSolvent.cpp:4-194): mark voxels inside atoms,
-// mark the probe shell around them, flood-fill the bulk solvent from a corner
-// of the box, take the bulk voxels within the solvation thickness of the
-// probe shell as the hydration layer, and reduce the layer to boxes
-// (ReduceSolventSpaceToIrregularBoxes, :263-363).  Distances are measured
-// from voxel centres to atom surfaces (|c - r_atom| - radius).  The box
-// decomposition is a greedy 3-D meshing (grow z, then y, then x); any
-// decomposition gives the same Fourier transform, fewer boxes cost less.
+// Restates PDBAmplitude::CalculateSolventSpace (Backend/Backend/VoxelizedSolvent.cpp:4-194)
+// step by step on a byte grid of voxel colours:
+//   range     FindPDBRanges (PDBAmplitude.cpp:710-750): atom extents +- radius,
+//             widened by max(probe, thickness) + 2 voxels and rounded to whole
+//             voxels, in the reference's float/double arithmetic; the z extent
+//             is padded to a multiple of 16 as SolventSpace::allocate does
+//             (Amplitude.cpp:4051-4069);
+//   atoms     voxels whose corner point lies within an atom radius -> ATOM
+//             (MarkVoxelsNeighboringAtoms, VoxelizedSolvent.cpp:365-396);
+//   probe     empty voxels within radius + probe -> PROBE;
+//   bulk      26-connected flood fill from voxel (0,0,0) through empty voxels
+//             -> BULK; with fill-holes the remaining empty voxels (cavities)
+//             become ATOM, otherwise BULK (Floodfill3D, :398-441);
+//   layer     PROBE voxels within round(probe/voxel) voxels of a BULK voxel
+//             that touches PROBE -> LAYER, then the layer grown into BULK (or
+//             shrunk back) by round(|thickness - probe| / voxel) voxels
+//             (MarkLayerOfSolvent, :196-261); leftover PROBE -> ATOM;
+//   corner    the reference's slice marking of y = 1..1/voxel, z = 1..4 for
+//             every x (:104-115) is kept, so a layer voxel there is dropped
+//             exactly as the reference drops it;
+//   boxes     LAYER voxels reduced to boxes.  Voxel (h,k,l) is the cube of
+//             side `vox` centred at (xMin + h vox, yMin + k vox, zMin + l vox),
+//             the reference's box-centre convention
+//             (ReduceSolventSpaceToIrregularBoxes, :263-362).  The partition is
+//             a greedy 3-D meshing (grow z, then y, then x); any partition of
+//             the same voxels has the same Fourier transform.
+// Every marking step only ever turns voxels of one colour into another, so the
+// per-atom and per-seed loops run in parallel with relaxed byte stores and give
+// the same grid as the reference's sequential loops.
 
 #include <algorithm>
 #include <cmath>
@@ -20,132 +40,190 @@
 
 namespace dplus {
 
+namespace {
+
+enum : char { kEmpty = 0, kAtom = 1, kProbe = 2, kBulk = 3, kLayer = 4, kCorner = 8 };
+
+struct Space {
+    long long X = 0, Y = 0, Z = 0;  // Z padded to a multiple of 16
+    std::vector<char> v;
+    char *at(long long i, long long j, long long k) { return &v[(size_t)((i * Y + j) * Z + k)]; }
+    char get(long long i, long long j, long long k) { return __atomic_load_n(at(i, j, k), __ATOMIC_RELAXED); }
+    void put(long long i, long long j, long long k, char c) { __atomic_store_n(at(i, j, k), c, __ATOMIC_RELAXED); }
+};
+
+struct Frame {
+    double xmin, ymin, zmin, vox;
+};
+
+// MarkVoxelsNeighboringAtoms: voxels of colour `from` whose corner point lies
+// within rad + r_sol of an atom become `to`
+void mark_atoms(Space &S, const Frame &F, int n, const float *x, const float *y, const float *z, const float *r,
+                double r_sol, char from, char to) {
+    parallel_for(n, [&](long long a) {
+        const double dd = (double)r[a] + r_sol;
+        const int dist = 2 + (int)(dd / F.vox);
+        const int xc = (int)((x[a] - F.xmin) / F.vox), yc = (int)((y[a] - F.ymin) / F.vox),
+                  zc = (int)((z[a] - F.zmin) / F.vox);
+        for (int h = std::max(0, xc - dist); h < std::min(xc + dist, (int)S.X); h++)
+            for (int k = std::max(0, yc - dist); k < std::min(yc + dist, (int)S.Y); k++)
+                for (int l = std::max(0, zc - dist); l < std::min(zc + dist, (int)S.Z); l++) {
+                    if (S.get(h, k, l) != from) continue;
+                    const double dx = F.xmin + (double)h * F.vox - x[a], dy = F.ymin + (double)k * F.vox - y[a],
+                                 dz = F.zmin + (double)l * F.vox - z[a];
+                    if (std::sqrt(dx * dx + dy * dy + dz * dz) <= dd) S.put(h, k, l, to);
+                }
+    });
+}
+
+// MarkLayerOfSolvent: seeds are interior voxels of colour `type` with a
+// `neighbor` among their 27 neighbours; voxels of colour `from` within
+// round(radius / vox) voxels (Euclidean, in voxel units) of a seed become `to`
+void mark_layer(Space &S, double radius, double vox, char type, char neighbor, char to, char from) {
+    const int d = (int)(0.5 + radius / vox);
+    std::vector<std::vector<long long>> seeds(S.X);
+    parallel_for(S.X, [&](long long i) {
+        if (i < 1 || i >= S.X - 1) return;
+        for (long long j = 1; j < S.Y - 1; j++)
+            for (long long k = 1; k < S.Z - 1; k++) {
+                if (*S.at(i, j, k) != type) continue;
+                bool hit = false;
+                for (int a = -1; a <= 1 && !hit; a++)
+                    for (int b = -1; b <= 1 && !hit; b++)
+                        for (int c = -1; c <= 1 && !hit; c++) hit = *S.at(i + a, j + b, k + c) == neighbor;
+                if (hit) seeds[i].push_back((j << 32) | k);
+            }
+    });
+    std::vector<long long> flat;  // (i, j, k) triples
+    for (long long i = 0; i < S.X; i++)
+        for (long long jk : seeds[i]) {
+            flat.push_back(i);
+            flat.push_back(jk >> 32);
+            flat.push_back(jk & 0xffffffffLL);
+        }
+    parallel_for((long long)flat.size() / 3, [&](long long s) {
+        const long long hh = flat[3 * s], kk = flat[3 * s + 1], ll = flat[3 * s + 2];
+        for (long long m = std::max(0LL, hh - d); m <= std::min(S.X - 1, hh + d); m++)
+            for (long long nn = std::max(0LL, kk - d); nn <= std::min(S.Y - 1, kk + d); nn++)
+                for (long long p = std::max(0LL, ll - d); p <= std::min(S.Z - 1, ll + d); p++) {
+                    if (S.get(m, nn, p) != from) continue;
+                    const double r = std::sqrt((double)((m - hh) * (m - hh) + (nn - kk) * (nn - kk) + (p - ll) * (p - ll)));
+                    if (r <= d) S.put(m, nn, p, to);
+                }
+    });
+}
+
+}  // namespace
+
 void hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness, double vox,
-                     std::vector<int> &dims, std::vector<double> &com) {
+                     bool fill_holes, std::vector<int> &dims, std::vector<double> &com) {
     dims.clear();
     com.clear();
-    if (n <= 0) return;
-    double rmax = 0.0, lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    if (n <= 0 || probe <= 0.0) return;  // the layer is marked only with a probe radius (:56, :75)
+    // the reference keeps PDB coordinates and radii in float
+    std::vector<float> x(n), y(n), z(n), r(n);
     for (int a = 0; a < n; a++) {
-        rmax = std::max(rmax, rad[a]);
-        for (int k = 0; k < 3; k++) {
-            lo[k] = std::min(lo[k], xyz[3 * a + k]);
-            hi[k] = std::max(hi[k], xyz[3 * a + k]);
+        x[a] = (float)xyz[3 * a];
+        y[a] = (float)xyz[3 * a + 1];
+        z[a] = (float)xyz[3 * a + 2];
+        r[a] = (float)rad[a];
+    }
+    // FindPDBRanges
+    double xmin = x[0] - r[0], xmax = x[0] + r[0], ymin = y[0] - r[0], ymax = y[0] + r[0], zmin = z[0] - r[0],
+           zmax = z[0] + r[0];
+    for (int a = 1; a < n; a++) {
+        xmin = std::min((float)xmin, x[a] - r[a]);
+        xmax = std::max((float)xmax, x[a] + r[a]);
+        ymin = std::min((float)ymin, y[a] - r[a]);
+        ymax = std::max((float)ymax, y[a] + r[a]);
+        zmin = std::min((float)zmin, z[a] - r[a]);
+        zmax = std::max((float)zmax, z[a] + r[a]);
+    }
+    const double dist = std::max(probe, thickness) + 2.0 * vox;
+    xmin -= dist, xmax += dist, ymin -= dist, ymax += dist, zmin -= dist, zmax += dist;
+    xmax = xmin + (double)(int)(2.0 + (xmax - xmin) / vox) * vox;
+    ymax = ymin + (double)(int)(2.0 + (ymax - ymin) / vox) * vox;
+    zmax = zmin + (double)(int)(2.0 + (zmax - zmin) / vox) * vox;
+    const long long xs = (long long)((xmax - xmin) / vox), ys = (long long)((ymax - ymin) / vox),
+                    zs = (long long)((zmax - zmin) / vox);
+    if (xs < 3 || ys < 3 || zs < 3) return;
+    Space S;
+    S.X = xs;
+    S.Y = ys;
+    S.Z = zs + (15 - (zs + 15) % 16);
+    if ((double)S.X * S.Y * S.Z > 4e9) throw Error(kErrInvalidArgs, "hydration voxel grid too large; use a coarser voxel step");
+    S.v.assign((size_t)(S.X * S.Y * S.Z), kEmpty);
+    const Frame F{xmin, ymin, zmin, vox};
+
+    mark_atoms(S, F, n, x.data(), y.data(), z.data(), r.data(), 0.0, kEmpty, kAtom);
+    mark_atoms(S, F, n, x.data(), y.data(), z.data(), r.data(), probe, kEmpty, kProbe);
+    if (fill_holes) {
+        std::deque<long long> q;
+        if (*S.at(0, 0, 0) == kEmpty) {
+            *S.at(0, 0, 0) = kBulk;
+            q.push_back(0);
         }
-    }
-    const double reach = rmax + probe + thickness;
-    const double margin = reach + 2 * vox;
-    long long dim[3];
-    double org[3];
-    for (int k = 0; k < 3; k++) {
-        org[k] = lo[k] - margin;
-        dim[k] = (long long)std::ceil((hi[k] - lo[k] + 2 * margin) / vox);
-    }
-    const long long nv = dim[0] * dim[1] * dim[2];
-    if (nv > 2000000000LL) throw Error(kErrInvalidArgs, "hydration voxel grid too large; use a coarser voxel step");
-    auto vid = [&](long long i, long long j, long long k) { return (i * dim[1] + j) * dim[2] + k; };
-    // distance to the nearest atom surface, stamped atom by atom; x-slabs are
-    // owned by one thread each so the min-updates never race
-    std::vector<float> dist(nv, 1e30f);
-    std::vector<std::vector<int>> slab_atoms(dim[0]);
-    for (int a = 0; a < n; a++) {
-        const double R = rad[a] + probe + thickness;
-        long long i0 = std::max(0LL, (long long)std::floor((xyz[3 * a] - R - org[0]) / vox));
-        long long i1 = std::min(dim[0] - 1, (long long)std::ceil((xyz[3 * a] + R - org[0]) / vox));
-        for (long long i = i0; i <= i1; i++) slab_atoms[i].push_back(a);
-    }
-    parallel_for(dim[0], [&](long long i) {
-        const double cx = org[0] + (i + 0.5) * vox;
-        for (int a : slab_atoms[i]) {
-            const double R = rad[a] + probe + thickness;
-            const double dx = cx - xyz[3 * a];
-            const double r2 = R * R - dx * dx;
-            if (r2 < 0) continue;
-            const double ry = std::sqrt(r2);
-            long long j0 = std::max(0LL, (long long)std::floor((xyz[3 * a + 1] - ry - org[1]) / vox));
-            long long j1 = std::min(dim[1] - 1, (long long)std::ceil((xyz[3 * a + 1] + ry - org[1]) / vox));
-            for (long long j = j0; j <= j1; j++) {
-                const double cy = org[1] + (j + 0.5) * vox;
-                const double dy = cy - xyz[3 * a + 1];
-                const double r2z = R * R - dx * dx - dy * dy;
-                if (r2z < 0) continue;
-                const double rz = std::sqrt(r2z);
-                long long k0 = std::max(0LL, (long long)std::floor((xyz[3 * a + 2] - rz - org[2]) / vox));
-                long long k1 = std::min(dim[2] - 1, (long long)std::ceil((xyz[3 * a + 2] + rz - org[2]) / vox));
-                for (long long k = k0; k <= k1; k++) {
-                    const double cz = org[2] + (k + 0.5) * vox;
-                    const double dz = cz - xyz[3 * a + 2];
-                    const float d = (float)(std::sqrt(dx * dx + dy * dy + dz * dz) - rad[a]);
-                    float &t = dist[vid(i, j, k)];
-                    if (d < t) t = d;
-                }
-            }
+        while (!q.empty()) {
+            const long long v = q.front();
+            q.pop_front();
+            const long long k = v % S.Z, j = (v / S.Z) % S.Y, i = v / (S.Y * S.Z);
+            for (long long a = std::max(0LL, i - 1); a <= std::min(S.X - 1, i + 1); a++)
+                for (long long b = std::max(0LL, j - 1); b <= std::min(S.Y - 1, j + 1); b++)
+                    for (long long c = std::max(0LL, k - 1); c <= std::min(S.Z - 1, k + 1); c++) {
+                        char *p = S.at(a, b, c);
+                        if (*p != kEmpty) continue;
+                        *p = kBulk;
+                        q.push_back((a * S.Y + b) * S.Z + c);
+                    }
         }
-    });
-    // bulk solvent: flood fill from the corner through voxels outside the probe shell
-    std::vector<uint8_t> bulk(nv, 0);
-    std::deque<long long> queue;
-    auto outside = [&](long long v) { return dist[v] > (float)probe; };
-    if (outside(0)) {
-        bulk[0] = 1;
-        queue.push_back(0);
+        for (auto &c : S.v)
+            if (c == kEmpty) c = kAtom;
+    } else {
+        for (auto &c : S.v)
+            if (c == kEmpty) c = kBulk;
     }
-    while (!queue.empty()) {
-        const long long v = queue.front();
-        queue.pop_front();
-        const long long k = v % dim[2], j = (v / dim[2]) % dim[1], i = v / (dim[1] * dim[2]);
-        const long long nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
-        for (auto &c : nb) {
-            if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= dim[0] || c[1] >= dim[1] || c[2] >= dim[2]) continue;
-            const long long w = vid(c[0], c[1], c[2]);
-            if (!bulk[w] && outside(w)) {
-                bulk[w] = 1;
-                queue.push_back(w);
-            }
-        }
-    }
-    // hydration layer: bulk voxels within the solvation thickness of the probe shell
-    const float layer_max = (float)(probe + thickness);
-    auto in_layer = [&](long long v) { return bulk[v] && dist[v] <= layer_max; };
-    // greedy 3-D meshing: from each unassigned layer voxel (in x, y, z order)
-    // grow a box along z, then y, then x while every covered voxel is an
-    // unassigned layer voxel; cover those voxels and continue
+    mark_layer(S, probe, vox, kBulk, kProbe, kLayer, kProbe);
+    if (probe < thickness)
+        mark_layer(S, std::fabs(thickness - probe), vox, kLayer, kBulk, kLayer, kBulk);
+    else
+        mark_layer(S, std::fabs(thickness - probe), vox, kLayer, kBulk, kBulk, kLayer);
+    for (auto &c : S.v)
+        if (c == kProbe) c = kAtom;
+    const int per_nm = (int)(1.0 / vox);
+    for (long long i = 0; i < S.X; i++)
+        for (int t = 0; t < per_nm && 1 + t < S.Y; t++)
+            for (int zk = 1; zk <= 4 && zk < S.Z; zk++) *S.at(i, 1 + t, zk) = kCorner;
+
+    // greedy 3-D meshing of the layer voxels
     struct Box {
         long long i0, ni, j0, nj, k0, nk;
     };
     std::vector<Box> boxes;
-    std::vector<uint8_t> todo(nv, 0);
-    parallel_for(dim[0], [&](long long i) {
-        for (long long j = 0; j < dim[1]; j++)
-            for (long long k = 0; k < dim[2]; k++) {
-                const long long v = vid(i, j, k);
-                todo[v] = in_layer(v) ? 1 : 0;
-            }
-    });
-    for (long long i = 0; i < dim[0]; i++)
-        for (long long j = 0; j < dim[1]; j++)
-            for (long long k = 0; k < dim[2]; k++) {
-                if (!todo[vid(i, j, k)]) continue;
+    auto todo = [&](long long i, long long j, long long k) { return *S.at(i, j, k) == kLayer; };
+    for (long long i = 0; i < S.X; i++)
+        for (long long j = 0; j < S.Y; j++)
+            for (long long k = 0; k < S.Z; k++) {
+                if (!todo(i, j, k)) continue;
                 long long nk = 1;
-                while (k + nk < dim[2] && todo[vid(i, j, k + nk)]) nk++;
+                while (k + nk < S.Z && todo(i, j, k + nk)) nk++;
                 long long nj = 1;
-                while (j + nj < dim[1]) {
+                while (j + nj < S.Y) {
                     bool ok = true;
-                    for (long long c = 0; c < nk && ok; c++) ok = todo[vid(i, j + nj, k + c)] != 0;
+                    for (long long c = 0; c < nk && ok; c++) ok = todo(i, j + nj, k + c);
                     if (!ok) break;
                     nj++;
                 }
                 long long ni = 1;
-                while (i + ni < dim[0]) {
+                while (i + ni < S.X) {
                     bool ok = true;
                     for (long long b2 = 0; b2 < nj && ok; b2++)
-                        for (long long c = 0; c < nk && ok; c++) ok = todo[vid(i + ni, j + b2, k + c)] != 0;
+                        for (long long c = 0; c < nk && ok; c++) ok = todo(i + ni, j + b2, k + c);
                     if (!ok) break;
                     ni++;
                 }
                 for (long long a2 = 0; a2 < ni; a2++)
                     for (long long b2 = 0; b2 < nj; b2++)
-                        for (long long c = 0; c < nk; c++) todo[vid(i + a2, j + b2, k + c)] = 0;
+                        for (long long c = 0; c < nk; c++) *S.at(i + a2, j + b2, k + c) = kEmpty;
                 boxes.push_back({i, ni, j, nj, k, nk});
             }
     dims.resize(3 * boxes.size());
@@ -155,9 +233,9 @@ void hydration_boxes(int n, const double *xyz, const double *rad, double probe, 
         dims[3 * b] = (int)B.ni;
         dims[3 * b + 1] = (int)B.nj;
         dims[3 * b + 2] = (int)B.nk;
-        com[3 * b] = org[0] + (B.i0 + 0.5 * B.ni) * vox;
-        com[3 * b + 1] = org[1] + (B.j0 + 0.5 * B.nj) * vox;
-        com[3 * b + 2] = org[2] + (B.k0 + 0.5 * B.nk) * vox;
+        com[3 * b] = xmin + ((double)B.i0 + 0.5 * (double)(B.ni - 1)) * vox;
+        com[3 * b + 1] = ymin + ((double)B.j0 + 0.5 * (double)(B.nj - 1)) * vox;
+        com[3 * b + 2] = zmin + ((double)B.k0 + 0.5 * (double)(B.nk - 1)) * vox;
     }
 }
 
@@ -166,13 +244,13 @@ void hydration_boxes(int n, const double *xyz, const double *rad, double probe, 
 using namespace dplus;
 
 extern "C" int dplus_hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness,
-                                     double vox, int capacity, int *nbox, int *dims, double *com) {
+                                     double vox, int fill_holes, int capacity, int *nbox, int *dims, double *com) {
     try {
-        if (n < 0 || !xyz || !rad || !nbox || vox <= 0 || probe < 0 || thickness < 0)
+        if (n < 0 || (n > 0 && (!xyz || !rad)) || !nbox || vox <= 0 || probe < 0 || thickness < 0)
             throw Error(kErrInvalidArgs, "bad hydration arguments");
         std::vector<int> d;
         std::vector<double> c;
-        hydration_boxes(n, xyz, rad, probe, thickness, vox, d, c);
+        hydration_boxes(n, xyz, rad, probe, thickness, vox, fill_holes != 0, d, c);
         *nbox = (int)(d.size() / 3);
         if (*nbox > capacity) return kOk;  // caller retries with capacity >= *nbox
         if (*nbox) {

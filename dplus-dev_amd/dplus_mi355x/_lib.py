@@ -86,7 +86,7 @@ SIGNATURES = {
     "dplus_synth_walk": (ctypes.c_int, [ctypes.c_ulonglong, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
     "dplus_hydration_boxes": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double,
-                                             ctypes.c_double, ctypes.c_int, _I, _I, _D]),
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, _I, _I, _D]),
     "dplus_structure_create": (ctypes.c_int, [ctypes.POINTER(PdbDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "dplus_structure_destroy": (None, [ctypes.c_void_p]),
     "dplus_structure_build_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),

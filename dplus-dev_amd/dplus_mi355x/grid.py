@@ -175,14 +175,16 @@ def dock_grid_space_filling(out, dock, distances, angles_deg, reps, scale=1.0):
                                                   L.dp(r)))
 
 
-def hydration_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05):
-    """Hydration-layer boxes (dims [n,3] voxels, centres [n,3] nm) around atoms of radius rad."""
+def hydration_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05, fill_holes=True):
+    """Hydration-layer boxes (dims [n,3] voxels, centres [n,3] nm) around atoms of radius rad,
+    with the reference's voxel marking (cavities count as atoms when fill_holes)."""
     xyz, rad = L.f64(xyz).reshape(-1, 3), L.f64(rad)
     n = ctypes.c_int()
-    L.check(L.lib().dplus_hydration_boxes(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, 0,
+    fh = 1 if fill_holes else 0
+    L.check(L.lib().dplus_hydration_boxes(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, fh, 0,
                                           ctypes.byref(n), None, None))
     dims, com = np.zeros((max(1, n.value), 3), np.int32), np.zeros((max(1, n.value), 3))
-    L.check(L.lib().dplus_hydration_boxes(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, n.value,
+    L.check(L.lib().dplus_hydration_boxes(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, fh, n.value,
                                           ctypes.byref(n), L.ip(dims), L.dp(com)))
     return dims[:n.value], com[:n.value]
 

@@ -184,12 +184,14 @@ int dplus_dock_grid_space_filling(dplus_grid *out, int nleaf, const dplus_leaf *
 
 /* ------------------------------------------------------------------ hydration layer */
 /* Voxelised hydration ("outer solvent") layer of PDBAmplitude::CalculateSolventSpace
- * (Backend/Backend/VoxelizedSolvent.cpp:4-194, box reduction :263-363): voxels of
- * bulk solvent within `thickness` of the probe shell (atom radius + probe), as
- * boxes (dims in voxels, centres in nm).  Writes up to `capacity` boxes and sets
- * *nbox to the number found; call again with capacity >= *nbox when it is larger. */
+ * (Backend/Backend/VoxelizedSolvent.cpp:4-194, box reduction :263-363) with the
+ * reference's voxel marking: the part of the probe shell (atom radius + probe)
+ * that faces the bulk solvent, grown into the bulk to the solvation thickness;
+ * cavities count as atoms when fill_holes != 0.  Output: boxes (dims in voxels,
+ * centres in nm).  Writes up to `capacity` boxes and sets *nbox to the number
+ * found; call again with capacity >= *nbox when it is larger. */
 int dplus_hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness,
-                          double vox, int capacity, int *nbox, int *dims, double *com);
+                          double vox, int fill_holes, int capacity, int *nbox, int *dims, double *com);
 
 /* ------------------------------------------------------------------ amplitude files */
 /* AmpGridAmplitude::ReadAmplitudeFromFile (Backend/Backend/Amplitude.cpp:3724-3796):

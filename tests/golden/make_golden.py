@@ -72,9 +72,10 @@ def main():
 
 
 GPU_REF = "/root/reference/PythonInterface/tests/reviewer_tests/files_for_tests/generate/gpu/short"
-SYM_CASES = [f"Man_Symm_Impl_No_Charges_wo_Hydr_{m}" for m in ("MC", "VEGAS", "Hybrid_MC", "Hybrid_VEGAS", "Ga_Kr")] + \
-            [f"Space_fill_Symm_Impl_No_Charges_wo_Hydr_{m}" for m in ("MC", "Hybrid_MC", "Hybrid_VEGAS", "Ga_Kr",
-                                                                       "Hybrid_Ga_Kr")]
+SYM_CASES = [f"Man_Symm_Impl_No_Charges_{h}_Hydr_{m}" for h in ("wo", "With")
+             for m in ("MC", "VEGAS", "Hybrid_MC", "Hybrid_VEGAS", "Ga_Kr")] + \
+            [f"Space_fill_Symm_Impl_No_Charges_{h}_Hydr_{m}" for h in ("wo", "With")
+             for m in ("MC", "Hybrid_MC", "Hybrid_VEGAS", "Ga_Kr", "Hybrid_Ga_Kr")]
 
 
 def symmetry_cases():
