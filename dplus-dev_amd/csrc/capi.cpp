@@ -139,6 +139,12 @@ Device::Device() {
         mem = p.totalGlobalMem;
     }
     ok = hipStreamCreate(&stream) == hipSuccess;
+    // keep freed pool memory cached for reuse instead of returning it at every sync
+    hipMemPool_t pool;
+    if (ok && hipDeviceGetDefaultMemPool(&pool, id) == hipSuccess) {
+        uint64_t keep = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
 }
 
 void Device::require() {
@@ -146,9 +152,13 @@ void Device::require() {
     check_hip(hipSetDevice(id), "hipSetDevice");
 }
 
+static thread_local hipStream_t t_alloc_stream = nullptr;
+hipStream_t alloc_stream() { return t_alloc_stream ? t_alloc_stream : Device::get().stream; }
+void set_alloc_stream(hipStream_t s) { t_alloc_stream = s; }
+
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFreeAsync(p, s);
     p = nullptr;
     n = 0;
 }
@@ -156,9 +166,11 @@ void DevBuf::ensure(size_t bytes) {
     if (bytes <= n) return;
     release();
     if (bytes == 0) return;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
+    s = alloc_stream();
+    if (hipMallocAsync(&p, bytes, s) != hipSuccess) {
         p = nullptr;
-        throw Error(kErrOutOfMemory, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        (void)hipGetLastError();
+        throw Error(kErrOutOfMemory, "device allocation of " + std::to_string(bytes) + " bytes failed");
     }
     n = bytes;
 }
@@ -205,7 +217,7 @@ Grid::Grid(int gs, double qm, int prec) {
     P = total_points(gs);
     Device::get().require();
     dev.ensure((size_t)P * node_bytes());
-    check_hip(hipMemsetAsync(dev.p, 0, (size_t)P * node_bytes(), Device::get().stream), "hipMemset");
+    check_hip(hipMemsetAsync(dev.p, 0, (size_t)P * node_bytes(), alloc_stream()), "hipMemset");
 }
 
 void Grid::ensure_host() {
@@ -580,8 +592,26 @@ void DockSet::finish(std::vector<Grid *> &gl, std::vector<RotDev> &rots, std::ve
     trb.upload(tr.data(), tr.size() * sizeof(double4), s);
 }
 
+void launch_mc_finish(const GroupDev *groups, int ngroups, int mstride, const double *out, double total, double *I,
+                      hipStream_t s);
+void launch_chi2(const double *I, const double *y, const double *sig, int n, double scale, double constant,
+                 double *chi2, hipStream_t s);
+
+// chi^2 of the device intensities dI against host data, on the device
+static void chi2_on_device(const double *dI, int nq, Chi2Request *chi, hipStream_t s) {
+    DevBuf by, bs, bc;
+    by.upload(chi->y, nq * sizeof(double), s);
+    bs.upload(chi->sigma, nq * sizeof(double), s);
+    bc.ensure(sizeof(double));
+    launch_chi2(dI, (const double *)by.p, (const double *)bs.p, nq, chi->scale, chi->constant, (double *)bc.p, s);
+    check_hip(hipGetLastError(), "chi2 launch");
+    check_hip(hipMemcpyAsync(&chi->chi2, bc.p, sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    chi->on_device = true;
+}
+
 void orientation_average(const DockSet &D, int nq, const double *q, const dplus_oa_params *p, double *out,
-                         dplus_oa_stats *st, hipStream_t s) {
+                         dplus_oa_stats *st, hipStream_t s, Chi2Request *chi) {
+    if (chi) chi->on_device = false;
     for (int i = 1; i < nq; i++)
         if (q[i] < q[i - 1]) throw Error(kErrInvalidArgs, "q must be ascending");
     if (nq <= 0) return;
@@ -622,12 +652,13 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                           (const double *)bq.p, D.step, p->seed, 0, nblk, (double *)bpart.p, (double *)bout.p, s);
             }
             check_hip(hipGetLastError(), "mc launch");
-            std::vector<double> r((size_t)ng * mstride);
-            check_hip(hipMemcpyAsync(r.data(), bout.p, r.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            DevBuf bI;
+            bI.ensure((size_t)nq * sizeof(double));
+            launch_mc_finish((const GroupDev *)bg.p, ng, mstride, (const double *)bout.p, (double)total, (double *)bI.p, s);
+            check_hip(hipGetLastError(), "mc finish launch");
+            if (chi) chi2_on_device((const double *)bI.p, nq, chi, s);
+            check_hip(hipMemcpyAsync(out, bI.p, nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
             check_hip(hipStreamSynchronize(s), "mc");
-            for (int gi = 0; gi < ng; gi++)
-                for (int j = 0; j < groups[gi].m; j++)
-                    out[groups[gi].q0 + j] = r[(size_t)gi * mstride + j] / (double)total;
             used = total;
         } else {
             // chunked with the reference's three-back convergence test (GPUHybridCalc.cu:705-720)
@@ -715,6 +746,7 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                 active.swap(next);
             }
         }
+        if (chi && p->eps <= 0.0) chi2_on_device((const double *)bh.p + (size_t)(iters - 1) * nq, nq, chi, s);
         check_hip(hipMemcpyAsync(hist.data(), bh.p, hist.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         check_hip(hipStreamSynchronize(s), "vegas");
         for (int g = 0; g < ng; g++)
@@ -724,6 +756,15 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
         gk_average(D, nq, q, p->eps, p->gk_max_depth > 0 ? p->gk_max_depth : (1LL << 40), out, st, s);
     } else {
         throw Error(kErrInvalidArgs, "unknown orientation method");
+    }
+    if (chi && !chi->on_device) {
+        double c2 = 0.0;
+        for (int i = 0; i < nq; i++)
+            if (chi->sigma[i] > 0.0) {
+                const double r = (chi->scale * out[i] + chi->constant - chi->y[i]) / chi->sigma[i];
+                c2 = std::fma(r, r, c2);
+            }
+        chi->chi2 = c2;
     }
 }
 

@@ -40,10 +40,19 @@ struct Device {
     Device();
 };
 
+// Stream that device buffers of the calling thread are allocated and released
+// on (stream-ordered allocation from the device memory pool): the library's
+// stream unless the thread has set its own, as each worker of the request
+// engine does.  Frees never synchronise the device, so work of concurrent
+// requests on other streams keeps running.
+hipStream_t alloc_stream();
+void set_alloc_stream(hipStream_t s);
+
 // owning device allocation that only grows
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
+    hipStream_t s = nullptr;  // stream the allocation is ordered on
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -162,8 +171,18 @@ struct Structure {
 
 std::vector<GroupDev> make_groups(const double *q, int nq, double step);
 void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s);
+// optional goodness of fit of an average: chi2 = sum ((scale I + constant - y) / sigma)^2
+// over q with sigma > 0; computed on the device from the device-resident
+// intensities when the average has a fixed count (eps <= 0, MC or VEGAS),
+// else from the host intensities
+struct Chi2Request {
+    const double *y = nullptr, *sigma = nullptr;  // host arrays, nq each
+    double scale = 1.0, constant = 0.0;
+    double chi2 = 0.0;                            // result
+    bool on_device = false;                       // result: where it was computed
+};
 void orientation_average(const DockSet &D, int nq, const double *q, const dplus_oa_params *p, double *out,
-                         dplus_oa_stats *st, hipStream_t s);
+                         dplus_oa_stats *st, hipStream_t s, Chi2Request *chi = nullptr);
 void gk_average(const DockSet &D, int nq, const double *q, double eps, long long max_depth, double *out,
                 dplus_oa_stats *st, hipStream_t s);
 // space-filling lattice of a docked grid: basis vectors v[3k..3k+2] (nm) and

@@ -1,0 +1,615 @@
+// engine.cpp -- the on-device request engine and fit sessions.
+//
+// Request engine (continuous batching).  A request is one I(q) curve: a PDB
+// leaf (atoms and hydration boxes) or a resident grid, optional symmetry
+// placements, a q vector and an orientation average, optionally scored by
+// chi^2 against data.  Submission copies the inputs and returns at once.
+// Requests are admitted in arrival order whenever a device slot is free and
+// their device memory fits the HBM budget; each slot is a worker thread with
+// its own HIP stream, so the kernels of admitted requests overlap on the GPU
+// while later arrivals wait.  Device buffers are stream-ordered pool
+// allocations, so one request's frees never stall the others.  A request's
+// result does not depend on what else runs: every kernel reads only that
+// request's buffers and its own seeded random streams.
+//
+// Fit session.  Resident subunit grids plus fixed q, data and sigma; each
+// evaluation re-docks the subunits with new placements (symmetry parameters of
+// a fitting iteration), averages by fixed-count Monte Carlo and returns I(q)
+// and chi^2 computed on the device.  The evaluation is one captured HIP graph
+// (placement upload, MC kernels, intensity, chi^2, result download), replayed
+// while the number of rotations stays the same and re-captured when it changes.
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "device.hpp"
+
+namespace dplus {
+
+void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
+               const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
+               double step, uint64_t seed, long long dir0, int nblk, double *partial, double *out,
+               hipStream_t s);
+void launch_mc_finish(const GroupDev *groups, int ngroups, int mstride, const double *out, double total, double *I,
+                      hipStream_t s);
+void launch_chi2_param(const double *I, const double *y, const double *sig, int n, const double *scale_constant,
+                       double *chi2, hipStream_t s);
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+    long long id = 0;
+    bool has_pdb = false;
+    std::vector<double> xyz, coef9, rad, box_com;
+    std::vector<int> ion, box_dims;
+    dplus_pdb_desc desc{};
+    Grid *grid = nullptr;
+    int grid_size = 0, precision = DPLUS_PRECISION_MIXED;
+    double qmax = 0.0;
+    std::vector<double> rot, trans;
+    std::vector<int> ntrans;
+    std::vector<double> q, y, sigma;
+    double fit_scale = 1.0, fit_constant = 0.0;
+    bool want_chi = false;
+    dplus_oa_params oa{};
+    size_t need = 0;
+    // results
+    std::vector<double> I;
+    double chi2 = 0.0;
+    bool chi_on_device = false;
+    dplus_oa_stats st{};
+    int code = 0;
+    std::string message;
+    int slot = -1;
+    bool done = false;
+    Clock::time_point t_submit, t_start, t_end;
+};
+
+// device bytes a request holds while it runs: its grid and structure, the
+// orientation-average scratch, and a fixed margin for small tables
+size_t request_bytes(const Request &r) {
+    size_t b = 16u << 20;
+    int gs = r.grid_size;
+    double step;
+    bool fp64;
+    if (r.has_pdb) {
+        if (gs % 2) gs++;
+        const long long P = total_points(gs);
+        fp64 = r.precision == DPLUS_PRECISION_FP64;
+        b += (size_t)P * (fp64 ? 32 : 16);
+        b += (size_t)(r.desc.natoms + r.desc.nbox) * 48;
+        step = r.qmax / (double)(gs / 2);
+    } else {
+        fp64 = r.grid->fp64();
+        step = r.grid->step;
+    }
+    (void)fp64;
+    const std::vector<GroupDev> groups = make_groups(r.q.data(), (int)r.q.size(), step);
+    size_t ng = groups.size(), mstride = 1;
+    for (auto &g : groups) mstride = std::max<size_t>(mstride, (size_t)g.m);
+    const dplus_oa_params &p = r.oa;
+    if (p.method == DPLUS_OA_MC) {
+        const long long chunk = p.chunk > 0 ? p.chunk : 8192;
+        const long long dirs = p.eps <= 0.0 ? ((std::max<long long>(1, p.max_dirs) + chunk - 1) / chunk) * chunk : chunk;
+        b += ng * (size_t)(dirs / 256) * mstride * 8 + ng * mstride * 8;
+    } else if (p.method == DPLUS_OA_VEGAS) {
+        b += ng * mstride * 1024 * 8 + ng * 1024 * 8 + (size_t)std::max(1, p.vegas_iterations) * r.q.size() * 8;
+    } else {
+        b += 64u << 20;
+    }
+    b += (r.rot.size() / 4 + 1) * sizeof(RotDev) + (r.trans.size() / 3 + 1) * sizeof(double4);
+    return b;
+}
+
+}  // namespace
+
+struct Engine {
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> pending, admitted;
+    std::map<long long, std::unique_ptr<Request>> all;
+    long long next_id = 1, submitted = 0, completed = 0, max_concurrent = 0;
+    size_t budget = 0, reserved = 0, peak_reserved = 0;
+    int busy = 0;
+    bool stopping = false;
+    std::vector<hipStream_t> streams;
+    std::vector<std::thread> workers;
+
+    Engine(long long budget_bytes, int nstreams) {
+        Device &d = Device::get();
+        d.require();
+        budget = budget_bytes > 0 ? (size_t)budget_bytes : (size_t)(0.85 * (double)d.mem);
+        nstreams = std::max(1, std::min(16, nstreams));
+        for (int i = 0; i < nstreams; i++) {
+            hipStream_t s;
+            check_hip(hipStreamCreate(&s), "hipStreamCreate");
+            streams.push_back(s);
+        }
+        for (int i = 0; i < nstreams; i++) workers.emplace_back([this, i] { work(i); });
+    }
+
+    ~Engine() {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_done.wait(lk, [&] { return pending.empty() && admitted.empty() && busy == 0; });
+            stopping = true;
+        }
+        cv_work.notify_all();
+        for (auto &t : workers) t.join();
+        for (auto s : streams) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+
+    // admit queued requests in arrival order while a slot is free and the HBM budget holds
+    void admit_locked() {
+        while (!pending.empty()) {
+            Request *r = pending.front();
+            if ((int)admitted.size() + busy >= (int)streams.size()) break;
+            if (reserved + r->need > budget) break;
+            pending.pop_front();
+            reserved += r->need;
+            peak_reserved = std::max(peak_reserved, reserved);
+            admitted.push_back(r);
+            cv_work.notify_one();
+        }
+    }
+
+    void work(int w) {
+        hipStream_t s = streams[w];
+        try {
+            Device::get().require();
+        } catch (...) {
+        }
+        set_alloc_stream(s);
+        for (;;) {
+            Request *r = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stopping || !admitted.empty(); });
+                if (admitted.empty()) return;
+                r = admitted.front();
+                admitted.pop_front();
+                busy++;
+                max_concurrent = std::max<long long>(max_concurrent, busy);
+            }
+            r->slot = w;
+            r->t_start = Clock::now();
+            run(*r, s);
+            r->t_end = Clock::now();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                busy--;
+                reserved -= r->need;
+                r->done = true;
+                completed++;
+                admit_locked();
+            }
+            cv_done.notify_all();
+        }
+    }
+
+    static void run(Request &r, hipStream_t s) {
+        try {
+            std::unique_ptr<Structure> S;
+            std::unique_ptr<Grid> G;
+            Grid *g = r.grid;
+            if (r.has_pdb) {
+                S.reset(new Structure(&r.desc, s));
+                G.reset(new Grid(r.grid_size, r.qmax, r.precision));
+                S->build(*G, false, s);
+                g = G.get();
+            }
+            const double single[4] = {0, 0, 0, 1}, origin[3] = {0, 0, 0};
+            const int one = 1;
+            dplus_leaf leaf{};
+            leaf.grid = (dplus_grid *)g;
+            if (r.rot.empty()) {
+                leaf.nrot = 1;
+                leaf.rot = single;
+                leaf.ntrans = &one;
+                leaf.trans = origin;
+            } else {
+                leaf.nrot = (int)(r.rot.size() / 4);
+                leaf.rot = r.rot.data();
+                leaf.ntrans = r.ntrans.data();
+                leaf.trans = r.trans.data();
+            }
+            DockSet D(1, &leaf, s);
+            r.I.assign(r.q.size(), 0.0);
+            Chi2Request chi;
+            if (r.want_chi) {
+                chi.y = r.y.data();
+                chi.sigma = r.sigma.data();
+                chi.scale = r.fit_scale;
+                chi.constant = r.fit_constant;
+            }
+            orientation_average(D, (int)r.q.size(), r.q.data(), &r.oa, r.I.data(), &r.st, s,
+                                r.want_chi ? &chi : nullptr);
+            check_hip(hipStreamSynchronize(s), "request");
+            r.chi2 = chi.chi2;
+            r.chi_on_device = chi.on_device;
+            r.code = 0;
+        } catch (const Error &e) {
+            r.code = e.code;
+            r.message = e.what();
+        } catch (const std::exception &e) {
+            r.code = kErrGeneral;
+            r.message = e.what();
+        }
+    }
+};
+
+// ------------------------------------------------------------------ fit session
+struct Session {
+    hipStream_t s = nullptr;
+    std::vector<Grid *> leaves;
+    bool fp64 = false;
+    double step = 0.0;
+    int nq = 0, ng = 0, mstride = 1, nblk = 0;
+    long long total = 0;
+    unsigned long long seed = 0;
+    std::vector<GroupDev> groups;
+    DevBuf bgrids, bq, bg, bact, bpart, bout, bI, by, bsig, bchi, bparam, brot, btr;
+    size_t cap_rot = 0, cap_tr = 0;
+    RotDev *h_rot = nullptr;
+    double4 *h_tr = nullptr;
+    double *h_param = nullptr, *h_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int graph_nrot = -1;
+    long long evaluations = 0, captures = 0;
+
+    ~Session() {
+        if (s) (void)hipStreamSynchronize(s);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void *p : {(void *)h_rot, (void *)h_tr, (void *)h_param, (void *)h_out})
+            if (p) (void)hipHostFree(p);
+        bgrids.release(), bq.release(), bg.release(), bact.release(), bpart.release(), bout.release();
+        bI.release(), by.release(), bsig.release(), bchi.release(), bparam.release(), brot.release(), btr.release();
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            (void)hipStreamDestroy(s);
+        }
+    }
+
+    void host_alloc(void **p, size_t bytes) {
+        check_hip(hipHostMalloc(p, bytes, hipHostMallocDefault), "hipHostMalloc");
+    }
+
+    void grow(size_t nrot, size_t ntr) {
+        if (nrot > cap_rot) {
+            if (h_rot) (void)hipHostFree(h_rot);
+            cap_rot = std::max<size_t>(nrot, 2 * cap_rot);
+            host_alloc((void **)&h_rot, cap_rot * sizeof(RotDev));
+            brot.ensure(cap_rot * sizeof(RotDev));
+            graph_nrot = -1;
+        }
+        if (ntr > cap_tr) {
+            if (h_tr) (void)hipHostFree(h_tr);
+            cap_tr = std::max<size_t>(ntr, 2 * cap_tr);
+            host_alloc((void **)&h_tr, cap_tr * sizeof(double4));
+            btr.ensure(cap_tr * sizeof(double4));
+            graph_nrot = -1;
+        }
+    }
+
+    void capture(int nrot) {
+        if (exec) (void)hipGraphExecDestroy(exec), exec = nullptr;
+        if (graph) (void)hipGraphDestroy(graph), graph = nullptr;
+        check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        check_hip(hipMemcpyAsync(brot.p, h_rot, cap_rot * sizeof(RotDev), hipMemcpyHostToDevice, s), "H2D");
+        check_hip(hipMemcpyAsync(btr.p, h_tr, cap_tr * sizeof(double4), hipMemcpyHostToDevice, s), "H2D");
+        check_hip(hipMemcpyAsync(bparam.p, h_param, 2 * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
+        launch_mc(fp64, (const void *const *)bgrids.p, (const RotDev *)brot.p, nrot, (const double4 *)btr.p,
+                  (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride, (const double *)bq.p, step, seed, 0, nblk,
+                  (double *)bpart.p, (double *)bout.p, s);
+        launch_mc_finish((const GroupDev *)bg.p, ng, mstride, (const double *)bout.p, (double)total, (double *)bI.p, s);
+        launch_chi2_param((const double *)bI.p, (const double *)by.p, (const double *)bsig.p, nq,
+                          (const double *)bparam.p, (double *)bchi.p, s);
+        check_hip(hipMemcpyAsync(h_out, bI.p, nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        check_hip(hipMemcpyAsync(h_out + nq, bchi.p, sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hipGraph_t gph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &gph);
+        check_hip(e, "end capture");
+        graph = gph;
+        check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+        graph_nrot = nrot;
+        captures++;
+    }
+};
+
+}  // namespace dplus
+
+using namespace dplus;
+
+#define ENGINE_GUARD(...)                                   \
+    try {                                                   \
+        __VA_ARGS__;                                        \
+        return kOk;                                         \
+    } catch (const Error &e) {                              \
+        return set_error(e.code, e.what());                 \
+    } catch (const std::exception &e) {                     \
+        return set_error(kErrGeneral, e.what());            \
+    } catch (...) {                                         \
+        return set_error(kErrGeneral, "unknown exception"); \
+    }
+
+extern "C" {
+
+int dplus_engine_create(const dplus_engine_config *cfg, dplus_engine **out) {
+    ENGINE_GUARD({
+        if (!out) throw Error(kErrInvalidArgs, "null output");
+        const long long budget = cfg ? cfg->hbm_budget_bytes : 0;
+        const int streams = cfg && cfg->streams > 0 ? cfg->streams : 4;
+        *out = (dplus_engine *)new Engine(budget, streams);
+    })
+}
+
+void dplus_engine_destroy(dplus_engine *e) { delete (Engine *)e; }
+
+int dplus_engine_submit(dplus_engine *eh, const dplus_request *rq, long long *ticket) {
+    ENGINE_GUARD({
+        Engine *e = (Engine *)eh;
+        if (!e || !rq || !ticket) throw Error(kErrInvalidArgs, "null argument");
+        if (rq->nq <= 0 || !rq->q) throw Error(kErrInvalidArgs, "a request needs q values");
+        std::unique_ptr<Request> r(new Request());
+        if (rq->pdb) {
+            const dplus_pdb_desc &d = *rq->pdb;
+            if (d.natoms < 0 || d.nion <= 0 || d.nbox < 0) throw Error(kErrInvalidArgs, "bad PDB description");
+            r->has_pdb = true;
+            r->xyz.assign(d.xyz, d.xyz + 3 * (size_t)d.natoms);
+            r->ion.assign(d.ion, d.ion + d.natoms);
+            r->coef9.assign(d.coef9, d.coef9 + 9 * (size_t)d.nion);
+            r->rad.assign(d.rad, d.rad + d.nion);
+            if (d.nbox > 0) {
+                r->box_dims.assign(d.box_dims, d.box_dims + 3 * (size_t)d.nbox);
+                r->box_com.assign(d.box_com, d.box_com + 3 * (size_t)d.nbox);
+            }
+            r->desc = d;
+            r->desc.xyz = r->xyz.data();
+            r->desc.ion = r->ion.data();
+            r->desc.coef9 = r->coef9.data();
+            r->desc.rad = r->rad.data();
+            r->desc.box_dims = d.nbox > 0 ? r->box_dims.data() : nullptr;
+            r->desc.box_com = d.nbox > 0 ? r->box_com.data() : nullptr;
+            r->grid_size = rq->grid_size;
+            r->qmax = rq->qmax;
+            r->precision = rq->precision;
+            if (r->grid_size < 2 || !(r->qmax > 0)) throw Error(kErrInvalidArgs, "bad grid size or qmax");
+        } else if (rq->grid) {
+            r->grid = (Grid *)rq->grid;
+        } else {
+            throw Error(kErrInvalidArgs, "a request needs a PDB description or a resident grid");
+        }
+        if (rq->nrot > 0) {
+            if (!rq->rot || !rq->ntrans || !rq->trans) throw Error(kErrInvalidArgs, "placements need rot/ntrans/trans");
+            r->rot.assign(rq->rot, rq->rot + 4 * (size_t)rq->nrot);
+            r->ntrans.assign(rq->ntrans, rq->ntrans + rq->nrot);
+            size_t nt = 0;
+            for (int i = 0; i < rq->nrot; i++) {
+                if (rq->ntrans[i] < 0) throw Error(kErrInvalidArgs, "negative translation count");
+                nt += (size_t)rq->ntrans[i];
+            }
+            r->trans.assign(rq->trans, rq->trans + 3 * nt);
+        }
+        r->q.assign(rq->q, rq->q + rq->nq);
+        for (int i = 1; i < rq->nq; i++)
+            if (r->q[i] < r->q[i - 1]) throw Error(kErrInvalidArgs, "q must be ascending");
+        if (rq->data) {
+            if (!rq->sigma) throw Error(kErrInvalidArgs, "chi^2 needs sigma with the data");
+            r->want_chi = true;
+            r->y.assign(rq->data, rq->data + rq->nq);
+            r->sigma.assign(rq->sigma, rq->sigma + rq->nq);
+            r->fit_scale = rq->fit_scale;
+            r->fit_constant = rq->fit_constant;
+        }
+        r->oa = rq->oa;
+        r->need = request_bytes(*r);
+        if (r->need > e->budget)
+            throw Error(kErrOutOfMemory, "the request needs " + std::to_string(r->need) +
+                                             " device bytes, more than the engine's budget of " +
+                                             std::to_string(e->budget));
+        r->t_submit = Clock::now();
+        std::lock_guard<std::mutex> lk(e->mu);
+        r->id = e->next_id++;
+        *ticket = r->id;
+        e->pending.push_back(r.get());
+        e->all[r->id] = std::move(r);
+        e->submitted++;
+        e->admit_locked();
+    })
+}
+
+int dplus_engine_poll(dplus_engine *eh, long long ticket, int *done) {
+    ENGINE_GUARD({
+        Engine *e = (Engine *)eh;
+        if (!e || !done) throw Error(kErrInvalidArgs, "null argument");
+        std::lock_guard<std::mutex> lk(e->mu);
+        auto it = e->all.find(ticket);
+        if (it == e->all.end()) throw Error(kErrJobNotFound, "unknown ticket " + std::to_string(ticket));
+        *done = it->second->done ? 1 : 0;
+    })
+}
+
+int dplus_engine_wait(dplus_engine *eh, long long ticket, double *intensity, dplus_request_result *res) {
+    ENGINE_GUARD({
+        Engine *e = (Engine *)eh;
+        if (!e) throw Error(kErrInvalidArgs, "null engine");
+        std::unique_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(e->mu);
+            auto it = e->all.find(ticket);
+            if (it == e->all.end()) throw Error(kErrJobNotFound, "unknown ticket " + std::to_string(ticket));
+            Request *rp = it->second.get();
+            e->cv_done.wait(lk, [&] { return rp->done; });
+            r = std::move(it->second);
+            e->all.erase(it);
+        }
+        if (res) {
+            memset(res, 0, sizeof *res);
+            res->queue_ms = ms_between(r->t_submit, r->t_start);
+            res->run_ms = ms_between(r->t_start, r->t_end);
+            res->latency_ms = ms_between(r->t_submit, r->t_end);
+            res->chi2 = r->chi2;
+            res->chi2_on_device = r->chi_on_device ? 1 : 0;
+            res->reserved_bytes = (long long)r->need;
+            res->slot = r->slot;
+            res->oa = r->st;
+        }
+        if (r->code) throw Error(r->code, r->message);
+        if (intensity) std::copy(r->I.begin(), r->I.end(), intensity);
+    })
+}
+
+int dplus_engine_stats(dplus_engine *eh, long long *submitted, long long *completed, long long *peak_reserved,
+                       long long *budget, long long *max_concurrent) {
+    ENGINE_GUARD({
+        Engine *e = (Engine *)eh;
+        if (!e) throw Error(kErrInvalidArgs, "null engine");
+        std::lock_guard<std::mutex> lk(e->mu);
+        if (submitted) *submitted = e->submitted;
+        if (completed) *completed = e->completed;
+        if (peak_reserved) *peak_reserved = (long long)e->peak_reserved;
+        if (budget) *budget = (long long)e->budget;
+        if (max_concurrent) *max_concurrent = e->max_concurrent;
+    })
+}
+
+int dplus_session_create(int nleaf, dplus_grid *const *leaves, int nq, const double *q, const double *data,
+                         const double *sigma, unsigned long long seed, long long directions, dplus_session **out) {
+    ENGINE_GUARD({
+        if (nleaf <= 0 || !leaves || nq <= 0 || !q || !data || !sigma || !out)
+            throw Error(kErrInvalidArgs, "bad session arguments");
+        Device::get().require();
+        std::unique_ptr<Session> S(new Session());
+        check_hip(hipStreamCreate(&S->s), "hipStreamCreate");
+        set_alloc_stream(S->s);
+        struct Restore {
+            ~Restore() { set_alloc_stream(nullptr); }
+        } restore;
+        std::vector<const void *> gp;
+        for (int l = 0; l < nleaf; l++) {
+            Grid *g = (Grid *)leaves[l];
+            if (!g) throw Error(kErrInvalidArgs, "null leaf grid");
+            if (l == 0) {
+                S->fp64 = g->fp64();
+                S->step = g->step;
+            } else if (g->fp64() != S->fp64 || g->step != S->step || g->N != ((Grid *)leaves[0])->N) {
+                throw Error(kErrInvalidArgs, "session leaves must share grid size, qmax and precision");
+            }
+            S->leaves.push_back(g);
+            gp.push_back(g->dev.p);
+        }
+        for (int i = 1; i < nq; i++)
+            if (q[i] < q[i - 1]) throw Error(kErrInvalidArgs, "q must be ascending");
+        S->nq = nq;
+        S->seed = seed;
+        S->groups = make_groups(q, nq, S->step);
+        S->ng = (int)S->groups.size();
+        for (auto &g : S->groups) S->mstride = std::max(S->mstride, g.m);
+        S->total = ((std::max<long long>(256, directions) + 255) / 256) * 256;
+        S->nblk = (int)(S->total / 256);
+        std::vector<int> act(S->ng);
+        for (int i = 0; i < S->ng; i++) act[i] = i;
+        hipStream_t s = S->s;
+        S->bgrids.upload(gp.data(), gp.size() * sizeof(void *), s);
+        S->bq.upload(q, nq * sizeof(double), s);
+        S->bg.upload(S->groups.data(), S->groups.size() * sizeof(GroupDev), s);
+        S->bact.upload(act.data(), act.size() * sizeof(int), s);
+        S->bpart.ensure((size_t)S->ng * S->nblk * S->mstride * sizeof(double));
+        S->bout.ensure((size_t)S->ng * S->mstride * sizeof(double));
+        S->bI.ensure((size_t)nq * sizeof(double));
+        S->by.upload(data, nq * sizeof(double), s);
+        S->bsig.upload(sigma, nq * sizeof(double), s);
+        S->bchi.ensure(sizeof(double));
+        S->bparam.ensure(2 * sizeof(double));
+        S->host_alloc((void **)&S->h_param, 2 * sizeof(double));
+        S->host_alloc((void **)&S->h_out, (nq + 1) * sizeof(double));
+        check_hip(hipStreamSynchronize(s), "session setup");
+        *out = (dplus_session *)S.release();
+    })
+}
+
+int dplus_session_evaluate(dplus_session *sh, const dplus_leaf *placements, int nleaf, double fit_scale,
+                           double fit_constant, double *intensity, double *chi2) {
+    ENGINE_GUARD({
+        Session *S = (Session *)sh;
+        if (!S || !placements || nleaf != (int)S->leaves.size()) throw Error(kErrInvalidArgs, "bad evaluation arguments");
+        Device::get().require();
+        set_alloc_stream(S->s);
+        struct Restore {
+            ~Restore() { set_alloc_stream(nullptr); }
+        } restore;
+        size_t nrot = 0, ntr = 0;
+        for (int l = 0; l < nleaf; l++) {
+            if (placements[l].grid && (Grid *)placements[l].grid != S->leaves[l])
+                throw Error(kErrInvalidArgs, "placement leaf differs from the session's leaf");
+            nrot += (size_t)placements[l].nrot;
+            for (int r = 0; r < placements[l].nrot; r++) {
+                if (placements[l].ntrans[r] < 0) throw Error(kErrInvalidArgs, "negative translation count");
+                ntr += (size_t)placements[l].ntrans[r];
+            }
+        }
+        if (nrot == 0) throw Error(kErrInvalidArgs, "no placements");
+        S->grow(nrot, std::max<size_t>(1, ntr));
+        size_t ri = 0, ti = 0;
+        for (int l = 0; l < nleaf; l++) {
+            const dplus_leaf &L = placements[l];
+            size_t t0 = 0;
+            for (int r = 0; r < L.nrot; r++) {
+                RotDev &rd = S->h_rot[ri++];
+                euler_rt(L.rot[4 * r], L.rot[4 * r + 1], L.rot[4 * r + 2], rd.m);
+                rd.scale = L.rot[4 * r + 3];
+                rd.leaf = l;
+                rd.t0 = (int)ti;
+                rd.nt = L.ntrans[r];
+                rd.pad = 0;
+                for (int t = 0; t < L.ntrans[r]; t++, t0++)
+                    S->h_tr[ti++] = make_double4(L.trans[3 * t0], L.trans[3 * t0 + 1], L.trans[3 * t0 + 2], 0.0);
+            }
+        }
+        S->h_param[0] = fit_scale;
+        S->h_param[1] = fit_constant;
+        if (S->graph_nrot != (int)nrot) S->capture((int)nrot);
+        check_hip(hipGraphLaunch(S->exec, S->s), "graph launch");
+        check_hip(hipStreamSynchronize(S->s), "session evaluation");
+        if (intensity) std::copy(S->h_out, S->h_out + S->nq, intensity);
+        if (chi2) *chi2 = S->h_out[S->nq];
+        S->evaluations++;
+    })
+}
+
+int dplus_session_info(dplus_session *sh, long long *evaluations, long long *graph_captures) {
+    ENGINE_GUARD({
+        Session *S = (Session *)sh;
+        if (!S) throw Error(kErrInvalidArgs, "null session");
+        if (evaluations) *evaluations = S->evaluations;
+        if (graph_captures) *graph_captures = S->captures;
+    })
+}
+
+void dplus_session_destroy(dplus_session *s) {
+    Session *S = (Session *)s;
+    if (!S) return;
+    set_alloc_stream(S->s);
+    delete S;
+    set_alloc_stream(nullptr);
+}
+
+}  // extern "C"

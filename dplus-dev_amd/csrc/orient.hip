@@ -253,6 +253,76 @@ __global__ void mc_reduce(const GroupDev *__restrict__ groups, const int *__rest
     }
 }
 
+// intensity of every q from the group sums of a fixed-count MC average:
+// I[q0 + j] = out[gi * mstride + j] / total (one block per group)
+__global__ void mc_finish(const GroupDev *__restrict__ groups, int mstride, const double *__restrict__ out,
+                          double total, double *__restrict__ I) {
+    const GroupDev g = groups[blockIdx.x];
+    for (int j = threadIdx.x; j < g.m; j += blockDim.x) I[g.q0 + j] = out[(size_t)blockIdx.x * mstride + j] / total;
+}
+
+// chi^2 = sum_q ((scale * I_q + constant - y_q) / sigma_q)^2 in one block, in a
+// fixed summation order; q with sigma <= 0 are skipped
+__global__ __launch_bounds__(1024) void chi2_kernel(const double *__restrict__ I, const double *__restrict__ y,
+                                                    const double *__restrict__ sig, int n, double scale,
+                                                    double constant, double *__restrict__ chi2) {
+    __shared__ double part[1024];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double sg = sig[i];
+        if (sg > 0.0) {
+            const double r = (scale * I[i] + constant - y[i]) / sg;
+            acc = fma(r, r, acc);
+        }
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *chi2 = part[0];
+}
+
+void launch_mc_finish(const GroupDev *groups, int ngroups, int mstride, const double *out, double total, double *I,
+                      hipStream_t s) {
+    mc_finish<<<ngroups, 64, 0, s>>>(groups, mstride, out, total, I);
+}
+
+void launch_chi2(const double *I, const double *y, const double *sig, int n, double scale, double constant,
+                 double *chi2, hipStream_t s) {
+    chi2_kernel<<<1, 1024, 0, s>>>(I, y, sig, n, scale, constant, chi2);
+}
+
+// the same with (scale, constant) read from device memory, for captured graphs
+// whose fit parameters change between replays
+__global__ __launch_bounds__(1024) void chi2_param_kernel(const double *__restrict__ I, const double *__restrict__ y,
+                                                          const double *__restrict__ sig, int n,
+                                                          const double *__restrict__ sc, double *__restrict__ chi2) {
+    __shared__ double part[1024];
+    const double scale = sc[0], constant = sc[1];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double sg = sig[i];
+        if (sg > 0.0) {
+            const double r = (scale * I[i] + constant - y[i]) / sg;
+            acc = fma(r, r, acc);
+        }
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *chi2 = part[0];
+}
+
+void launch_chi2_param(const double *I, const double *y, const double *sig, int n, const double *scale_constant,
+                       double *chi2, hipStream_t s) {
+    chi2_param_kernel<<<1, 1024, 0, s>>>(I, y, sig, n, scale_constant, chi2);
+}
+
 // ------------------------------------------------------------------ VEGAS
 constexpr int kVB = 32;  // bins per axis (GPUHybridCalc.cu:30-31)
 

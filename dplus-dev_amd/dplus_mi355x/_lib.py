@@ -58,6 +58,27 @@ class OAStats(ctypes.Structure):
                 ("evaluations", ctypes.c_longlong), ("seconds", ctypes.c_double)]
 
 
+class EngineConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_longlong), ("streams", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("pdb", ctypes.POINTER(PdbDesc)), ("grid", ctypes.c_void_p), ("grid_size", ctypes.c_int),
+        ("qmax", ctypes.c_double), ("precision", ctypes.c_int), ("nrot", ctypes.c_int), ("rot", _D),
+        ("ntrans", _I), ("trans", _D), ("nq", ctypes.c_int), ("q", _D), ("oa", OAParams), ("data", _D),
+        ("sigma", _D), ("fit_scale", ctypes.c_double), ("fit_constant", ctypes.c_double),
+    ]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [
+        ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
+        ("chi2", ctypes.c_double), ("chi2_on_device", ctypes.c_int), ("slot", ctypes.c_int),
+        ("reserved_bytes", ctypes.c_longlong), ("oa", OAStats),
+    ]
+
+
 # every symbol of include/dplus_mi355x.h with its ctypes signature
 SIGNATURES = {
     "dplus_last_error": (ctypes.c_char_p, []),
@@ -111,6 +132,19 @@ SIGNATURES = {
                                                    ctypes.POINTER(ctypes.c_longlong)]),
     "dplus_backend_save_amplitude": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_char_p]),
     "dplus_backend_stop": (ctypes.c_int, [ctypes.c_void_p]),
+    "dplus_engine_create": (ctypes.c_int, [ctypes.POINTER(EngineConfig), ctypes.POINTER(ctypes.c_void_p)]),
+    "dplus_engine_destroy": (None, [ctypes.c_void_p]),
+    "dplus_engine_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]),
+    "dplus_engine_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _I]),
+    "dplus_engine_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _D, ctypes.POINTER(RequestResult)]),
+    "dplus_engine_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 5),
+    "dplus_session_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _D, _D, _D,
+                                            ctypes.c_ulonglong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)]),
+    "dplus_session_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Leaf), ctypes.c_int, ctypes.c_double,
+                                              ctypes.c_double, _D, _D]),
+    "dplus_session_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+                                          ctypes.POINTER(ctypes.c_longlong)]),
+    "dplus_session_destroy": (None, [ctypes.c_void_p]),
 }
 
 _lib = None

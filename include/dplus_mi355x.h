@@ -245,6 +245,78 @@ int dplus_backend_save_amplitude(dplus_backend *b, unsigned model_ptr, const cha
 /* Stop (PythonBackendWrapper.cpp:225-235): the running job ends with ERROR_STOPPED (1) */
 int dplus_backend_stop(dplus_backend *b);
 
+/* ------------------------------------------------------------------ request engine */
+/* On-device serving of I(q) requests; no counterpart in the reference, whose
+ * backend runs one job per process (JobManager.cpp:137-205).  Requests are
+ * admitted in arrival order as device slots (one HIP stream each) and the HBM
+ * budget allow, run concurrently, and give the same result alone or in a mix. */
+typedef struct dplus_engine dplus_engine;
+
+typedef struct {
+    long long hbm_budget_bytes; /* admission budget; <= 0: 85 % of device memory */
+    int streams;                /* concurrent requests (1..16); <= 0: 4 */
+} dplus_engine_config;
+
+typedef struct {
+    const dplus_pdb_desc *pdb; /* a PDB leaf built for this request, or NULL with `grid` */
+    dplus_grid *grid;          /* a resident leaf grid (outlives the request) instead of `pdb` */
+    int grid_size;             /* grid of the PDB leaf */
+    double qmax;
+    int precision;             /* DPLUS_PRECISION_* of the PDB leaf */
+    int nrot;                  /* placements as in dplus_leaf; 0: the leaf once at the origin */
+    const double *rot;
+    const int *ntrans;
+    const double *trans;
+    int nq;
+    const double *q;           /* ascending */
+    dplus_oa_params oa;
+    const double *data;        /* optional: chi^2 of fit_scale * I + fit_constant against data/sigma */
+    const double *sigma;
+    double fit_scale;
+    double fit_constant;
+} dplus_request;
+
+typedef struct {
+    double queue_ms;    /* submission to start on the device */
+    double run_ms;      /* on the device */
+    double latency_ms;  /* submission to result */
+    double chi2;
+    int chi2_on_device; /* 1 when chi^2 was reduced on the device */
+    int slot;           /* stream the request ran on */
+    long long reserved_bytes;
+    dplus_oa_stats oa;
+} dplus_request_result;
+
+int dplus_engine_create(const dplus_engine_config *cfg, dplus_engine **out);
+/* waits for every admitted and queued request, then releases the engine */
+void dplus_engine_destroy(dplus_engine *e);
+/* copies the request's inputs and queues it; ERROR_OUT_OF_MEMORY (13) when it
+ * could never fit the budget */
+int dplus_engine_submit(dplus_engine *e, const dplus_request *r, long long *ticket);
+int dplus_engine_poll(dplus_engine *e, long long ticket, int *done);
+/* blocks until the request is done, writes nq intensities and releases the ticket */
+int dplus_engine_wait(dplus_engine *e, long long ticket, double *intensity, dplus_request_result *res);
+int dplus_engine_stats(dplus_engine *e, long long *submitted, long long *completed, long long *peak_reserved,
+                       long long *budget, long long *max_concurrent);
+
+/* ------------------------------------------------------------------ fit sessions */
+/* Fitting-loop evaluation over resident subunit grids (the hybrid docking of
+ * Amplitude.cpp:2028-2303 with DomainModel's MC average, OAJacob.cu:132-383),
+ * replayed as one HIP graph per placement shape; chi^2 against data/sigma is
+ * reduced on the device.  No counterpart in the reference's API. */
+typedef struct dplus_session dplus_session;
+/* leaves: resident grids (same size, qmax, precision); fixed q, data, sigma,
+ * seed and direction count (rounded up to a multiple of 256) */
+int dplus_session_create(int nleaf, dplus_grid *const *leaves, int nq, const double *q, const double *data,
+                         const double *sigma, unsigned long long seed, long long directions,
+                         dplus_session **out);
+/* placements[l] places leaf l (its grid field may be NULL); writes nq
+ * intensities and chi^2 of fit_scale * I + fit_constant */
+int dplus_session_evaluate(dplus_session *s, const dplus_leaf *placements, int nleaf, double fit_scale,
+                           double fit_constant, double *intensity, double *chi2);
+int dplus_session_info(dplus_session *s, long long *evaluations, long long *graph_captures);
+void dplus_session_destroy(dplus_session *s);
+
 /* ------------------------------------------------------------------ synthetic inputs */
 /* Seeded self-avoiding random walk (step, minimum separation in nm) confined to
  * the ellipsoid with semi-axes (ax, ay, az); xyz receives 3*n coordinates. */

@@ -1,0 +1,157 @@
+"""Serving on the GPU: the request engine gives every request the same result
+alone and in a concurrent mix, admits by HBM budget, and matches the FP64
+oracle; fit sessions replay captured graphs with the same intensities as the
+eager orientation average and chi^2 reduced on the device."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_pdb(n, seed, radius):
+    import dplus_mi355x.synth as S
+    xyz, el = S.walk_structure(seed, n, (radius, radius, radius))
+    coef9, rad, ion, _ = S.structure_tables(el)
+    return dict(xyz=xyz, ion=ion, coef9=coef9, rad=rad, sol_ed=333.0, c1=1.0, flags=3, scale=1.0)
+
+
+def requests(D):
+    import dplus_mi355x.synth as S
+    out = []
+    rng = np.random.default_rng(5)
+    for i in range(10):
+        n = int(rng.integers(150, 2500))
+        qmax = float(rng.uniform(2.0, 6.0))
+        gs = int(2 * round(rng.uniform(12, 36) / 2))
+        q = np.linspace(0.02, qmax, int(rng.integers(40, 160)))
+        kw = dict(pdb=make_pdb(n, 100 + i, 1.0 + n / 2500.0), grid_size=gs, qmax=qmax,
+                  precision=D.PRECISION_FP64 if i % 4 == 3 else D.PRECISION_MIXED, seed=7 + i)
+        if i % 2:
+            kw.update(method=D.OA_VEGAS, vegas_iterations=3, vegas_points_per_bin=4)
+        else:
+            kw.update(method=D.OA_MC, max_dirs=8192, chunk=4096)
+        if i % 3 == 1:
+            kw["placements"] = S.rows_to_docking(S.helix_rows(6, 2.0, 0.7, 31.0))
+        if i % 3 == 2:
+            kw.update(data=np.ones(len(q)), sigma=np.full(len(q), 0.5), fit_scale=1e-6, fit_constant=0.1)
+        out.append((q, kw))
+    return out
+
+
+def test_engine_same_alone_and_in_the_mix(dplus):
+    from dplus_mi355x.serving import Engine
+    say("[case] engine: 10 mixed requests alone (1 stream, one at a time) vs concurrently (4 streams)")
+    reqs = requests(dplus)
+    alone = []
+    e1 = Engine(streams=1)
+    for q, kw in reqs:
+        alone.append(e1.wait(e1.submit(q, **kw)))
+    e1.close()
+    e4 = Engine(streams=4)
+    tickets = [e4.submit(q, **kw) for q, kw in reqs]
+    mix = [e4.wait(t) for t in tickets]
+    st = e4.stats()
+    e4.close()
+    say(f"    engine stats {st}")
+    assert st["completed"] == len(reqs) and st["max_concurrent"] > 1
+    for (I1, r1), (I4, r4), (q, kw) in zip(alone, mix, reqs):
+        assert np.array_equal(I1, I4), kw.get("method")
+        assert r1["chi2"] == r4["chi2"]
+        if "data" in kw:
+            model = kw["fit_scale"] * I1 + kw["fit_constant"]
+            ref = float(np.sum(((model - kw["data"]) / kw["sigma"]) ** 2))
+            assert r1["chi2"] == pytest.approx(ref, rel=1e-12)
+            assert r1["chi2_on_device"] == 1
+
+
+def test_engine_matches_oracle(dplus, oracle):
+    from dplus_mi355x.serving import Engine
+    say("[case] engine request against the FP64 oracle")
+    import dplus_mi355x.synth as S
+    pdb = make_pdb(400, 3, 1.2)
+    gs, qmax = 20, 4.0
+    q = np.linspace(0.05, qmax, 60)
+    rows = S.helix_rows(5, 1.5, 0.6, 40.0)
+    rot, nt, tr = S.rows_to_docking(rows)
+    ions = oracle.Ions(pdb["coef9"], pdb["rad"], np.bincount(pdb["ion"], minlength=len(pdb["coef9"])), 333.0, 1.0,
+                       oracle.CALC_ATOMIC_FORMFACTORS | oracle.CALC_DUMMY_SOLVENT)
+    data = oracle.grid_fill_pdb(gs, qmax, pdb["xyz"], pdb["ion"], ions)
+    co = oracle.grid_splines(gs, data)
+    ref, _ = oracle.mc_oa(oracle.Docked([(gs, qmax, data, co, rot, nt, tr)]), q, 11, 8192, 4096)
+    e = Engine(streams=2)
+    for prec, tol in ((dplus.PRECISION_FP64, 1e-6), (dplus.PRECISION_MIXED, 1e-4)):
+        I, _ = e.wait(e.submit(q, pdb=pdb, grid_size=gs, qmax=qmax, precision=prec, placements=(rot, nt, tr),
+                               method=dplus.OA_MC, seed=11, max_dirs=8192, chunk=4096))
+        assert np.max(np.abs(I - ref) / ref) < tol
+    e.close()
+
+
+def test_engine_admission_budget_and_resident_grid(dplus):
+    from dplus_mi355x.serving import Engine
+    say("[case] engine admission by HBM budget; resident grid request")
+    pdb = make_pdb(300, 9, 1.0)
+    q = np.linspace(0.05, 3.0, 50)
+    kw = dict(pdb=pdb, grid_size=24, qmax=3.0, method=dplus.OA_MC, seed=2, max_dirs=4096, chunk=4096)
+    probe = Engine(streams=1)
+    I0, r0 = probe.wait(probe.submit(q, **kw))
+    probe.close()
+    need = r0["reserved_bytes"]
+    tight = Engine(hbm_budget_bytes=need + need // 2, streams=4)  # room for one request at a time
+    ts = [tight.submit(q, **kw) for _ in range(6)]
+    outs = [tight.wait(t)[0] for t in ts]
+    st = tight.stats()
+    assert st["max_concurrent"] == 1 and st["peak_reserved"] <= st["budget"]
+    assert all(np.array_equal(o, I0) for o in outs)
+    with pytest.raises(dplus.DplusError) as err:
+        tight.submit(q, **dict(kw, grid_size=200))
+    assert err.value.error_code == 13
+    tight.close()
+    # a resident grid gives the same curve as the direct average of that grid
+    st_ = dplus.Structure(pdb["xyz"], pdb["ion"], pdb["coef9"], pdb["rad"], sol_ed=333.0, flags=3)
+    g = dplus.Grid(24, 3.0, dplus.PRECISION_MIXED)
+    st_.build(g)
+    direct, _ = dplus.orientation_average(dplus.Docking.single(g), q, seed=2, max_dirs=4096, chunk=4096)
+    e = Engine(streams=2)
+    Ig, _ = e.wait(e.submit(q, grid=g, method=dplus.OA_MC, seed=2, max_dirs=4096, chunk=4096))
+    e.close()
+    assert np.array_equal(Ig, direct)
+    assert np.array_equal(Ig, I0)
+
+
+def test_fit_session_graph_replays(dplus):
+    from dplus_mi355x.serving import FitSession
+    import dplus_mi355x.synth as S
+    say("[case] fit session: captured graph replays vs eager averages, chi^2 on the device")
+    pdb = make_pdb(500, 21, 1.2)
+    g = dplus.Grid(24, 4.0, dplus.PRECISION_MIXED)
+    dplus.Structure(pdb["xyz"], pdb["ion"], pdb["coef9"], pdb["rad"], sol_ed=333.0, flags=3).build(g)
+    q = np.linspace(0.05, 4.0, 80)
+    y = np.linspace(2.0, 1.0, 80)
+    sig = np.full(80, 0.1)
+    dirs = 16384
+    sess = FitSession([g], q, y, sig, seed=4, directions=dirs)
+    cases = [(12, 2.0, 0.8, 30.0), (12, 2.2, 0.8, 31.5), (12, 2.0, 0.8, 30.0), (20, 1.8, 0.7, 25.0)]
+    prev = {}
+    for n, rad, rise, twist in cases:
+        rot, nt, tr = S.rows_to_docking(S.helix_rows(n, rad, rise, twist))
+        I, c2 = sess.evaluate([(rot, nt, tr)], fit_scale=1e-5, fit_constant=0.2)
+        eager, _ = dplus.orientation_average(dplus.Docking([(g, rot, nt, tr)]), q, seed=4, max_dirs=dirs,
+                                             chunk=8192)
+        assert np.array_equal(I, eager)
+        ref = float(np.sum(((1e-5 * eager + 0.2 - y) / sig) ** 2))
+        assert c2 == pytest.approx(ref, rel=1e-12)
+        key = (n, rad, rise, twist)
+        if key in prev:
+            assert np.array_equal(prev[key], I)
+        prev[key] = I
+    info = sess.info()
+    say(f"    session {info}")
+    assert info["evaluations"] == 4 and info["graph_captures"] == 2
+    sess.close()
