@@ -176,6 +176,90 @@ __global__ __launch_bounds__(256) void build_atoms_mixed(
     }
 }
 
+// Mixed-precision builder on row-aligned node sextets.  Every shell row of the
+// canonical half holds 6i nodes and starts at a multiple of 6, so the six
+// consecutive canonical nodes a thread owns share the row's polar angle: q_z
+// (and |q|) is common, and q_z z is formed once per atom for all six.  With
+// kReduce false the phase (in revolutions) goes to v_sin_f32 / v_cos_f32
+// unreduced; the host selects that form only when every |q.r| / 2 pi of the
+// structure is below 128 revolutions, inside the units' input domain.
+constexpr int kRing = 6;
+
+template <bool kReduce>
+__global__ __launch_bounds__(256) void build_atoms_ring(
+    const float4 *__restrict__ atoms, const int4 *__restrict__ tiles, int ntiles, const double *__restrict__ ff,
+    int nion, const int4 *__restrict__ bdims, double vox, double contrast, double step, long long nhalf,
+    float4 *__restrict__ grid, double scale) {
+    const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * kRing;
+    if (c0 >= nhalf) return;
+    // the sextet's row: shell i, polar index j, azimuthal indices k0 .. k0+5
+    const int shell = shell_of_half(c0);
+    const long long h = c0 - half_base(shell);
+    const long long j = h / (6LL * shell), k0 = h % (6LL * shell);
+    float qx[kRing], qy[kRing], qz;
+    double accR[kRing], accI[kRing];
+    const double inv2pi = 1.0 / k2Pi;
+    {
+        const double q = (double)shell * step;
+        double st, ct;
+        sincos(kPi * (double)j / (double)(3 * shell), &st, &ct);
+        qz = (float)(q * ct * inv2pi);
+#pragma unroll
+        for (int p = 0; p < kRing; p++) {
+            double sp, cp;
+            sincos(k2Pi * (double)(k0 + p) / (double)(6 * shell), &sp, &cp);
+            qx[p] = (float)(q * st * cp * inv2pi);
+            qy[p] = (float)(q * st * sp * inv2pi);
+            accR[p] = 0.0;
+            accI[p] = 0.0;
+        }
+    }
+    for (int t = 0; t < ntiles; t++) {
+        const int4 tl = tiles[t];
+        float sr[kRing], si[kRing];
+#pragma unroll
+        for (int p = 0; p < kRing; p++) sr[p] = si[p] = 0.f;
+        const int a1 = tl.x + tl.y;
+#pragma unroll 2
+        for (int a = tl.x; a < a1; a++) {
+            const float4 at = atoms[a];
+            const float zz = qz * at.z;
+#pragma unroll
+            for (int p = 0; p < kRing; p++) {
+                const float ph = fmaf(qx[p], at.x, fmaf(qy[p], at.y, zz));
+                const float r = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
+                sr[p] += __builtin_amdgcn_cosf(r);
+                si[p] += __builtin_amdgcn_sinf(r);
+            }
+        }
+        if (tl.w == 0) {
+            const double f = ff[(size_t)shell * nion + tl.z];
+#pragma unroll
+            for (int p = 0; p < kRing; p++) {
+                accR[p] = fma(f, (double)sr[p], accR[p]);
+                accI[p] = fma(f, (double)si[p], accI[p]);
+            }
+        } else {
+            const int4 d = bdims[tl.z];
+            const float k2 = (float)k2Pi;
+#pragma unroll
+            for (int p = 0; p < kRing; p++) {
+                const double f = contrast * (double)box_weight32(qx[p] * k2, qy[p] * k2, qz * k2, d, (float)vox);
+                accR[p] = fma(f, (double)sr[p], accR[p]);
+                accI[p] = fma(f, (double)si[p], accI[p]);
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < kRing; p++) {
+        const long long k = k0 + p;
+        const float re = (float)(accR[p] * scale), im = (float)(accI[p] * scale);
+        grid[index_from_indices(shell, j, k)] = make_float4(re, im, 0.f, 0.f);
+        grid[index_from_indices(shell, 3LL * shell - j, (k + 3LL * shell) % (6LL * shell))] =
+            make_float4(re, -im, 0.f, 0.f);
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void build_atoms_fp64(
     const double4 *__restrict__ atoms, const int4 *__restrict__ tiles, int ntiles,
@@ -268,12 +352,22 @@ static inline unsigned blocks_for(long long n, int per_block) {
 
 constexpr int kNodesPerThread = 4;
 
-void launch_build_atoms(bool fp64, bool phase64, const void *atoms, const void *atoms64,
+void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms, const void *atoms64,
                         const int4 *tiles, int ntiles, const double *ff, int nion, const int4 *bdims,
                         double vox, double contrast, double step, long long nhalf, void *grid,
                         double scale, hipStream_t s) {
     const int K = kNodesPerThread;
     unsigned nb = blocks_for(nhalf, 256 * K);
+    if (!fp64 && !phase64) {
+        const unsigned nr = blocks_for(nhalf, 256 * kRing);
+        if (reduce)
+            build_atoms_ring<true><<<nr, 256, 0, s>>>((const float4 *)atoms, tiles, ntiles, ff, nion, bdims, vox,
+                                                     contrast, step, nhalf, (float4 *)grid, scale);
+        else
+            build_atoms_ring<false><<<nr, 256, 0, s>>>((const float4 *)atoms, tiles, ntiles, ff, nion, bdims, vox,
+                                                      contrast, step, nhalf, (float4 *)grid, scale);
+        return;
+    }
     if (fp64) {
         build_atoms_fp64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, bdims,
                                                vox, contrast, step, nhalf, (double4 *)grid, scale);

@@ -256,9 +256,9 @@ void Grid::upload(hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ PDB builder
-void launch_build_atoms(bool fp64, bool phase64, const void *atoms, const void *atoms64, const int4 *tiles,
-                        int ntiles, const double *ff, int nion, const int4 *bdims, double vox, double contrast,
-                        double step, long long nhalf, void *grid, double scale, hipStream_t s);
+void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms, const void *atoms64,
+                        const int4 *tiles, int ntiles, const double *ff, int nion, const int4 *bdims, double vox,
+                        double contrast, double step, long long nhalf, void *grid, double scale, hipStream_t s);
 void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s);
 
 // Atomic Form Factor.cpp:11-27 (cube-mean radius), 91-115 (dummy-atom
@@ -383,6 +383,9 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
     this->boxes_on = boxes_on;
     ntiles = (int)tiles.size();
     nsources = (long long)nsrc;
+    rmax = 0.0;
+    for (size_t i = 0; i < nsrc; i++)
+        rmax = std::max(rmax, std::sqrt(at64[i].x * at64[i].x + at64[i].y * at64[i].y + at64[i].z * at64[i].z));
     b32.upload(at32.data(), at32.size() * sizeof(float4), s);
     b64.upload(at64.data(), at64.size() * sizeof(double4), s);
     bt.upload(tiles.data(), std::max<size_t>(1, tiles.size()) * sizeof(int4), s);
@@ -403,7 +406,10 @@ void Structure::build(Grid &g, bool host_copy, hipStream_t s) {
     const bool phase64 = getenv("DPLUS_PHASE64") != nullptr;
     {
         KTimer kt("build", s);
-        launch_build_atoms(g.fp64(), phase64, b32.p, b64.p, (const int4 *)bt.p, ntiles, (const double *)bff.p,
+        // unreduced hardware sin/cos while every phase stays below 128 revolutions
+        const double qtop = (double)(g.N + 1) * g.step;
+        const bool reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
+        launch_build_atoms(g.fp64(), phase64, reduce, b32.p, b64.p, (const int4 *)bt.p, ntiles, (const double *)bff.p,
                            desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast, g.step, nhalf, g.dev.p,
                            desc.scale, s);
         check_hip(hipGetLastError(), "build_atoms launch");

@@ -161,6 +161,7 @@ struct Structure {
     std::vector<double> coef9, rad;
     std::vector<int> ion_counts;
     double box_volume = 0.0, last_terms = 0.0;
+    double rmax = 0.0;  // largest |position| of an atom or box centre (nm)
     bool atoms_on = false, boxes_on = false;
     int ntiles = 0;
     long long nsources = 0;
