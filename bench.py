@@ -128,7 +128,7 @@ def main():
         st.build(g32)
         nbytes = 256 * 2 ** 20
         ms = D.copy_probe(nbytes)
-        print(json.dumps({"probe": "build_atoms_mixed + copy_probe_kernel", "copy_bytes": nbytes,
+        print(json.dumps({"probe": "build_atoms_ring + copy_probe_kernel", "copy_bytes": nbytes,
                           "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
         return
     g64 = D.Grid(W["gs"], W["qmax"], D.PRECISION_FP64)
@@ -178,7 +178,7 @@ def main():
     # HBM bytes per builder launch from the two rocprofv3 --pmc passes of this workload
     # (FETCH_SIZE, WRITE_SIZE, each scaled by a known-size copy), kept in profiles/
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "build_atoms_mixed_c2_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", "build_atoms_ring_c2_pmc.json")
     if os.path.exists(pmc):
         traffic = json.load(open(pmc)).get("traffic_bytes")
 
@@ -214,7 +214,7 @@ def main():
         "max_rel_err_vs_fp64": err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "build_atoms_mixed", "launch_ms": t_launch * 1e3,
+                     "kernel": "build_atoms_ring", "launch_ms": t_launch * 1e3,
                      "terms_per_launch": terms, "flop_eq_per_term": flop_eq,
                      "transcendental_weight": w_t},
         "cpu_baseline": cpu,
