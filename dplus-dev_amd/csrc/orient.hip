@@ -197,7 +197,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // grid: x = direction block, y = active group.  mstride >= every group's m.
 // partial[(gi * nblk + bx) * mstride + j] = sum over the block's directions of |A_j|^2
 template <typename T, typename T4>
-__global__ __launch_bounds__(256) void mc_kernel(const T4 *const *__restrict__ grids,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T4 *const *__restrict__ grids,
                                                  const RotDev *__restrict__ rots, int nrot,
                                                  const double4 *__restrict__ trans,
                                                  const GroupDev *__restrict__ groups,
@@ -326,7 +326,7 @@ void launch_chi2_param(const double *I, const double *y, const double *sig, int 
 // ------------------------------------------------------------------ VEGAS
 constexpr int kVB = 32;  // bins per axis (GPUHybridCalc.cu:30-31)
 
-constexpr int kVM = 16;        // q-values per pass in the VEGAS kernels
+constexpr int kVM = 8;         // q-values per pass in the VEGAS kernels
 constexpr int kVPts = 4;       // points per lane and bin: points per bin <= 256
 
 // bounds[g*66 + 0..32] theta-axis boundaries in u = (1-cos)/2, [33..65] phi-axis in phi/2pi.
@@ -335,7 +335,7 @@ constexpr int kVPts = 4;       // points per lane and bin: points per bin <= 256
 // binvar[gi * 1024 + bin] = ppb * sum_p (vol * S_p - mean)^2 with S_p = sum_j I_j(p)
 // (HybridVEGASBinReduceToIKernel / HybridVEGASBinReduceKernel, HybridOA.cu:300-449).
 template <typename T, typename T4>
-__global__ __launch_bounds__(256) void vegas_sample(const T4 *const *__restrict__ grids,
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(const T4 *const *__restrict__ grids,
                                                     const RotDev *__restrict__ rots, int nrot,
                                                     const double4 *__restrict__ trans,
                                                     const GroupDev *__restrict__ groups,
