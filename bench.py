@@ -99,6 +99,180 @@ def cpu_baseline(W, g64_data, g64_interp, budget_nodes=1500):
                       f"(scaled x{W['iters'] * W['ppb']}); full curve estimate {total:.0f} s"}
 
 
+def serving_stream(D, S, n_requests=512, rate=400.0, pool_size=48, seed=4):
+    """Batched serving stream: n_requests with Poisson arrivals at `rate`/s; each
+    request is one of `pool_size` seeded random-walk structures (atoms
+    log-uniform in 500..200,000 at 50 atoms/nm^3, dummy-atom solvent), qmax
+    U(2, 8) nm^-1, grid size round-up-10(2 qmax L / pi + 3) with L the structure's
+    diameter, 200..1,000 q points, MC (65,536 directions) or VEGAS (5 x 4,096
+    points) 50/50.  The mixed engine (4 streams, 200 GB HBM budget) serves the
+    stream; the plain FP64 path then runs the same requests one at a time."""
+    from dplus_mi355x.serving import Engine
+    rng = np.random.default_rng(seed)
+    t0 = time.time()
+    pool = []
+    for k in range(pool_size):
+        n = int(round(np.exp(rng.uniform(np.log(500), np.log(200000)))))
+        R = (3.0 * n / (4.0 * np.pi * 50.0)) ** (1.0 / 3.0)
+        xyz, el = S.walk_structure(5000 + k, n, (R, R, R))
+        coef9, rad, ion, _ = S.structure_tables(el)
+        L = 2.0 * float(np.max(np.linalg.norm(xyz, axis=1)))
+        pool.append((dict(xyz=xyz, ion=ion, coef9=coef9, rad=rad, sol_ed=333.0, c1=1.0, flags=3, scale=1.0), L, n))
+    reqs = []
+    for i in range(n_requests):
+        pdb, L, n = pool[int(rng.integers(pool_size))]
+        qmax = float(rng.uniform(2.0, 8.0))
+        gs = int(np.ceil((2.0 * qmax * L / np.pi + 3.0) / 10.0) * 10)
+        q = np.linspace(0.0, qmax, int(rng.integers(200, 1001)))
+        kw = dict(pdb=pdb, grid_size=gs, qmax=qmax, seed=1000 + i)
+        if rng.random() < 0.5:
+            kw.update(method=D.OA_MC, max_dirs=1 << 16, chunk=8192)
+        else:
+            kw.update(method=D.OA_VEGAS, vegas_iterations=5, vegas_points_per_bin=4)
+        reqs.append((q, kw, n))
+    log(f"[serving] {n_requests} requests over {pool_size} structures prepared in {time.time() - t0:.1f} s")
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+    eng = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=4)
+    tickets = []
+    start = time.perf_counter()
+    for (q, kw, _), at in zip(reqs, arrivals):
+        lag = start + at - time.perf_counter()
+        if lag > 0:
+            time.sleep(lag)
+        tickets.append(eng.submit(q, precision=D.PRECISION_MIXED, **kw))
+    mixed = [eng.wait(t) for t in tickets]
+    span = time.perf_counter() - start
+    st = eng.stats()
+    eng.close()
+    lat = np.array([r["latency_ms"] for _, r in mixed])
+    log(f"[serving] mixed: {n_requests / span:.2f} curves/s over {span:.2f} s, p50 {np.percentile(lat, 50):.1f} ms, "
+        f"p99 {np.percentile(lat, 99):.1f} ms, max concurrent {st['max_concurrent']}")
+    eng64 = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=1)
+    t64 = time.perf_counter()
+    errs, lat64 = [], []
+    for (q, kw, _), (Im, _) in zip(reqs, mixed):
+        I64, r64 = eng64.wait(eng64.submit(q, precision=D.PRECISION_FP64, **kw))
+        lat64.append(r64["latency_ms"])
+        errs.append(float(np.max(np.abs(Im - I64) / np.abs(I64))))
+    span64 = time.perf_counter() - t64
+    eng64.close()
+    log(f"[serving] plain FP64, one request at a time: {n_requests / span64:.2f} curves/s; "
+        f"max rel err mixed vs FP64 {max(errs):.2e}")
+    return {"workload": f"{n_requests} Poisson requests at {rate:g}/s over {pool_size} seeded structures "
+                        "(500..200,000 atoms), qmax U(2,8), 200..1,000 q, MC 65,536 dirs or VEGAS 5x4,096",
+            "value": n_requests / span, "unit": "curves/s", "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "fp64_value": n_requests / span64,
+            "fp64_p50_ms": float(np.percentile(lat64, 50)), "fp64_p99_ms": float(np.percentile(lat64, 99)),
+            "speedup_vs_fp64": (n_requests / span) / (n_requests / span64), "max_rel_err_vs_fp64": max(errs),
+            "streams": 4, "max_concurrent": st["max_concurrent"], "peak_reserved_bytes": st["peak_reserved"],
+            "hbm_budget_bytes": st["budget"], "atoms_total": int(sum(n for _, _, n in reqs))}
+
+
+def fit_sessions(D, S, n_sessions=16, evals=4, directions=8192, nq=600, qmax=5.0, seed=5):
+    """Fitting-loop serving: n_sessions concurrent sessions, each with 1..4 cached
+    subunit grids of 2,000..20,000 atoms (grid 150..250, FP32) placed by a helix
+    (radius 6..12 nm, +-10 %; twist 20..40 deg, +-5 deg) or a 60-fold icosahedral
+    shell (radius 10..20 nm, +-10 %), 60..2,000 copies in total; each evaluation
+    re-docks, averages `directions` MC directions at nq q and reduces chi^2 on the
+    device against Poisson-noised synthetic data.  Sessions run on their own threads."""
+    import threading
+    from dplus_mi355x.serving import FitSession
+    rng = np.random.default_rng(seed)
+    q = np.linspace(0.01, qmax, nq)
+    plans = []
+    t0 = time.time()
+    for s in range(n_sessions):
+        gs = int(rng.choice([150, 200, 250]))
+        nleaf = int(rng.integers(1, 5))
+        copies = int(rng.integers(1, 34)) * 60
+        kind = "helix" if rng.random() < 0.5 else "icosahedral"
+        leaves = []
+        for l in range(nleaf):
+            n = int(rng.integers(2000, 20001))
+            R = (3.0 * n / (4.0 * np.pi * 50.0)) ** (1.0 / 3.0)
+            xyz, el = S.walk_structure(7000 + 10 * s + l, n, (R, R, R))
+            coef9, rad, ion, _ = S.structure_tables(el)
+            leaves.append((xyz, ion, coef9, rad))
+        plans.append(dict(gs=gs, leaves=leaves, copies=copies, kind=kind, r0=float(rng.uniform(6.0, 12.0)),
+                          tw0=float(rng.uniform(20.0, 40.0)), R0=float(rng.uniform(10.0, 20.0)), seed=s))
+
+    def placements(p, scale_r, dtw):
+        per = max(1, p["copies"] // len(p["leaves"]))
+        out = []
+        for l in range(len(p["leaves"])):
+            if p["kind"] == "helix":
+                rows = S.helix_rows(per, p["r0"] * scale_r, 0.9, p["tw0"] + dtw)
+                rows[:, 2] += 0.45 * l
+            else:
+                rows = S.icosahedral_rows(p["R0"] * scale_r, max(60, (per // 60) * 60), seed=p["seed"] * 7 + l)
+            out.append(S.rows_to_docking(rows))
+        return out
+
+    def build(p, prec):
+        gl = []
+        for xyz, ion, coef9, rad in p["leaves"]:
+            g = D.Grid(p["gs"], qmax, prec)
+            D.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=3).build(g)
+            gl.append(g)
+        return gl
+
+    sessions = []
+    for p in plans:
+        grids = build(p, D.PRECISION_MIXED)
+        prob = FitSession(grids, q, np.ones(nq), np.ones(nq), seed=p["seed"], directions=directions)
+        I0, _ = prob.evaluate(placements(p, 1.0, 0.0))
+        prob.close()
+        k = 1e4 / I0.max()
+        y = rng.poisson(I0 * k) / k
+        sig = np.sqrt(np.maximum(y * k, 1.0)) / k
+        sessions.append((p, grids, FitSession(grids, q, y, sig, seed=p["seed"], directions=directions)))
+    log(f"[sessions] {n_sessions} sessions prepared in {time.time() - t0:.1f} s "
+        f"(copies {[p['copies'] for p in plans]})")
+    lat = []
+    lock = threading.Lock()
+
+    def run(sess_tuple, idx):
+        p, grids, sess = sess_tuple
+        r = np.random.default_rng(100 + idx)
+        for _ in range(evals):
+            pl = placements(p, float(r.uniform(0.9, 1.1)), float(r.uniform(-5.0, 5.0)))
+            t = time.perf_counter()
+            sess.evaluate(pl, fit_scale=1.0, fit_constant=0.0)
+            dt = (time.perf_counter() - t) * 1e3
+            with lock:
+                lat.append(dt)
+
+    threads = [threading.Thread(target=run, args=(st, i)) for i, st in enumerate(sessions)]
+    start = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    span = time.perf_counter() - start
+    lat = np.array(lat)
+    # accuracy against the plain FP64 path on the first two sessions' data point
+    errs = []
+    for p, grids, sess in sessions[:2]:
+        g64 = build(p, D.PRECISION_FP64)
+        pl = placements(p, 1.0, 0.0)
+        Im, _ = sess.evaluate(pl)
+        s64 = FitSession(g64, q, sess._y, sess._s, seed=p["seed"], directions=directions)
+        I64, _ = s64.evaluate(pl)
+        s64.close()
+        errs.append(float(np.max(np.abs(Im - I64) / np.abs(I64))))
+    captures = sum(s.info()["graph_captures"] for _, _, s in sessions)
+    for _, _, s in sessions:
+        s.close()
+    n_eval = n_sessions * evals
+    log(f"[sessions] {n_eval / span:.2f} evaluations/s, p50 {np.percentile(lat, 50):.1f} ms, p99 "
+        f"{np.percentile(lat, 99):.1f} ms, graph captures {captures}, max rel err vs FP64 {max(errs):.2e}")
+    return {"workload": f"{n_sessions} concurrent fit sessions x {evals} evaluations (helix / icosahedral re-docking, "
+                        f"{directions} MC directions, {nq} q, chi^2 on device)",
+            "value": n_eval / span, "unit": "evaluations/s", "p50_ms": float(np.percentile(lat, 50)),
+            "p99_ms": float(np.percentile(lat, 99)), "max_rel_err_vs_fp64": max(errs), "graph_captures": captures,
+            "copies": [p["copies"] for p in plans]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +280,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream and fit-session measurements")
+    ap.add_argument("--serving-requests", type=int, default=512)
+    ap.add_argument("--session-evals", type=int, default=4)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="one mixed builder launch plus a 256 MiB device copy (counter calibration), for rocprofv3 --pmc")
     args = ap.parse_args()
@@ -188,6 +365,11 @@ def main():
         cpu = cpu_baseline(W, g64.data.copy(), g64.interp.copy())
         log(f"[bench] cpu baseline: {cpu}")
 
+    serving = sessions = None
+    if not args.no_serving:
+        serving = serving_stream(D, S, n_requests=args.serving_requests)
+        sessions = fit_sessions(D, S, evals=args.session_evals)
+
     value = args.steps / elapsed
     fp64_value = 1.0 / float(np.mean(lat64))
     out = {
@@ -218,6 +400,8 @@ def main():
                      "terms_per_launch": terms, "flop_eq_per_term": flop_eq,
                      "transcendental_weight": w_t},
         "cpu_baseline": cpu,
+        "serving": serving,
+        "fit_sessions": sessions,
     }
     print(json.dumps(out))
 
