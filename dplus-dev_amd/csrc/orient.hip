@@ -47,36 +47,41 @@ __device__ __forceinline__ void four_point(const cplx<T> y[4], cplx<T> &d2, cplx
     d3.y = e * (y[1].y - y[0].y) - c * (y[2].y - y[0].y) + b * (y[3].y - y[1].y) - a * (y[3].y - y[2].y);
 }
 
-// one theta-phi plane (shell >= 1); kZeroTiny reproduces the CPU variant's
-// snapping of interpolation fractions below 1e-10 (Grid.cpp:1242-1245)
+// one theta-phi plane (shell >= 1) at theta = ut pi, phi = up 2 pi; kZeroTiny
+// reproduces the CPU variant's snapping of interpolation fractions below 1e-10
+// (Grid.cpp:1242-1245).  The cell and fractions are the reference's
+// theta / (pi / 3i) and phi / (2 pi / 6i) (CommonJacobGPUMethods.cu:69-154),
+// formed as products with the direction's ut and up; the row and column
+// wrap-arounds are branches instead of integer remainders.
 template <typename T, typename T4, bool kZeroTiny>
-__device__ cplx<T> plane_lookup(const T4 *__restrict__ plane, int shell, double theta, double phi) {
+__device__ cplx<T> plane_lookup(const T4 *__restrict__ plane, int shell, double ut, double up) {
     const int phiPoints = 6 * shell, thePoints = 3 * shell;
-    const double edge = k2Pi / (double)phiPoints;
-    const int tI = (int)((theta / kPi) * (double)thePoints);
-    int pI = (int)((phi / k2Pi) * (double)phiPoints);
-    double tphd = phi / edge - (double)pI;
-    double tthd = theta / edge - (double)tI;
+    const double xt = ut * (double)thePoints, xp = up * (double)phiPoints;
+    const int tI = (int)xt;
+    int pI = (int)xp;
+    double tphd = xp - (double)pI;
+    double tthd = xt - (double)tI;
     if (kZeroTiny) {
         if (fabs(tthd) < 1.0e-10) tthd = 0.0;
         if (fabs(tphd) < 1.0e-10) tphd = 0.0;
     }
     const T tph = (T)tphd, tth = (T)tthd;
     pI = (pI == phiPoints) ? 0 : pI;
-    const int pI1 = (pI + 1) % phiPoints;
+    const int pI1 = (pI + 1 == phiPoints) ? 0 : pI + 1;
     const int half = phiPoints / 2;
     cplx<T> nn[4];
 #pragma unroll
     for (int rl = -1; rl < 3; rl++) {
-        int theInd = (thePoints + 1 + tI + rl) % (thePoints + 1);
+        const int v = tI + rl;  // in [-1, thePoints + 2]
         int pos1, pos2;
-        if (theInd != tI + rl) {
-            if (tI + rl > thePoints) theInd = (tI + rl) - (thePoints + 1);
+        if (v < 0 || v > thePoints) {
+            // past a pole: the row on the other side, half a turn away in phi
+            const int theInd = v < 0 ? v + thePoints + 1 : v - (thePoints + 1);
             pos1 = phiPoints * theInd + pI + ((2 * pI >= phiPoints) ? -half : half);
             pos2 = phiPoints * theInd + pI1 + ((2 * pI1 >= phiPoints) ? -half : half);
         } else {
-            pos1 = phiPoints * theInd + pI;
-            pos2 = phiPoints * theInd + pI1;
+            pos1 = phiPoints * v + pI;
+            pos2 = phiPoints * v + pI1;
         }
         const T4 a = plane[pos1], b = plane[pos2];
         nn[rl + 1] = hermite<T>({a.x, a.y}, {b.x, b.y}, {a.z, a.w}, {b.z, b.w}, tph);
@@ -85,6 +90,9 @@ __device__ cplx<T> plane_lookup(const T4 *__restrict__ plane, int shell, double 
     four_point<T>(nn, d1, d2);
     return hermite<T>(nn[1], nn[2], d1, d2, tth);
 }
+
+constexpr double kInvPi = 0.31830988618379067153776752674503;
+constexpr double kInv2Pi = 0.15915494309189533576888376337251;
 
 __device__ __forceinline__ void antipode(double th, double ph, double &nth, double &nph) {
     double a = kPi - th, b = ph + kPi;
@@ -98,18 +106,19 @@ __device__ __forceinline__ void antipode(double th, double ph, double &nth, doub
 template <typename T, typename T4>
 __device__ __forceinline__ void group_planes(const T4 *__restrict__ g, int layer, double th, double ph,
                                              cplx<T> amps[4], cplx<T> &d1, cplx<T> &d2) {
+    const double ut = th * kInvPi, up = ph * kInv2Pi;
 #pragma unroll
     for (int i = -1; i <= 2; i++) {
         const int lqi = i + layer;
         if (lqi == -1) {
             double nth, nph;
             antipode(th, ph, nth, nph);
-            amps[i + 1] = plane_lookup<T, T4, false>(g + 1, 1, nth, nph);
+            amps[i + 1] = plane_lookup<T, T4, false>(g + 1, 1, nth * kInvPi, nph * kInv2Pi);
         } else if (lqi == 0) {
             const T4 o = g[0];
             amps[i + 1] = {o.x, o.y};
         } else {
-            amps[i + 1] = plane_lookup<T, T4, false>(g + shell_base(lqi), lqi, th, ph);
+            amps[i + 1] = plane_lookup<T, T4, false>(g + shell_base(lqi), lqi, ut, up);
         }
     }
     four_point<T>(amps, d1, d2);
@@ -148,6 +157,7 @@ __device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ g
                                                  double step, const double qv[3],
                                                  const double *__restrict__ qs, int m,
                                                  cplx<T> acc[M]) {
+    const double inv_step = 1.0 / step;
     for (int r = 0; r < nrot; r++) {
         const RotDev rd = rots[r];
         double rv[3];
@@ -164,7 +174,7 @@ __device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ g
 #pragma unroll
         for (int j = 0; j < M; j++) {
             if (j < m) {
-                const T t = (T)((qs[j] - layer * step) / step);
+                const T t = (T)((qs[j] - layer * step) * inv_step);
                 cplx<T> a = hermite<T>(amps[1], amps[2], d1, d2, t);
                 if (trivial) {
                     acc[j].x += a.x * sc;
@@ -510,19 +520,20 @@ __device__ cplx<T> get_sphr(const T4 *__restrict__ g, int N, double step, double
     const int fInd = (int)frc;
     const double frac = frc - (double)fInd;
     cplx<T> p[4];
-    p[1] = fInd > 0 ? plane_lookup<T, T4, true>(g + shell_base(fInd), fInd, th, ph) : cplx<T>{o.x, o.y};
+    const double ut = th * kInvPi, up = ph * kInv2Pi;
+    p[1] = fInd > 0 ? plane_lookup<T, T4, true>(g + shell_base(fInd), fInd, ut, up) : cplx<T>{o.x, o.y};
     if (frac < 1.e-9) return p[1];
-    p[2] = plane_lookup<T, T4, true>(g + shell_base(fInd + 1), fInd + 1, th, ph);
+    p[2] = plane_lookup<T, T4, true>(g + shell_base(fInd + 1), fInd + 1, ut, up);
     if (fInd == 0) {
         double nth, nph;
         antipode(th, ph, nth, nph);
-        p[0] = plane_lookup<T, T4, true>(g + 1, 1, nth, nph);
+        p[0] = plane_lookup<T, T4, true>(g + 1, 1, nth * kInvPi, nph * kInv2Pi);
     } else if (fInd == 1) {
         p[0] = {o.x, o.y};
     } else {
-        p[0] = plane_lookup<T, T4, true>(g + shell_base(fInd - 1), fInd - 1, th, ph);
+        p[0] = plane_lookup<T, T4, true>(g + shell_base(fInd - 1), fInd - 1, ut, up);
     }
-    p[3] = (fInd + 2 < N) ? plane_lookup<T, T4, true>(g + shell_base(fInd + 2), fInd + 2, th, ph) : p[2];
+    p[3] = (fInd + 2 < N) ? plane_lookup<T, T4, true>(g + shell_base(fInd + 2), fInd + 2, ut, up) : p[2];
     cplx<T> d1, d2;
     four_point<T>(p, d1, d2);
     return hermite<T>(p[1], p[2], d1, d2, (T)frac);
