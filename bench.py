@@ -305,7 +305,7 @@ def main():
         st.build(g32)
         nbytes = 256 * 2 ** 20
         ms = D.copy_probe(nbytes)
-        print(json.dumps({"probe": "build_atoms_ring + copy_probe_kernel", "copy_bytes": nbytes,
+        print(json.dumps({"probe": "build_atoms_pairs + copy_probe_kernel", "copy_bytes": nbytes,
                           "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
         return
     g64 = D.Grid(W["gs"], W["qmax"], D.PRECISION_FP64)
@@ -349,13 +349,15 @@ def main():
     nhalf = 3 * (N + 1) ** 2 * N
     sources = len(W["xyz"]) + len(W["dims"])
     terms = float(nhalf) * sources
-    flop_eq = 8.0 + 2.0 * w_t
+    # executed per atom by one pair unit (three phi / phi + pi node pairs, six terms):
+    # q_z z (1 flop) + its sin/cos, and per pair q_x x + q_y y (3 flops), sin/cos, four FMAs (8 flops)
+    flop_eq = (1.0 + 3 * 11.0) / 6.0 + (8.0 / 6.0) * w_t
     t_launch = ms_build / max(nb, 1) * 1e-3
     achieved = terms * flop_eq / t_launch / 1e12
     # HBM bytes per builder launch from the two rocprofv3 --pmc passes of this workload
     # (FETCH_SIZE, WRITE_SIZE, each scaled by a known-size copy), kept in profiles/
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", "build_atoms_ring_c2_pmc.json")
+    pmc = os.path.join(ROOT, "profiles", "build_atoms_pairs_c2_pmc.json")
     if os.path.exists(pmc):
         traffic = json.load(open(pmc)).get("traffic_bytes")
 
@@ -396,7 +398,7 @@ def main():
         "max_rel_err_vs_fp64": err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "build_atoms_ring", "launch_ms": t_launch * 1e3,
+                     "kernel": "build_atoms_pairs", "launch_ms": t_launch * 1e3,
                      "terms_per_launch": terms, "flop_eq_per_term": flop_eq,
                      "transcendental_weight": w_t},
         "cpu_baseline": cpu,

@@ -176,28 +176,34 @@ __global__ __launch_bounds__(256) void build_atoms_mixed(
     }
 }
 
-// Mixed-precision builder on row-aligned node sextets.  Every shell row of the
-// canonical half holds 6i nodes and starts at a multiple of 6, so the six
-// consecutive canonical nodes a thread owns share the row's polar angle: q_z
-// (and |q|) is common, and q_z z is formed once per atom for all six.  With
-// kReduce false the phase (in revolutions) goes to v_sin_f32 / v_cos_f32
+// Paired builder (mixed).  The nodes (j, k) and (j, k + 3i) of a row lie at
+// azimuths phi and phi + pi: q_z is shared and (q_x, q_y) changes sign, so with
+// z = q_z z_a and u = q_x x_a + q_y y_a their phases are z + u and z - u, and
+//   e^{i(z+u)} = (cz cu - sz su) + i (sz cu + cz su)
+//   e^{i(z-u)} = (cz cu + sz su) + i (sz cu - cz su).
+// One unit owns three such pairs; per atom it takes sin/cos of z once and of u
+// once per pair, and sums the four products cz cu, sz su, sz cu, cz su: eight
+// terms from eight transcendentals instead of twelve.  Box weights are even in
+// (q_x, q_y), so one per pair.  Rows off the equator also write both
+// antipodes; on the equator (j = 3i/2) the partner is the antipode itself.
+// With kReduce false the phases (in revolutions) go to v_sin_f32 / v_cos_f32
 // unreduced; the host selects that form only when every |q.r| / 2 pi of the
-// structure is below 128 revolutions, inside the units' input domain.
-constexpr int kRing = 6;
+// structure is below 128 revolutions (|z| and |u| are both bounded by |q||r|),
+// inside the units' input domain.
+constexpr int kPair = 3;
 
 template <bool kReduce>
-__global__ __launch_bounds__(256) void build_atoms_ring(
+__global__ __launch_bounds__(256) void build_atoms_pairs(
     const float4 *__restrict__ atoms, const int4 *__restrict__ tiles, int ntiles, const double *__restrict__ ff,
-    int nion, const int4 *__restrict__ bdims, double vox, double contrast, double step, long long nhalf,
+    int nion, const int4 *__restrict__ bdims, double vox, double contrast, double step, long long nunits,
     float4 *__restrict__ grid, double scale) {
-    const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * kRing;
-    if (c0 >= nhalf) return;
-    // the sextet's row: shell i, polar index j, azimuthal indices k0 .. k0+5
-    const int shell = shell_of_half(c0);
-    const long long h = c0 - half_base(shell);
-    const long long j = h / (6LL * shell), k0 = h % (6LL * shell);
-    float qx[kRing], qy[kRing], qz;
-    double accR[kRing], accI[kRing];
+    const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    const int shell = shell_of_pair_unit(u);
+    const long long r = u - pair_units_before(shell - 1);
+    const long long j = r / shell, k0 = 3 * (r % shell), half = 3LL * shell;
+    float qx[kPair], qy[kPair], qz;
+    double aR[kPair], aI[kPair], bR[kPair], bI[kPair];
     const double inv2pi = 1.0 / k2Pi;
     {
         const double q = (double)shell * step;
@@ -205,58 +211,64 @@ __global__ __launch_bounds__(256) void build_atoms_ring(
         sincos(kPi * (double)j / (double)(3 * shell), &st, &ct);
         qz = (float)(q * ct * inv2pi);
 #pragma unroll
-        for (int p = 0; p < kRing; p++) {
+        for (int p = 0; p < kPair; p++) {
             double sp, cp;
             sincos(k2Pi * (double)(k0 + p) / (double)(6 * shell), &sp, &cp);
             qx[p] = (float)(q * st * cp * inv2pi);
             qy[p] = (float)(q * st * sp * inv2pi);
-            accR[p] = 0.0;
-            accI[p] = 0.0;
+            aR[p] = aI[p] = bR[p] = bI[p] = 0.0;
         }
     }
     for (int t = 0; t < ntiles; t++) {
         const int4 tl = tiles[t];
-        float sr[kRing], si[kRing];
+        float cc[kPair], ss[kPair], sc[kPair], cs[kPair];
 #pragma unroll
-        for (int p = 0; p < kRing; p++) sr[p] = si[p] = 0.f;
+        for (int p = 0; p < kPair; p++) cc[p] = ss[p] = sc[p] = cs[p] = 0.f;
         const int a1 = tl.x + tl.y;
 #pragma unroll 2
         for (int a = tl.x; a < a1; a++) {
             const float4 at = atoms[a];
-            const float zz = qz * at.z;
+            const float zz = kReduce ? __builtin_amdgcn_fractf(qz * at.z) : qz * at.z;
+            const float cz = __builtin_amdgcn_cosf(zz), sz = __builtin_amdgcn_sinf(zz);
 #pragma unroll
-            for (int p = 0; p < kRing; p++) {
-                const float ph = fmaf(qx[p], at.x, fmaf(qy[p], at.y, zz));
-                const float r = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
-                sr[p] += __builtin_amdgcn_cosf(r);
-                si[p] += __builtin_amdgcn_sinf(r);
+            for (int p = 0; p < kPair; p++) {
+                const float ph = fmaf(qx[p], at.x, qy[p] * at.y);
+                const float v = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
+                const float cu = __builtin_amdgcn_cosf(v), su = __builtin_amdgcn_sinf(v);
+                cc[p] = fmaf(cz, cu, cc[p]);
+                ss[p] = fmaf(sz, su, ss[p]);
+                sc[p] = fmaf(sz, cu, sc[p]);
+                cs[p] = fmaf(cz, su, cs[p]);
             }
         }
-        if (tl.w == 0) {
-            const double f = ff[(size_t)shell * nion + tl.z];
 #pragma unroll
-            for (int p = 0; p < kRing; p++) {
-                accR[p] = fma(f, (double)sr[p], accR[p]);
-                accI[p] = fma(f, (double)si[p], accI[p]);
+        for (int p = 0; p < kPair; p++) {
+            double f;
+            if (tl.w == 0) {
+                f = ff[(size_t)shell * nion + tl.z];
+            } else {
+                const float k2 = (float)k2Pi;
+                f = contrast * (double)box_weight32(qx[p] * k2, qy[p] * k2, qz * k2, bdims[tl.z], (float)vox);
             }
-        } else {
-            const int4 d = bdims[tl.z];
-            const float k2 = (float)k2Pi;
-#pragma unroll
-            for (int p = 0; p < kRing; p++) {
-                const double f = contrast * (double)box_weight32(qx[p] * k2, qy[p] * k2, qz * k2, d, (float)vox);
-                accR[p] = fma(f, (double)sr[p], accR[p]);
-                accI[p] = fma(f, (double)si[p], accI[p]);
-            }
+            const double c2 = (double)cc[p], s2 = (double)ss[p], sc2 = (double)sc[p], cs2 = (double)cs[p];
+            aR[p] = fma(f, c2 - s2, aR[p]);
+            aI[p] = fma(f, sc2 + cs2, aI[p]);
+            bR[p] = fma(f, c2 + s2, bR[p]);
+            bI[p] = fma(f, sc2 - cs2, bI[p]);
         }
     }
+    const bool equator = 2 * j == 3LL * shell;
 #pragma unroll
-    for (int p = 0; p < kRing; p++) {
+    for (int p = 0; p < kPair; p++) {
         const long long k = k0 + p;
-        const float re = (float)(accR[p] * scale), im = (float)(accI[p] * scale);
-        grid[index_from_indices(shell, j, k)] = make_float4(re, im, 0.f, 0.f);
-        grid[index_from_indices(shell, 3LL * shell - j, (k + 3LL * shell) % (6LL * shell))] =
-            make_float4(re, -im, 0.f, 0.f);
+        const float ar = (float)(aR[p] * scale), ai = (float)(aI[p] * scale);
+        const float br = (float)(bR[p] * scale), bi = (float)(bI[p] * scale);
+        grid[index_from_indices(shell, j, k)] = make_float4(ar, ai, 0.f, 0.f);
+        grid[index_from_indices(shell, j, k + half)] = make_float4(br, bi, 0.f, 0.f);
+        if (!equator) {
+            grid[index_from_indices(shell, 3LL * shell - j, k + half)] = make_float4(ar, -ai, 0.f, 0.f);
+            grid[index_from_indices(shell, 3LL * shell - j, k)] = make_float4(br, -bi, 0.f, 0.f);
+        }
     }
 }
 
@@ -359,13 +371,16 @@ void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms,
     const int K = kNodesPerThread;
     unsigned nb = blocks_for(nhalf, 256 * K);
     if (!fp64 && !phase64) {
-        const unsigned nr = blocks_for(nhalf, 256 * kRing);
+        // nhalf = half_base(n + 1) for a grid of n shells
+        const int nshell = shell_of_half(nhalf) - 1;
+        const long long nunits = pair_units_before(nshell);
+        const unsigned nr = blocks_for(nunits, 256);
         if (reduce)
-            build_atoms_ring<true><<<nr, 256, 0, s>>>((const float4 *)atoms, tiles, ntiles, ff, nion, bdims, vox,
-                                                     contrast, step, nhalf, (float4 *)grid, scale);
+            build_atoms_pairs<true><<<nr, 256, 0, s>>>((const float4 *)atoms, tiles, ntiles, ff, nion, bdims, vox,
+                                                      contrast, step, nunits, (float4 *)grid, scale);
         else
-            build_atoms_ring<false><<<nr, 256, 0, s>>>((const float4 *)atoms, tiles, ntiles, ff, nion, bdims, vox,
-                                                      contrast, step, nhalf, (float4 *)grid, scale);
+            build_atoms_pairs<false><<<nr, 256, 0, s>>>((const float4 *)atoms, tiles, ntiles, ff, nion, bdims, vox,
+                                                       contrast, step, nunits, (float4 *)grid, scale);
         return;
     }
     if (fp64) {

@@ -48,6 +48,22 @@ DPLUS_HD int shell_of_half(long long c) {
     return (int)i;
 }
 
+// Pair units of the builder: (shell i, row j <= 3i/2, triple t < i) owns the
+// nodes (j, 3t+p) and their phi + pi partners (j, 3t+p+3i), p = 0..2.  Shell i
+// holds i (floor(3i/2) + 1) units; pair_units_before(n) counts shells 1..n.
+DPLUS_HD long long pair_units_before(long long n) {
+    const long long m = (n + 1) / 2;
+    return n * (n + 1) / 2 + (n * (n + 1) * (2 * n + 1) / 2 - m * m) / 2;
+}
+
+DPLUS_HD int shell_of_pair_unit(long long u) {
+    long long i = (long long)cbrt(2.0 * (double)u);
+    if (i < 1) i = 1;
+    while (i > 1 && pair_units_before(i - 1) > u) i--;
+    while (pair_units_before(i) <= u) i++;
+    return (int)i;
+}
+
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11).  Counter = (c0..c3),
 // key = seed.  Two 53-bit uniforms in (0,1) per call.
