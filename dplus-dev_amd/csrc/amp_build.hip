@@ -181,27 +181,90 @@ __global__ __launch_bounds__(256) void build_atoms_mixed(
 // z = q_z z_a and u = q_x x_a + q_y y_a their phases are z + u and z - u, and
 //   e^{i(z+u)} = (cz cu - sz su) + i (sz cu + cz su)
 //   e^{i(z-u)} = (cz cu + sz su) + i (sz cu - cz su).
-// One unit owns three such pairs; per atom it takes sin/cos of z once and of u
-// once per pair, and sums the four products cz cu, sz su, sz cu, cz su: eight
-// terms from eight transcendentals instead of twelve.  Box weights are even in
-// (q_x, q_y), so one per pair.  Rows off the equator also write both
-// antipodes; on the equator (j = 3i/2) the partner is the antipode itself.
+// One unit (thread) owns three such pairs; per atom it takes sin/cos of u once
+// per pair and sums the four products cz cu, sz su, sz cu, cz su: one
+// transcendental per term.  cz, sz depend only on the row and the atom, so a
+// workgroup (whose 256 units span at most kRowsLds rows for all but the
+// smallest shells) evaluates them once per (row, atom) into LDS, chunk by
+// chunk, double-buffered with one barrier per chunk; a workgroup spanning more
+// rows takes them per unit.  Box weights are even in (q_x, q_y), so one per
+// pair.  Rows off the equator also write both antipodes; on the equator
+// (j = 3i/2) the partner is the antipode itself.
+// Every workgroup sums all atoms of its slice, so the atoms are cut into
+// slices (tile ranges, blockIdx.y) to fill the device's last round of
+// resident workgroups; slices end in FP64 partial sums added in order.
 // With kReduce false the phases (in revolutions) go to v_sin_f32 / v_cos_f32
 // unreduced; the host selects that form only when every |q.r| / 2 pi of the
 // structure is below 128 revolutions (|z| and |u| are both bounded by |q||r|),
 // inside the units' input domain.
 constexpr int kPair = 3;
+constexpr int kRowsLds = 16, kChunk = 64;
+constexpr int kMaxSlices = 8;
 
+// Tile range of each atom slice: slice y sums tiles [begin[y], begin[y + 1]).
+struct SliceTab {
+    int begin[kMaxSlices + 1];
+};
+
+// pair unit u -> its row and first azimuthal index
+struct PairUnit {
+    int shell;
+    long long j, k0;
+};
+
+__device__ __forceinline__ PairUnit pair_unit(long long u) {
+    PairUnit pu;
+    pu.shell = shell_of_pair_unit(u);
+    const long long r = u - pair_units_before(pu.shell - 1);
+    pu.j = r / pu.shell;
+    pu.k0 = 3 * (r % pu.shell);
+    return pu;
+}
+
+// the four nodes of pair p of a unit: (j, k), (j, k + 3i) and, off the
+// equator, their antipodes (3i - j, k + 3i), (3i - j, k)
+__device__ __forceinline__ void store_pair(float4 *__restrict__ grid, const PairUnit &pu, int p, double aR,
+                                           double aI, double bR, double bI, double scale) {
+    const long long k = pu.k0 + p, half = 3LL * pu.shell;
+    const float ar = (float)(aR * scale), ai = (float)(aI * scale);
+    const float br = (float)(bR * scale), bi = (float)(bI * scale);
+    grid[index_from_indices(pu.shell, pu.j, k)] = make_float4(ar, ai, 0.f, 0.f);
+    grid[index_from_indices(pu.shell, pu.j, k + half)] = make_float4(br, bi, 0.f, 0.f);
+    if (2 * pu.j != half) {
+        grid[index_from_indices(pu.shell, half - pu.j, k + half)] = make_float4(ar, -ai, 0.f, 0.f);
+        grid[index_from_indices(pu.shell, half - pu.j, k)] = make_float4(br, -bi, 0.f, 0.f);
+    }
+}
+
+// Workgroup (x, y) sums the tiles of atom slice y for units 256 x .. 256 x + 255
+// and leaves its FP64 sums, unscaled, in partial[(y * 12 + 4 p + m) * nunits + u]
+// (m = aR, aI, bR, bI); pairs_finish adds the slices in order and stores.
 template <bool kReduce>
 __global__ __launch_bounds__(256) void build_atoms_pairs(
-    const float4 *__restrict__ atoms, const int4 *__restrict__ tiles, int ntiles, const double *__restrict__ ff,
-    int nion, const int4 *__restrict__ bdims, double vox, double contrast, double step, long long nunits,
-    float4 *__restrict__ grid, double scale) {
-    const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= nunits) return;
-    const int shell = shell_of_pair_unit(u);
-    const long long r = u - pair_units_before(shell - 1);
-    const long long j = r / shell, k0 = 3 * (r % shell), half = 3LL * shell;
+    const float4 *__restrict__ atoms, const int4 *__restrict__ tiles, SliceTab slices,
+    const double *__restrict__ ff, int nion, const int4 *__restrict__ bdims, double vox, double contrast,
+    double step, long long nunits, double *__restrict__ partial) {
+    __shared__ float2 rowz[2][kRowsLds][kChunk];
+    __shared__ float rowqz[kRowsLds];
+    const long long u0 = (long long)blockIdx.x * blockDim.x;
+    const long long u = u0 + threadIdx.x;
+    const bool live = u < nunits;
+    const long long uc = live ? u : nunits - 1;
+    const PairUnit pu = pair_unit(uc);
+    const int shell = pu.shell;
+    const long long j = pu.j, k0 = pu.k0;
+    // rows of this workgroup: global row index of the first and last unit
+    int lrow, nrows;
+    {
+        const int s0 = shell_of_pair_unit(u0);
+        const long long g0 = rows_before(s0 - 1) + (u0 - pair_units_before(s0 - 1)) / s0;
+        const long long ul = min(u0 + (long long)blockDim.x, nunits) - 1;
+        const int s1 = shell_of_pair_unit(ul);
+        const long long g1 = rows_before(s1 - 1) + (ul - pair_units_before(s1 - 1)) / s1;
+        lrow = (int)(rows_before(shell - 1) + j - g0);
+        nrows = (int)(g1 - g0 + 1);
+    }
+    const bool shared_z = nrows <= kRowsLds;  // uniform over the workgroup
     float qx[kPair], qy[kPair], qz;
     double aR[kPair], aI[kPair], bR[kPair], bI[kPair];
     const double inv2pi = 1.0 / k2Pi;
@@ -219,26 +282,65 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
             aR[p] = aI[p] = bR[p] = bI[p] = 0.0;
         }
     }
-    for (int t = 0; t < ntiles; t++) {
+    if (shared_z) {
+        // any unit of a row may publish the row's q_z: all write the same value
+        rowqz[lrow] = qz;
+        __syncthreads();
+    }
+    int buf = 0;
+    const int t_end = slices.begin[blockIdx.y + 1];
+    for (int t = slices.begin[blockIdx.y]; t < t_end; t++) {
         const int4 tl = tiles[t];
         float cc[kPair], ss[kPair], sc[kPair], cs[kPair];
 #pragma unroll
         for (int p = 0; p < kPair; p++) cc[p] = ss[p] = sc[p] = cs[p] = 0.f;
         const int a1 = tl.x + tl.y;
+        if (shared_z) {
+            for (int c0 = tl.x; c0 < a1; c0 += kChunk) {
+                const int n = min(kChunk, a1 - c0);
+                // cos / sin of q_z z for every (row, atom) of the chunk, once per workgroup
+                for (int e = threadIdx.x; e < nrows * kChunk; e += blockDim.x) {
+                    const int rr = e / kChunk, c = e % kChunk;
+                    if (c < n) {
+                        const float zz = rowqz[rr] * atoms[c0 + c].z;
+                        const float v = kReduce ? __builtin_amdgcn_fractf(zz) : zz;
+                        rowz[buf][rr][c] = make_float2(__builtin_amdgcn_cosf(v), __builtin_amdgcn_sinf(v));
+                    }
+                }
+                __syncthreads();
 #pragma unroll 2
-        for (int a = tl.x; a < a1; a++) {
-            const float4 at = atoms[a];
-            const float zz = kReduce ? __builtin_amdgcn_fractf(qz * at.z) : qz * at.z;
-            const float cz = __builtin_amdgcn_cosf(zz), sz = __builtin_amdgcn_sinf(zz);
+                for (int c = 0; c < n; c++) {
+                    const float4 at = atoms[c0 + c];
+                    const float2 z = rowz[buf][lrow][c];
 #pragma unroll
-            for (int p = 0; p < kPair; p++) {
-                const float ph = fmaf(qx[p], at.x, qy[p] * at.y);
-                const float v = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
-                const float cu = __builtin_amdgcn_cosf(v), su = __builtin_amdgcn_sinf(v);
-                cc[p] = fmaf(cz, cu, cc[p]);
-                ss[p] = fmaf(sz, su, ss[p]);
-                sc[p] = fmaf(sz, cu, sc[p]);
-                cs[p] = fmaf(cz, su, cs[p]);
+                    for (int p = 0; p < kPair; p++) {
+                        const float ph = fmaf(qx[p], at.x, qy[p] * at.y);
+                        const float v = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
+                        const float cu = __builtin_amdgcn_cosf(v), su = __builtin_amdgcn_sinf(v);
+                        cc[p] = fmaf(z.x, cu, cc[p]);
+                        ss[p] = fmaf(z.y, su, ss[p]);
+                        sc[p] = fmaf(z.y, cu, sc[p]);
+                        cs[p] = fmaf(z.x, su, cs[p]);
+                    }
+                }
+                buf ^= 1;
+            }
+        } else {
+#pragma unroll 2
+            for (int a = tl.x; a < a1; a++) {
+                const float4 at = atoms[a];
+                const float zz = kReduce ? __builtin_amdgcn_fractf(qz * at.z) : qz * at.z;
+                const float cz = __builtin_amdgcn_cosf(zz), sz = __builtin_amdgcn_sinf(zz);
+#pragma unroll
+                for (int p = 0; p < kPair; p++) {
+                    const float ph = fmaf(qx[p], at.x, qy[p] * at.y);
+                    const float v = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
+                    const float cu = __builtin_amdgcn_cosf(v), su = __builtin_amdgcn_sinf(v);
+                    cc[p] = fmaf(cz, cu, cc[p]);
+                    ss[p] = fmaf(sz, su, ss[p]);
+                    sc[p] = fmaf(sz, cu, sc[p]);
+                    cs[p] = fmaf(cz, su, cs[p]);
+                }
             }
         }
 #pragma unroll
@@ -257,18 +359,29 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
             bI[p] = fma(f, sc2 - cs2, bI[p]);
         }
     }
-    const bool equator = 2 * j == 3LL * shell;
+    if (!live) return;
+    double *dst = partial + (size_t)blockIdx.y * 12 * nunits + u;
 #pragma unroll
     for (int p = 0; p < kPair; p++) {
-        const long long k = k0 + p;
-        const float ar = (float)(aR[p] * scale), ai = (float)(aI[p] * scale);
-        const float br = (float)(bR[p] * scale), bi = (float)(bI[p] * scale);
-        grid[index_from_indices(shell, j, k)] = make_float4(ar, ai, 0.f, 0.f);
-        grid[index_from_indices(shell, j, k + half)] = make_float4(br, bi, 0.f, 0.f);
-        if (!equator) {
-            grid[index_from_indices(shell, 3LL * shell - j, k + half)] = make_float4(ar, -ai, 0.f, 0.f);
-            grid[index_from_indices(shell, 3LL * shell - j, k)] = make_float4(br, -bi, 0.f, 0.f);
-        }
+        dst[(4 * p + 0) * nunits] = aR[p];
+        dst[(4 * p + 1) * nunits] = aI[p];
+        dst[(4 * p + 2) * nunits] = bR[p];
+        dst[(4 * p + 3) * nunits] = bI[p];
+    }
+}
+
+// Sum of the atom slices of every pair unit, in slice order, scaled and stored.
+__global__ __launch_bounds__(256) void pairs_finish(const double *__restrict__ partial, int nslices,
+                                                    long long nunits, float4 *__restrict__ grid, double scale) {
+    const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    const PairUnit pu = pair_unit(u);
+    for (int p = 0; p < kPair; p++) {
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int y = 0; y < nslices; y++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) v[m] += partial[((size_t)y * 12 + 4 * p + m) * nunits + u];
+        store_pair(grid, pu, p, v[0], v[1], v[2], v[3], scale);
     }
 }
 
@@ -370,19 +483,6 @@ void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms,
                         double scale, hipStream_t s) {
     const int K = kNodesPerThread;
     unsigned nb = blocks_for(nhalf, 256 * K);
-    if (!fp64 && !phase64) {
-        // nhalf = half_base(n + 1) for a grid of n shells
-        const int nshell = shell_of_half(nhalf) - 1;
-        const long long nunits = pair_units_before(nshell);
-        const unsigned nr = blocks_for(nunits, 256);
-        if (reduce)
-            build_atoms_pairs<true><<<nr, 256, 0, s>>>((const float4 *)atoms, tiles, ntiles, ff, nion, bdims, vox,
-                                                      contrast, step, nunits, (float4 *)grid, scale);
-        else
-            build_atoms_pairs<false><<<nr, 256, 0, s>>>((const float4 *)atoms, tiles, ntiles, ff, nion, bdims, vox,
-                                                       contrast, step, nunits, (float4 *)grid, scale);
-        return;
-    }
     if (fp64) {
         build_atoms_fp64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, bdims,
                                                vox, contrast, step, nhalf, (double4 *)grid, scale);
@@ -395,6 +495,54 @@ void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms,
                                                        tiles, ntiles, ff, nion, bdims, vox, contrast,
                                                        step, nhalf, (float4 *)grid, scale);
     }
+}
+
+long long pair_units(long long nhalf) {
+    // nhalf = half_base(n + 1) for a grid of n shells
+    return pair_units_before(shell_of_half(nhalf) - 1);
+}
+
+// Number of atom slices of the paired builder.  Every workgroup sums all atoms
+// of its slice, so a launch runs in whole rounds of the device's resident
+// workgroups; S slices turn W workgroups into W S shorter ones.  S minimises
+// ceil(W S / resident) / S, with 1 % per extra slice for its partial sums.
+int pair_slices(long long nunits, int ntiles) {
+    static const int resident = [] {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, build_atoms_pairs<false>, 256, 0) !=
+                hipSuccess)
+            return 0;
+        return cus * per_cu;
+    }();
+    if (resident <= 0) return 1;
+    const long long W = blocks_for(nunits, 256);
+    int best = 1;
+    double best_cost = 1e300;
+    for (int S = 1; S <= std::min(kMaxSlices, std::max(ntiles, 1)); S++) {
+        const double cost = std::ceil((double)(W * S) / resident) / S * (1.0 + 0.01 * (S - 1));
+        if (cost < best_cost - 1e-12) {
+            best_cost = cost;
+            best = S;
+        }
+    }
+    return best;
+}
+
+void launch_build_pairs(bool reduce, const float4 *atoms, const int4 *tiles, const int *slice_begin, int nslices,
+                        const double *ff, int nion, const int4 *bdims, double vox, double contrast, double step,
+                        long long nunits, float4 *grid, double scale, double *partial, hipStream_t s) {
+    SliceTab tab{};
+    for (int y = 0; y <= nslices; y++) tab.begin[y] = slice_begin[y];
+    const dim3 grid_dim(blocks_for(nunits, 256), nslices);
+    if (reduce)
+        build_atoms_pairs<true><<<grid_dim, 256, 0, s>>>(atoms, tiles, tab, ff, nion, bdims, vox, contrast, step,
+                                                        nunits, partial);
+    else
+        build_atoms_pairs<false><<<grid_dim, 256, 0, s>>>(atoms, tiles, tab, ff, nion, bdims, vox, contrast, step,
+                                                         nunits, partial);
+    pairs_finish<<<blocks_for(nunits, 256), 256, 0, s>>>(partial, nslices, nunits, grid, scale);
 }
 
 void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s) {

@@ -260,6 +260,9 @@ void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms,
                         const int4 *tiles, int ntiles, const double *ff, int nion, const int4 *bdims, double vox,
                         double contrast, double step, long long nhalf, void *grid, double scale, hipStream_t s);
 void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s);
+void launch_build_pairs(bool reduce, const float4 *atoms, const int4 *tiles, const int *slice_begin, int nslices,
+                        const double *ff, int nion, const int4 *bdims, double vox, double contrast, double step,
+                        long long nunits, float4 *grid, double scale, double *partial, hipStream_t s);
 
 // Atomic Form Factor.cpp:11-27 (cube-mean radius), 91-115 (dummy-atom
 // solvent), 160-176 (4 Gaussians + c); evaluated in double per shell radius
@@ -382,6 +385,8 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
     this->atoms_on = atoms_on;
     this->boxes_on = boxes_on;
     ntiles = (int)tiles.size();
+    tile_len.resize(tiles.size());
+    for (size_t t = 0; t < tiles.size(); t++) tile_len[t] = tiles[t].y;
     nsources = (long long)nsrc;
     rmax = 0.0;
     for (size_t i = 0; i < nsrc; i++)
@@ -409,9 +414,28 @@ void Structure::build(Grid &g, bool host_copy, hipStream_t s) {
         // unreduced hardware sin/cos while every phase stays below 128 revolutions
         const double qtop = (double)(g.N + 1) * g.step;
         const bool reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
-        launch_build_atoms(g.fp64(), phase64, reduce, b32.p, b64.p, (const int4 *)bt.p, ntiles, (const double *)bff.p,
-                           desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast, g.step, nhalf, g.dev.p,
-                           desc.scale, s);
+        if (!g.fp64() && !phase64) {
+            // atom slices of roughly equal work (a tile costs its terms + 8 for its fold)
+            const long long nunits = pair_units(nhalf);
+            const int S = pair_slices(nunits, ntiles);
+            std::vector<int> begin(S + 1, ntiles);
+            begin[0] = 0;
+            double total = 0.0;
+            for (int t = 0; t < ntiles; t++) total += tile_len[t] + 8.0;
+            double acc = 0.0;
+            for (int t = 0, y = 1; t < ntiles && y < S; t++) {
+                acc += tile_len[t] + 8.0;
+                while (y < S && acc >= total * y / S) begin[y++] = t + 1;
+            }
+            bpart.ensure((size_t)S * 12 * nunits * sizeof(double));
+            launch_build_pairs(reduce, (const float4 *)b32.p, (const int4 *)bt.p, begin.data(), S,
+                               (const double *)bff.p, desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast,
+                               g.step, nunits, (float4 *)g.dev.p, desc.scale, (double *)bpart.p, s);
+        } else {
+            launch_build_atoms(g.fp64(), phase64, reduce, b32.p, b64.p, (const int4 *)bt.p, ntiles,
+                               (const double *)bff.p, desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast,
+                               g.step, nhalf, g.dev.p, desc.scale, s);
+        }
         check_hip(hipGetLastError(), "build_atoms launch");
     }
     launch_set_origin(g.fp64(), g.dev.p, origin * desc.scale, s);

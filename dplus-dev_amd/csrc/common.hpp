@@ -56,6 +56,9 @@ DPLUS_HD long long pair_units_before(long long n) {
     return n * (n + 1) / 2 + (n * (n + 1) * (2 * n + 1) / 2 - m * m) / 2;
 }
 
+// rows of shells 1..n: sum of floor(3s/2) + 1
+DPLUS_HD long long rows_before(long long n) { return n + (3 * n * (n + 1) / 2 - (n + 1) / 2) / 2; }
+
 DPLUS_HD int shell_of_pair_unit(long long u) {
     long long i = (long long)cbrt(2.0 * (double)u);
     if (i < 1) i = 1;

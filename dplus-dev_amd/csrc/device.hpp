@@ -164,13 +164,18 @@ struct Structure {
     double rmax = 0.0;  // largest |position| of an atom or box centre (nm)
     bool atoms_on = false, boxes_on = false;
     int ntiles = 0;
+    std::vector<int> tile_len;  // terms of each tile
     long long nsources = 0;
-    DevBuf b32, b64, bt, bd, bff;
+    DevBuf b32, b64, bt, bd, bff, bpart;
     Structure(const dplus_pdb_desc *d, hipStream_t s);
     void build(Grid &g, bool host_copy, hipStream_t s);
 };
 
 std::vector<GroupDev> make_groups(const double *q, int nq, double step);
+// paired builder: work units of a grid with nhalf canonical nodes, and the
+// number of atom slices a launch over nunits units is split into
+long long pair_units(long long nhalf);
+int pair_slices(long long nunits, int ntiles);
 void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s);
 // optional goodness of fit of an average: chi2 = sum ((scale I + constant - y) / sigma)^2
 // over q with sigma > 0; computed on the device from the device-resident

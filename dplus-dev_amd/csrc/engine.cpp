@@ -90,6 +90,12 @@ size_t request_bytes(const Request &r) {
         const long long P = total_points(gs);
         fp64 = r.precision == DPLUS_PRECISION_FP64;
         b += (size_t)P * (fp64 ? 32 : 16);
+        // FP64 partial sums of the paired builder's atom slices
+        if (!fp64) {
+            const long long units = pair_units(half_base(gs / 2 + 4));
+            const int S = pair_slices(units, 1 << 20);
+            b += (size_t)S * 12 * units * sizeof(double);
+        }
         b += (size_t)(r.desc.natoms + r.desc.nbox) * 48;
         step = r.qmax / (double)(gs / 2);
     } else {
