@@ -12,18 +12,20 @@
 //  * Friedel symmetry: for real form factors A(-q) = conj(A(q)); every shell is
 //    antipodally closed, so only its canonical half (3i(3i+1) nodes) is summed
 //    and the conjugate is stored at the antipode.  Half the sincos work.
-//  * Atoms are sorted by ion and cut into tiles of <= kTile atoms of one ion.
-//    The atom loop index is wave-uniform, so atom coordinates arrive through
-//    scalar loads (s_load) and feed v_fma as SGPR operands: no LDS traffic and
-//    no barriers in the inner loop.
-//  * Mixed precision: q is pre-scaled to revolutions, the phase is an FP32 FMA
-//    chain (or an FP64 one, kPhase64), reduced with v_fract and fed to the
-//    hardware v_sin_f32 / v_cos_f32.  Each tile is summed in FP32 and folded
-//    into FP64 accumulators with the ion's form factor at the node's shell:
-//    a two-level compensated sum whose FP32 partials never exceed kTile terms.
-//  * FP64 mode: the same structure with double coordinates and double sincos.
-//  * Each thread owns K canonical nodes, so every scalar atom load feeds K
-//    phase evaluations.
+//  * Atoms are sorted by ion and cut into tiles of <= kTile atoms of one ion;
+//    solvent boxes into tiles of one shape.  The atom loop index is
+//    wave-uniform, so coordinates arrive through scalar loads (s_load) and feed
+//    v_fma as SGPR operands.
+//  * Mixed precision (build_atoms_pairs): q is pre-scaled to revolutions and the
+//    phases are FP32 FMA chains fed to the hardware v_sin_f32 / v_cos_f32.  The
+//    nodes phi and phi + pi of a row are evaluated together from sin/cos of
+//    q_z z (shared by the row, through LDS) and of q_x x + q_y y, one
+//    transcendental per term.  Each tile is summed in FP32 and folded into FP64
+//    accumulators with the ion's form factor at the node's shell (or the box
+//    shape's weight): a two-level compensated sum whose FP32 partials never
+//    exceed kTile terms.
+//  * FP64 mode (build_atoms_fp64): the plain form -- double coordinates,
+//    phases and sincos, one node at a time, K nodes per thread.
 
 #include "common.hpp"
 
@@ -90,13 +92,27 @@ __device__ __forceinline__ float box_weight32(float qx, float qy, float qz, int4
     return w;
 }
 
+// L sinc(q L / 2) for a box edge of n voxels, L = n vox (q in nm^-1): sin on the
+// hardware unit with the argument in revolutions reduced to [-1/2, 1/2], the
+// quotient by the hardware reciprocal; below |x| = 1e-3 the series 1 - x^2/6
+__device__ __forceinline__ float edge_weight(float q, int n, float vox) {
+    const float L = (float)n * vox, x = 0.5f * q * L;
+    float r = x * (float)(1.0 / k2Pi);
+    r -= rintf(r);
+    const float s = __builtin_amdgcn_sinf(r);
+    return (fabsf(x) < 1e-3f || n == 0) ? L * (1.0f - x * x * (1.0f / 6.0f)) : s * __builtin_amdgcn_rcpf(x) * L;
+}
+
 // Tiles: (first, count, index, kind).  kind 0: atoms of ion `index`, weighted by
 // the ion's form factor at the node's shell; kind 1: solvent boxes of shape
 // bdims[index], weighted by contrast * box_weight at the node.
-template <int K, bool kPhase64>
-__global__ __launch_bounds__(256) void build_atoms_mixed(
-    const float4 *__restrict__ atoms, const double4 *__restrict__ atoms64,
-    const int4 *__restrict__ tiles, int ntiles, const double *__restrict__ ff, int nion,
+//
+// build_atoms_phase64: FP32 grid and FP32 tile sums, but every phase q.r formed
+// and reduced in FP64 -- a diagnostic form of the mixed builder (environment
+// variable DPLUS_PHASE64) that isolates the FP32 phase error.
+template <int K>
+__global__ __launch_bounds__(256) void build_atoms_phase64(
+    const double4 *__restrict__ atoms64, const int4 *__restrict__ tiles, int ntiles, const double *__restrict__ ff, int nion,
     const int4 *__restrict__ bdims, double vox, double contrast,
     double step, long long nhalf, float4 *__restrict__ grid, double scale) {
     const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * K;
@@ -128,24 +144,13 @@ __global__ __launch_bounds__(256) void build_atoms_mixed(
         const int a1 = tl.x + tl.y;
 #pragma unroll 2
         for (int a = tl.x; a < a1; a++) {
-            if constexpr (kPhase64) {
-                const double4 at = atoms64[a];
+            const double4 at = atoms64[a];
 #pragma unroll
-                for (int p = 0; p < K; p++) {
-                    double ph = fma(dqx[p], at.x, fma(dqy[p], at.y, dqz[p] * at.z));
-                    float r = (float)(ph - floor(ph));
-                    sr[p] += __builtin_amdgcn_cosf(r);
-                    si[p] += __builtin_amdgcn_sinf(r);
-                }
-            } else {
-                const float4 at = atoms[a];
-#pragma unroll
-                for (int p = 0; p < K; p++) {
-                    float ph = fmaf(qx[p], at.x, fmaf(qy[p], at.y, qz[p] * at.z));
-                    float r = __builtin_amdgcn_fractf(ph);
-                    sr[p] += __builtin_amdgcn_cosf(r);
-                    si[p] += __builtin_amdgcn_sinf(r);
-                }
+            for (int p = 0; p < K; p++) {
+                double ph = fma(dqx[p], at.x, fma(dqy[p], at.y, dqz[p] * at.z));
+                float r = (float)(ph - floor(ph));
+                sr[p] += __builtin_amdgcn_cosf(r);
+                si[p] += __builtin_amdgcn_sinf(r);
             }
         }
         if (tl.w == 0) {
@@ -291,6 +296,8 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
     const int t_end = slices.begin[blockIdx.y + 1];
     for (int t = slices.begin[blockIdx.y]; t < t_end; t++) {
         const int4 tl = tiles[t];
+        // the ion's form factor at this shell, loaded now and used after the atom loop
+        const double fion = tl.w == 0 ? ff[(size_t)shell * nion + tl.z] : 0.0;
         float cc[kPair], ss[kPair], sc[kPair], cs[kPair];
 #pragma unroll
         for (int p = 0; p < kPair; p++) cc[p] = ss[p] = sc[p] = cs[p] = 0.f;
@@ -343,20 +350,23 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
                 }
             }
         }
+        // box tiles: contrast * prod_a L_a sinc(q_a L_a / 2); the z edge is common to the row
+        const float k2 = (float)k2Pi;
+        int4 d = make_int4(0, 0, 0, 0);
+        float wz = 0.f;
+        if (tl.w != 0) {
+            d = bdims[tl.z];
+            wz = edge_weight(qz * k2, d.z, (float)vox);
+        }
 #pragma unroll
         for (int p = 0; p < kPair; p++) {
-            double f;
-            if (tl.w == 0) {
-                f = ff[(size_t)shell * nion + tl.z];
-            } else {
-                const float k2 = (float)k2Pi;
-                f = contrast * (double)box_weight32(qx[p] * k2, qy[p] * k2, qz * k2, bdims[tl.z], (float)vox);
-            }
-            const double c2 = (double)cc[p], s2 = (double)ss[p], sc2 = (double)sc[p], cs2 = (double)cs[p];
-            aR[p] = fma(f, c2 - s2, aR[p]);
-            aI[p] = fma(f, sc2 + cs2, aI[p]);
-            bR[p] = fma(f, c2 + s2, bR[p]);
-            bI[p] = fma(f, sc2 - cs2, bI[p]);
+            const double f = tl.w == 0 ? fion
+                                       : contrast * (double)(wz * edge_weight(qx[p] * k2, d.x, (float)vox) *
+                                                             edge_weight(qy[p] * k2, d.y, (float)vox));
+            aR[p] = fma(f, (double)(cc[p] - ss[p]), aR[p]);
+            aI[p] = fma(f, (double)(sc[p] + cs[p]), aI[p]);
+            bR[p] = fma(f, (double)(cc[p] + ss[p]), bR[p]);
+            bI[p] = fma(f, (double)(sc[p] - cs[p]), bI[p]);
         }
     }
     if (!live) return;
@@ -486,14 +496,12 @@ void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms,
     if (fp64) {
         build_atoms_fp64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, bdims,
                                                vox, contrast, step, nhalf, (double4 *)grid, scale);
-    } else if (phase64) {
-        build_atoms_mixed<K, true><<<nb, 256, 0, s>>>((const float4 *)atoms, (const double4 *)atoms64,
-                                                      tiles, ntiles, ff, nion, bdims, vox, contrast,
-                                                      step, nhalf, (float4 *)grid, scale);
     } else {
-        build_atoms_mixed<K, false><<<nb, 256, 0, s>>>((const float4 *)atoms, (const double4 *)atoms64,
-                                                       tiles, ntiles, ff, nion, bdims, vox, contrast,
-                                                       step, nhalf, (float4 *)grid, scale);
+        // mixed grids reach this launcher only for FP64 phases (the paired builder
+        // takes every other mixed build)
+        (void)phase64;
+        build_atoms_phase64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, bdims,
+                                                  vox, contrast, step, nhalf, (float4 *)grid, scale);
     }
 }
 
