@@ -456,7 +456,7 @@ void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s) 
 // ------------------------------------------------------------------ orientation averages
 void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
                const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
-               double step, uint64_t seed, long long dir0, int nblk, double *partial, double *out,
+               double step, uint64_t seed, long long dir0, int nblk, int chunk, double *partial, double *out,
                hipStream_t s);
 void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
                        const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
@@ -679,7 +679,8 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                 KTimer kt("mc", s);
                 launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
                           (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
-                          (const double *)bq.p, D.step, p->seed, 0, nblk, (double *)bpart.p, (double *)bout.p, s);
+                          (const double *)bq.p, D.step, p->seed, 0, nblk, chunk, (double *)bpart.p, (double *)bout.p,
+                          s);
             }
             check_hip(hipGetLastError(), "mc launch");
             DevBuf bI;
@@ -705,7 +706,7 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                     launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
                               (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
                               (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k * chunk, nblk,
-                              (double *)bpart.p, (double *)bout.p, s);
+                              chunk, (double *)bpart.p, (double *)bout.p, s);
                 }
                 check_hip(hipGetLastError(), "mc launch");
                 check_hip(hipMemcpyAsync(r.data(), bout.p, active.size() * mstride * sizeof(double),

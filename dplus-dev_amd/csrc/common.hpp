@@ -99,6 +99,27 @@ DPLUS_HD void uniform_pair(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
     v = ((double)b + 0.5) * (1.0 / 9007199254740992.0);
 }
 
+// Stratified MC directions.  The directions of a chunk of C (a multiple of 256)
+// are cut into runs of 64, one wave each; run w of the chunk's W = C / 64 fills
+// patch (w / nv, w % nv) of an nu x nv grid of equal-area patches in
+// (u, v) = ((1 - cos theta) / 2, phi / 2 pi), nu the largest divisor of W with
+// 2 nu^2 <= W.  Every chunk covers the sphere once with equal weight per patch,
+// so the chunk mean stays an unbiased estimate of the orientation average, and
+// the 64 directions of a wave land on neighbouring grid cells.
+DPLUS_HD void mc_strata(int chunk, int &nu, int &nv) {
+    const int W = chunk / 64;
+    nu = 1;
+    for (int a = 1; 2 * a * a <= W; a++)
+        if (W % a == 0) nu = a;
+    nv = W / nu;
+}
+
+DPLUS_HD void mc_stratify(long long d, int chunk, int nu, int nv, double &u, double &v) {
+    const long long w = (d % chunk) / 64;
+    u = ((double)(w / nv) + u) / (double)nu;
+    v = ((double)(w % nv) + v) / (double)nv;
+}
+
 constexpr uint32_t kStreamMC = 0x4D43u;
 constexpr uint32_t kStreamVegas = 0x5645u;
 

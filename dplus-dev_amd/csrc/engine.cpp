@@ -35,7 +35,7 @@ namespace dplus {
 
 void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
                const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
-               double step, uint64_t seed, long long dir0, int nblk, double *partial, double *out,
+               double step, uint64_t seed, long long dir0, int nblk, int chunk, double *partial, double *out,
                hipStream_t s);
 void launch_mc_finish(const GroupDev *groups, int ngroups, int mstride, const double *out, double total, double *I,
                       hipStream_t s);
@@ -268,6 +268,7 @@ struct Session {
     double step = 0.0;
     int nq = 0, ng = 0, mstride = 1, nblk = 0;
     long long total = 0;
+    int chunk = 0;  // MC stratification period (directions)
     unsigned long long seed = 0;
     std::vector<GroupDev> groups;
     DevBuf bgrids, bq, bg, bact, bpart, bout, bI, by, bsig, bchi, bparam, brot, btr;
@@ -324,7 +325,7 @@ struct Session {
         check_hip(hipMemcpyAsync(bparam.p, h_param, 2 * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
         launch_mc(fp64, (const void *const *)bgrids.p, (const RotDev *)brot.p, nrot, (const double4 *)btr.p,
                   (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride, (const double *)bq.p, step, seed, 0, nblk,
-                  (double *)bpart.p, (double *)bout.p, s);
+                  chunk, (double *)bpart.p, (double *)bout.p, s);
         launch_mc_finish((const GroupDev *)bg.p, ng, mstride, (const double *)bout.p, (double)total, (double *)bI.p, s);
         launch_chi2_param((const double *)bI.p, (const double *)by.p, (const double *)bsig.p, nq,
                           (const double *)bparam.p, (double *)bchi.p, s);
@@ -531,6 +532,12 @@ int dplus_session_create(int nleaf, dplus_grid *const *leaves, int nq, const dou
         for (auto &g : S->groups) S->mstride = std::max(S->mstride, g.m);
         S->total = ((std::max<long long>(256, directions) + 255) / 256) * 256;
         S->nblk = (int)(S->total / 256);
+        // the orientation average's default chunk of 8192 directions when it
+        // divides the count (so a session equals that eager average), else one
+        // chunk of all directions
+        if (S->total % 8192 != 0 && S->total > (1LL << 30))
+            throw Error(kErrInvalidArgs, "a session's direction count above 2^30 must be a multiple of 8192");
+        S->chunk = (int)(S->total % 8192 == 0 ? 8192 : S->total);
         std::vector<int> act(S->ng);
         for (int i = 0; i < S->ng; i++) act[i] = i;
         hipStream_t s = S->s;

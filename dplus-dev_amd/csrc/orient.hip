@@ -9,7 +9,9 @@
 // MC: directions (theta = acos(2u-1), phi = 2 pi v) from a counter-based Philox
 //   stream keyed by (seed; direction, shell group), so any launch shape and any
 //   batch composition reproduce the same numbers (the reference draws them from
-//   a clock-seeded MTGP32, GPUHybridCalc.cu:196-198).
+//   a clock-seeded MTGP32, GPUHybridCalc.cu:196-198), stratified per chunk
+//   (mc_stratify: a wave's 64 directions share one equal-area patch, so its
+//   gathers hit neighbouring grid cells).
 // VEGAS: 32x32 (cos theta, phi) bins, bin-volume weighted estimate, per-bin
 //   variance marginals and the damped re-binning of HybridOA.cu:181-449.
 //
@@ -146,7 +148,7 @@ __device__ __forceinline__ void phase_sincos<double>(double qd, double &s, doubl
     sincos(qd, &s, &c);
 }
 
-constexpr int kMaxM = 8;  // q-values of one shell group handled per pass
+constexpr int kMaxM = 16;  // q-values of one shell group handled per pass (one lookup serves all)
 
 
 // accumulate the docked amplitude along unit direction qv for q-values qs[0..m), m <= M
@@ -213,13 +215,14 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T
                                                  const GroupDev *__restrict__ groups,
                                                  const int *__restrict__ active, int mstride,
                                                  const double *__restrict__ qs, double step,
-                                                 uint64_t seed, long long dir0, int nblk,
-                                                 double *__restrict__ partial) {
+                                                 uint64_t seed, long long dir0, int nblk, int chunk, int nu,
+                                                 int nv, double *__restrict__ partial) {
     const int gi = blockIdx.y;
     const GroupDev G = groups[active[gi]];
     const long long d = dir0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
     double u, v;
     uniform_pair(seed, (uint32_t)d, (uint32_t)(d >> 32), (uint32_t)G.layer, kStreamMC, u, v);
+    mc_stratify(d, chunk, nu, nv, u, v);
     const double th = acos(2.0 * u - 1.0), ph = k2Pi * v;
     double st, ct, sp, cp;
     sincos(th, &st, &ct);
@@ -614,17 +617,19 @@ __global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const Lea
 // ------------------------------------------------------------------ launchers
 void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot,
                const double4 *trans, const GroupDev *groups, const int *active, int nactive,
-               int mstride, const double *qs, double step, uint64_t seed, long long dir0, int nblk,
+               int mstride, const double *qs, double step, uint64_t seed, long long dir0, int nblk, int chunk,
                double *partial, double *out, hipStream_t s) {
     dim3 grid(nblk, nactive);
+    int nu, nv;
+    mc_strata(chunk, nu, nv);
     if (fp64)
         mc_kernel<double, double4><<<grid, 256, 0, s>>>((const double4 *const *)grids, rots, nrot, trans,
                                                          groups, active, mstride, qs, step, seed, dir0,
-                                                         nblk, partial);
+                                                         nblk, chunk, nu, nv, partial);
     else
         mc_kernel<float, float4><<<grid, 256, 0, s>>>((const float4 *const *)grids, rots, nrot, trans,
                                                        groups, active, mstride, qs, step, seed, dir0,
-                                                       nblk, partial);
+                                                       nblk, chunk, nu, nv, partial);
     mc_reduce<<<nactive, 64, 0, s>>>(groups, active, partial, nblk, mstride, out);
 }
 

@@ -143,7 +143,7 @@ typedef struct {
     int method;             /* DPLUS_OA_* */
     unsigned long long seed;
     long long max_dirs;     /* MC: directions per shell group ("OrientationIterations") */
-    int chunk;              /* MC: directions between convergence checks (multiple of 256) */
+    int chunk;              /* MC: directions between convergence checks and the stratification period (multiple of 256) */
     double eps;             /* "Convergence"; <= 0 runs the full count */
     int vegas_iterations;
     int vegas_points_per_bin; /* 32x32 bins; points per iteration = 1024 * this */
@@ -306,7 +306,9 @@ int dplus_engine_stats(dplus_engine *e, long long *submitted, long long *complet
  * reduced on the device.  No counterpart in the reference's API. */
 typedef struct dplus_session dplus_session;
 /* leaves: resident grids (same size, qmax, precision); fixed q, data, sigma,
- * seed and direction count (rounded up to a multiple of 256) */
+ * seed and direction count (rounded up to a multiple of 256).  The directions
+ * are stratified in chunks of 8192 when that divides the count (the same
+ * directions as dplus_orientation_average with chunk 8192), else as one chunk. */
 int dplus_session_create(int nleaf, dplus_grid *const *leaves, int nq, const double *q, const double *data,
                          const double *sigma, unsigned long long seed, long long directions,
                          dplus_session **out);

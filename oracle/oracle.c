@@ -112,6 +112,24 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
     out[3] = c3;
 }
 
+/* Stratified MC directions of a chunk of C directions (C a multiple of 256):
+ * direction d of the chunk belongs to run w = d / 64 of W = C / 64 runs; run w
+ * covers patch (w / nv, w % nv) of the nu x nv grid of equal-area patches in
+ * (u, v) = ((1 - cos theta) / 2, phi / 2 pi), with nu the largest divisor of W
+ * with 2 nu^2 <= W and nv = W / nu.  (This build's sampling design: the
+ * reference draws unstratified directions from a clock-seeded MTGP32 stream,
+ * GPUHybridCalc.cu:196-198, so only the estimator, not the stream, is shared.) */
+void orc_mc_stratify(long long d, int chunk, double *u, double *v) {
+    int W = chunk / 64, nu = 1, a;
+    if (W < 1 || chunk % 64) return;
+    for (a = 1; 2 * a * a <= W; a++)
+        if (W % a == 0) nu = a;
+    int nv = W / nu;
+    long long w = (d % chunk) / 64;
+    *u = ((double)(w / nv) + *u) / (double)nu;
+    *v = ((double)(w % nv) + *v) / (double)nv;
+}
+
 /* two doubles in (0,1) with 53 random bits each */
 void orc_uniform_pair(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, double *u,
                       double *v) {
@@ -705,6 +723,7 @@ int orc_mc_oa(int nleaf, const orc_leaf *leaves, int nq, const double *qs, uint6
                     double u, v;
                     orc_uniform_pair(seed, (uint32_t)gd, (uint32_t)(gd >> 32), (uint32_t)layer,
                                      ORC_STREAM_MC, &u, &v);
+                    orc_mc_stratify(d, chunk, &u, &v);
                     double th = acos(2.0 * u - 1.0), ph = M_2PI_D * v;
                     orc_docked_group(nleaf, leaves, layer, th, ph, m, qs + q0, amp);
                     for (int j = 0; j < m; j++)

@@ -30,6 +30,7 @@ void orc_qvector_from_index(int grid_size, double qmax, long long index, double 
 
 /* ---- counter-based RNG shared with the device code ---- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_mc_stratify(long long d, int chunk, double *u, double *v);
 void orc_uniform_pair(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
                       double *u, double *v);
 

@@ -60,6 +60,7 @@ def lib():
         L.orc_index_from_indices.argtypes = [ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong]
         L.orc_step_size.argtypes = [ctypes.c_int, ctypes.c_double]
         L.orc_uniform_pair.argtypes = [ctypes.c_uint64] + [ctypes.c_uint32] * 4 + [_D, _D]
+        L.orc_mc_stratify.argtypes = [ctypes.c_longlong, ctypes.c_int, _D, _D]
         L.orc_grid_intensity_dirs.restype = ctypes.c_double
         L.orc_mc_oa.argtypes = [ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_int, _D, ctypes.c_uint64,
                                 ctypes.c_longlong, ctypes.c_int, ctypes.c_double, _D, _LL]
@@ -120,6 +121,13 @@ def philox(ctr, key):
     o = (ctypes.c_uint32 * 4)()
     lib().orc_philox4x32_10(c, k, o)
     return list(o)
+
+
+def mc_stratify(d, chunk, u, v):
+    """(u, v) of direction d of an MC chunk mapped into its run's patch."""
+    cu, cv = ctypes.c_double(u), ctypes.c_double(v)
+    lib().orc_mc_stratify(d, chunk, ctypes.byref(cu), ctypes.byref(cv))
+    return cu.value, cv.value
 
 
 def uniform_pair(seed, c0, c1, c2, c3):
