@@ -156,21 +156,33 @@ static thread_local hipStream_t t_alloc_stream = nullptr;
 hipStream_t alloc_stream() { return t_alloc_stream ? t_alloc_stream : Device::get().stream; }
 void set_alloc_stream(hipStream_t s) { t_alloc_stream = s; }
 
+std::shared_mutex &capture_mutex() {
+    static std::shared_mutex m;
+    return m;
+}
+
 DevBuf::~DevBuf() { release(); }
-void DevBuf::release() {
+static void free_locked(void *&p, size_t &n, hipStream_t s) {
     if (p) (void)hipFreeAsync(p, s);
     p = nullptr;
     n = 0;
 }
+void DevBuf::release() {
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    free_locked(p, n, s);
+}
 void DevBuf::ensure(size_t bytes) {
     if (bytes <= n) return;
-    release();
+    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    free_locked(p, n, s);
     if (bytes == 0) return;
     s = alloc_stream();
-    if (hipMallocAsync(&p, bytes, s) != hipSuccess) {
+    const hipError_t e = hipMallocAsync(&p, bytes, s);
+    if (e != hipSuccess) {
         p = nullptr;
         (void)hipGetLastError();
-        throw Error(kErrOutOfMemory, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        throw Error(kErrOutOfMemory, "device allocation of " + std::to_string(bytes) + " bytes failed (" +
+                                         hipGetErrorName(e) + ")");
     }
     n = bytes;
 }

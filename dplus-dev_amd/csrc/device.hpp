@@ -12,6 +12,8 @@
 #include <string>
 #include <vector>
 
+#include <shared_mutex>
+
 #include "../../include/dplus_mi355x.h"
 #include "common.hpp"
 
@@ -47,6 +49,11 @@ struct Device {
 // requests on other streams keeps running.
 hipStream_t alloc_stream();
 void set_alloc_stream(hipStream_t s);
+// Graph capture against allocation: a thread capturing a stream holds this
+// exclusively from begin to instantiation; stream-ordered and pinned
+// allocations and frees take it shared, so none runs during another thread's
+// capture (HIP rejects them there).
+std::shared_mutex &capture_mutex();
 
 // owning device allocation that only grows
 struct DevBuf {

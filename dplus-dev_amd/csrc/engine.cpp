@@ -286,7 +286,7 @@ struct Session {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (void *p : {(void *)h_rot, (void *)h_tr, (void *)h_param, (void *)h_out})
-            if (p) (void)hipHostFree(p);
+            if (p) host_free(p);
         bgrids.release(), bq.release(), bg.release(), bact.release(), bpart.release(), bout.release();
         bI.release(), by.release(), bsig.release(), bchi.release(), bparam.release(), brot.release(), btr.release();
         if (s) {
@@ -295,20 +295,25 @@ struct Session {
         }
     }
 
-    void host_alloc(void **p, size_t bytes) {
+    static void host_alloc(void **p, size_t bytes) {
+        std::shared_lock<std::shared_mutex> lk(capture_mutex());
         check_hip(hipHostMalloc(p, bytes, hipHostMallocDefault), "hipHostMalloc");
+    }
+    static void host_free(void *p) {
+        std::shared_lock<std::shared_mutex> lk(capture_mutex());
+        (void)hipHostFree(p);
     }
 
     void grow(size_t nrot, size_t ntr) {
         if (nrot > cap_rot) {
-            if (h_rot) (void)hipHostFree(h_rot);
+            if (h_rot) host_free(h_rot);
             cap_rot = std::max<size_t>(nrot, 2 * cap_rot);
             host_alloc((void **)&h_rot, cap_rot * sizeof(RotDev));
             brot.ensure(cap_rot * sizeof(RotDev));
             graph_nrot = -1;
         }
         if (ntr > cap_tr) {
-            if (h_tr) (void)hipHostFree(h_tr);
+            if (h_tr) host_free(h_tr);
             cap_tr = std::max<size_t>(ntr, 2 * cap_tr);
             host_alloc((void **)&h_tr, cap_tr * sizeof(double4));
             btr.ensure(cap_tr * sizeof(double4));
@@ -319,7 +324,19 @@ struct Session {
     void capture(int nrot) {
         if (exec) (void)hipGraphExecDestroy(exec), exec = nullptr;
         if (graph) (void)hipGraphDestroy(graph), graph = nullptr;
+        // no allocation anywhere in the process while this stream captures
+        std::unique_lock<std::shared_mutex> lk(capture_mutex());
         check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        // a failure inside the capture still ends it, so the stream stays usable
+        struct EndOnThrow {
+            hipStream_t s;
+            bool armed = true;
+            ~EndOnThrow() {
+                if (!armed) return;
+                hipGraph_t g = nullptr;
+                if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            }
+        } end_on_throw{s};
         check_hip(hipMemcpyAsync(brot.p, h_rot, cap_rot * sizeof(RotDev), hipMemcpyHostToDevice, s), "H2D");
         check_hip(hipMemcpyAsync(btr.p, h_tr, cap_tr * sizeof(double4), hipMemcpyHostToDevice, s), "H2D");
         check_hip(hipMemcpyAsync(bparam.p, h_param, 2 * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
@@ -332,6 +349,7 @@ struct Session {
         check_hip(hipMemcpyAsync(h_out, bI.p, nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         check_hip(hipMemcpyAsync(h_out + nq, bchi.p, sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         hipGraph_t gph = nullptr;
+        end_on_throw.armed = false;
         const hipError_t e = hipStreamEndCapture(s, &gph);
         check_hip(e, "end capture");
         graph = gph;

@@ -343,27 +343,35 @@ constexpr int kVM = 8;         // q-values per pass in the VEGAS kernels
 constexpr int kVPts = 4;       // points per lane and bin: points per bin <= 256
 
 // bounds[g*66 + 0..32] theta-axis boundaries in u = (1-cos)/2, [33..65] phi-axis in phi/2pi.
-// One wave per (group, bin): the wave evaluates the bin's ppb points, and writes
-// binsum[(gi * mstride + j) * 1024 + bin] = sum over the bin's points of I_j and
-// binvar[gi * 1024 + bin] = ppb * sum_p (vol * S_p - mean)^2 with S_p = sum_j I_j(p)
-// (HybridVEGASBinReduceToIKernel / HybridVEGASBinReduceKernel, HybridOA.cu:300-449).
-template <typename T, typename T4>
+// A bin is served by L lanes (L a power of two, ppb <= kVPts L), a wave by 64 / L
+// bins: lane l of the wave takes the points l % L + L k of bin slot l / L.  It
+// writes binsum[(gi * mstride + j) * 1024 + bin] = sum over the bin's points of
+// I_j and binvar[gi * 1024 + bin] = ppb * sum_p (vol * S_p - mean)^2 with
+// S_p = sum_j I_j(p) (HybridVEGASBinReduceToIKernel / HybridVEGASBinReduceKernel,
+// HybridOA.cu:300-449).
+__device__ __forceinline__ double lanes_sum(double v, int L) {
+    for (int o = L >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <typename T, typename T4, int kL>  // kL: lanes per bin fixed at compile time, or 0 for L
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(const T4 *const *__restrict__ grids,
                                                     const RotDev *__restrict__ rots, int nrot,
                                                     const double4 *__restrict__ trans,
                                                     const GroupDev *__restrict__ groups,
                                                     const int *__restrict__ active, int mstride,
                                                     const double *__restrict__ qs, double step,
-                                                    uint64_t seed, int iter, int ppb,
+                                                    uint64_t seed, int iter, int ppb, int Lrt,
                                                     const double *__restrict__ bounds,
                                                     double *__restrict__ binsum,
                                                     double *__restrict__ binvar) {
+    const int L = kL > 0 ? kL : Lrt;
     const int gi = blockIdx.y;
     const int g = active[gi];
     const GroupDev G = groups[g];
-    const int lane = threadIdx.x & 63;
-    const int bin = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (bin >= kVB * kVB) return;
+    const int lane = threadIdx.x & 63, pl = lane & (L - 1);
+    // every bin of the launch exists: 1024 is a multiple of the 4 (64 / L) bins of a block
+    const int bin = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / L) + lane / L;
     const int ti = bin / kVB, pj = bin % kVB;
     const double *bt = bounds + (size_t)g * 66, *bp = bt + 33;
     const double vol = (bt[ti + 1] - bt[ti]) * (bp[pj + 1] - bp[pj]);
@@ -372,7 +380,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
 #pragma unroll
     for (int k = 0; k < kVPts; k++) {
         so[k] = 0.0;
-        const int p = lane + 64 * k;
+        const int p = pl + L * k;
         double u = 0.5, v = 0.5;
         if (p < ppb) uniform_pair(seed, (uint32_t)(bin * ppb + p), (uint32_t)iter, (uint32_t)G.layer, kStreamVegas, u, v);
         const double th = acos(1. - 2. * (u * (bt[ti + 1] - bt[ti]) + bt[ti]));
@@ -391,7 +399,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
         for (int j = 0; j < kVM; j++) e[j] = 0.0;
 #pragma unroll
         for (int k = 0; k < kVPts; k++) {
-            if (lane + 64 * k >= ppb) continue;
+            if (pl + L * k >= ppb) continue;
             cplx<T> acc[kVM];
 #pragma unroll
             for (int j = 0; j < kVM; j++) acc[j] = {(T)0, (T)0};
@@ -407,22 +415,22 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
         }
 #pragma unroll
         for (int j = 0; j < kVM; j++) {
-            const double t = wave_sum(e[j]);
-            if (lane == 0 && j < m) binsum[((size_t)gi * mstride + jb + j) * (kVB * kVB) + bin] = t;
+            const double t = lanes_sum(e[j], L);
+            if (pl == 0 && j < m) binsum[((size_t)gi * mstride + jb + j) * (kVB * kVB) + bin] = t;
         }
     }
     // variance of vol * S over the bin's points, two-pass (equal to the reference's Welford sum)
     double sx = 0.0;
 #pragma unroll
     for (int k = 0; k < kVPts; k++)
-        if (lane + 64 * k < ppb) sx += vol * so[k];
-    const double mean = wave_sum(sx) / (double)ppb;
+        if (pl + L * k < ppb) sx += vol * so[k];
+    const double mean = lanes_sum(sx, L) / (double)ppb;
     double sq = 0.0;
 #pragma unroll
     for (int k = 0; k < kVPts; k++)
-        if (lane + 64 * k < ppb) sq += (vol * so[k] - mean) * (vol * so[k] - mean);
-    sq = wave_sum(sq);
-    if (lane == 0) binvar[(size_t)gi * (kVB * kVB) + bin] = sq * ppb;
+        if (pl + L * k < ppb) sq += (vol * so[k] - mean) * (vol * so[k] - mean);
+    sq = lanes_sum(sq, L);
+    if (pl == 0) binvar[(size_t)gi * (kVB * kVB) + bin] = sq * ppb;
 }
 
 // serial damped re-binning of one axis (HybridOA.cu:181-263); boundaries the
@@ -637,15 +645,22 @@ void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, 
                        const double4 *trans, const GroupDev *groups, const int *active, int nactive,
                        int mstride, const double *qs, double step, uint64_t seed, int iter, int ppb,
                        double *bounds, double *binsum, double *binvar, double *hist, int nq, hipStream_t s) {
-    dim3 grid(kVB * kVB / 4, nactive);
-    if (fp64)
-        vegas_sample<double, double4><<<grid, 256, 0, s>>>((const double4 *const *)grids, rots, nrot,
-                                                            trans, groups, active, mstride, qs, step,
-                                                            seed, iter, ppb, bounds, binsum, binvar);
+    // lanes per bin: the least power of two >= ppb, at most a wave (ppb <= kVPts * 64)
+    int L = 1;
+    while (L < ppb && L < 64) L <<= 1;
+    dim3 grid(kVB * kVB / (4 * (64 / L)), nactive);
+#define DPLUS_VEGAS(T, T4, KL)                                                                                 \
+    vegas_sample<T, T4, KL><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups, active, \
+                                                 mstride, qs, step, seed, iter, ppb, L, bounds, binsum, binvar)
+    if (fp64 && L == 64)
+        DPLUS_VEGAS(double, double4, 64);
+    else if (fp64)
+        DPLUS_VEGAS(double, double4, 0);
+    else if (L == 64)
+        DPLUS_VEGAS(float, float4, 64);
     else
-        vegas_sample<float, float4><<<grid, 256, 0, s>>>((const float4 *const *)grids, rots, nrot, trans,
-                                                          groups, active, mstride, qs, step, seed, iter,
-                                                          ppb, bounds, binsum, binvar);
+        DPLUS_VEGAS(float, float4, 0);
+#undef DPLUS_VEGAS
     vegas_reduce<<<nactive, 1024, 0, s>>>(groups, active, mstride, ppb, iter, binsum, binvar, bounds, hist, nq);
 }
 

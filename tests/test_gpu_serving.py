@@ -155,3 +155,45 @@ def test_fit_session_graph_replays(dplus):
     say(f"    session {info}")
     assert info["evaluations"] == 4 and info["graph_captures"] == 2
     sess.close()
+
+
+def test_concurrent_sessions_capture_while_others_allocate(dplus):
+    """Eight fit sessions in threads, each re-capturing its graph as the copy
+    count grows, while the main thread keeps allocating grids: every capture
+    and allocation succeeds and every session still equals its eager average."""
+    import threading
+    from dplus_mi355x.serving import FitSession
+    import dplus_mi355x.synth as S
+    say("[case] 8 sessions capturing concurrently with allocations on other threads")
+    pdb = make_pdb(300, 31, 1.0)
+    g = dplus.Grid(20, 3.0, dplus.PRECISION_MIXED)
+    dplus.Structure(pdb["xyz"], pdb["ion"], pdb["coef9"], pdb["rad"], sol_ed=333.0, flags=3).build(g)
+    q = np.linspace(0.05, 3.0, 40)
+    y, sig = np.linspace(2.0, 1.0, 40), np.full(40, 0.1)
+    results, errors = {}, []
+
+    def run(k):
+        try:
+            sess = FitSession([g], q, y, sig, seed=50 + k, directions=2048)
+            for n in (3, 7, 12, 20):
+                rot, nt, tr = S.rows_to_docking(S.helix_rows(n + k, 1.5, 0.5, 30.0 + k))
+                results[(k, n)] = (sess.evaluate([(rot, nt, tr)])[0], (rot, nt, tr))
+            sess.close()
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(8)]
+    for t in threads:
+        t.start()
+    scratch = []
+    while any(t.is_alive() for t in threads):
+        scratch.append(dplus.Grid(16, 2.0, dplus.PRECISION_MIXED))
+        if len(scratch) > 6:
+            scratch.pop(0)
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for (k, n), (I, (rot, nt, tr)) in sorted(results.items())[::5]:
+        eager, _ = dplus.orientation_average(dplus.Docking([(g, rot, nt, tr)]), q, seed=50 + k, max_dirs=2048,
+                                             chunk=2048)
+        assert np.array_equal(I, eager)
