@@ -284,7 +284,8 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--session-evals", type=int, default=4)
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="one mixed builder launch plus a 256 MiB device copy (counter calibration), for rocprofv3 --pmc")
+                    help="one mixed build, a 256 MiB device copy (counter calibration), one VEGAS curve and one "
+                         "MC average of 262,144 directions on the built grid, for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (see DESIGN.md for the "
@@ -305,8 +306,22 @@ def main():
         st.build(g32)
         nbytes = 256 * 2 ** 20
         ms = D.copy_probe(nbytes)
-        print(json.dumps({"probe": "build_atoms_pairs + copy_probe_kernel", "copy_bytes": nbytes,
-                          "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9}))
+        # orientation averages on the built grid: every (direction, shell group) reads
+        # 4 planes x 4 theta rows x 2 phi nodes of 16 B (mixed) = 512 B of nodes
+        D.timing_reset()
+        D.timing_enable(True)
+        D.orientation_average(D.Docking.single(g32), W["q"], method=D.OA_VEGAS, seed=7,
+                              vegas_iterations=W["iters"], vegas_points_per_bin=W["ppb"])
+        D.orientation_average(D.Docking.single(g32), W["q"], method=D.OA_MC, seed=7, max_dirs=1 << 18, chunk=8192)
+        D.timing_enable(False)
+        # shell groups: q in (l step, (l + 1) step] share layer l, q = 0 joins layer 0
+        ngroups = len(np.unique(np.maximum(np.ceil(W["q"] / g32.step_size) - 1, 0)))
+        print(json.dumps({"probe": "build_atoms_pairs + copy_probe_kernel + vegas_sample + mc_kernel",
+                          "copy_bytes": nbytes, "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9,
+                          "shell_groups": ngroups,
+                          "vegas_ms": D.timing_get("vegas")[1], "vegas_units": W["iters"] * 1024 * W["ppb"] * ngroups,
+                          "mc_ms": D.timing_get("mc")[1], "mc_units": (1 << 18) * ngroups,
+                          "node_bytes_per_unit": 512}))
         return
     g64 = D.Grid(W["gs"], W["qmax"], D.PRECISION_FP64)
 

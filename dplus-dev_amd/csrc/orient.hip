@@ -433,27 +433,26 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
     if (pl == 0) binvar[(size_t)gi * (kVB * kVB) + bin] = sq * ppb;
 }
 
-// serial damped re-binning of one axis (HybridOA.cu:181-263); boundaries the
-// serial sweep does not reach keep their previous value
-__device__ void vegas_resize(const double *var, double *b) {
-    double total = 0.0;
-    for (int i = 0; i < kVB; i++) total += var[i];
-    if (total == 0.) return;
-    double w[kVB], nb[kVB];
-    for (int i = 0; i < kVB; i++) {
-        double x;
-        if (i == 0)
-            x = (var[0] + var[1]) / 2.;
-        else if (i == kVB - 1)
-            x = (var[i] + var[i - 1]) / 2.;
-        else
-            x = (var[i - 1] + var[i] + var[i + 1]) / 3.;
-        x = total / x;
-        x = pow((x - 1.) / (x * log(x)), 1.5);
-        if (x != x) x = 0.;
-        w[i] = x;
-    }
-    double tw = 0.0;
+// damped re-binning weight of axis bin i from the axis's variance marginal
+// (HybridOA.cu:181-263): x = total / (local mean of var), w = ((x-1)/(x ln x))^1.5
+__device__ __forceinline__ double vegas_weight(const double *var, double total, int i) {
+    double x;
+    if (i == 0)
+        x = (var[0] + var[1]) / 2.;
+    else if (i == kVB - 1)
+        x = (var[i] + var[i - 1]) / 2.;
+    else
+        x = (var[i - 1] + var[i] + var[i + 1]) / 3.;
+    x = total / x;
+    x = pow((x - 1.) / (x * log(x)), 1.5);
+    if (x != x) x = 0.;
+    return x;
+}
+
+// serial sweep of the re-binning: new interior boundaries at equal shares of
+// the weights; boundaries the sweep does not reach keep their previous value
+__device__ void vegas_sweep(const double *w, double *b) {
+    double tw = 0.0, nb[kVB];
     for (int i = 0; i < kVB; i++) tw += w[i];
     for (int i = 0; i < kVB; i++) nb[i] = b[i + 1];
     double xold, xnew = 0., dw = 0., pts = tw / kVB;
@@ -471,8 +470,9 @@ __device__ void vegas_resize(const double *var, double *b) {
 }
 
 // one block (1024 threads = the 32x32 bins) per active group: the iteration's
-// integral sum_bins vol * mean_bin(I_j), the theta/phi marginals of the bin
-// variances, re-binning, and the running mean over iterations
+// integral sum_bins vol * mean_bin(I_j) (wave w takes q = w, w + 16, ...), the
+// theta/phi marginals of the bin variances, re-binning (weights on 64 lanes,
+// then one serial sweep per axis), and the running mean over iterations
 // (HybridVEGASResizeGridKernel; GPUHybridCalc.cu:654-701).
 // hist[iter * nq + q] holds the running mean after iteration iter.
 __global__ __launch_bounds__(1024) void vegas_reduce(const GroupDev *__restrict__ groups,
@@ -486,37 +486,47 @@ __global__ __launch_bounds__(1024) void vegas_reduce(const GroupDev *__restrict_
     const GroupDev G = groups[g];
     const int bin = threadIdx.x, ti = bin / kVB, pj = bin % kVB;
     double *bt = bounds + (size_t)g * 66, *bp = bt + 33;
-    const double vol = (bt[ti + 1] - bt[ti]) * (bp[pj + 1] - bp[pj]);
     __shared__ double sv[1024];
-    __shared__ double se[16];
+    __shared__ double svol[1024];
     __shared__ double marg[2][kVB];
+    __shared__ double wgt[2][kVB];
+    __shared__ double tot[2];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     sv[bin] = binvar[(size_t)gi * 1024 + bin];
-    for (int j = 0; j < G.m; j++) {
-        const double c = wave_sum(binsum[((size_t)gi * mstride + j) * 1024 + bin] * vol / (double)ppb);
-        if (lane == 0) se[w] = c;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double e = 0.0;
-            for (int k = 0; k < 16; k++) e += se[k];
+    svol[bin] = (bt[ti + 1] - bt[ti]) * (bp[pj + 1] - bp[pj]) / (double)ppb;
+    __syncthreads();
+    for (int j = w; j < G.m; j += 16) {
+        const double *bs = binsum + ((size_t)gi * mstride + j) * 1024;
+        double c = 0.0;
+        for (int b = lane; b < 1024; b += 64) c += bs[b] * svol[b];
+        c = wave_sum(c);
+        if (lane == 0) {
             const int q = G.q0 + j;
+            double e = c;
             if (iter > 0) e = (hist[(size_t)(iter - 1) * nq + q] * (double)iter + e) / (double)(iter + 1);
             hist[(size_t)iter * nq + q] = e;
         }
-        __syncthreads();
     }
-    if (threadIdx.x < kVB) {
-        double a = 0.0, b = 0.0;
-        for (int k = 0; k < kVB; k++) {
-            a += sv[threadIdx.x * kVB + k];  // theta bin threadIdx.x
-            b += sv[k * kVB + threadIdx.x];  // phi bin threadIdx.x
-        }
-        marg[0][threadIdx.x] = a;
-        marg[1][threadIdx.x] = b;
+    // every boundary is read above before any is rewritten below
+    if (threadIdx.x < 2 * kVB) {
+        const int ax = threadIdx.x / kVB, i = threadIdx.x % kVB;
+        double a = 0.0;
+        for (int k = 0; k < kVB; k++) a += ax == 0 ? sv[i * kVB + k] : sv[k * kVB + i];
+        marg[ax][i] = a;
     }
     __syncthreads();
-    if (threadIdx.x == 0) vegas_resize(marg[0], bt);
-    if (threadIdx.x == 64) vegas_resize(marg[1], bp);
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int i = 0; i < kVB; i++) t += marg[threadIdx.x][i];
+        tot[threadIdx.x] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * kVB) {
+        const int ax = threadIdx.x / kVB, i = threadIdx.x % kVB;
+        wgt[ax][i] = tot[ax] == 0. ? 0. : vegas_weight(marg[ax], tot[ax], i);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && tot[threadIdx.x] != 0.) vegas_sweep(wgt[threadIdx.x], threadIdx.x == 0 ? bt : bp);
 }
 
 // ------------------------------------------------------------------ single points
