@@ -288,8 +288,8 @@ def main():
                          "MC average of 262,144 directions on the built grid, for rocprofv3 --pmc")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (see DESIGN.md for the "
-                                   "multi-GPU sharding design)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (the path shards by shell "
+                                   "ranges and directions across GPUs, which is not built)"}))
         sys.exit(2)
 
     import dplus_mi355x as D

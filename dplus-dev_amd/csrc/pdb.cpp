@@ -202,7 +202,7 @@ PdbAtoms read_pdb(const std::string &path) {
         }
         const int s = species_index(key);
         if (s < 0)
-            throw Error(kErrUnimplemented, "no X-ray form factor in this build for atom type \"" + key + "\" (" + path + ")");
+            throw Error(kErrUnimplemented, "no X-ray form factor table entry for atom type \"" + key + "\" (" + path + ")");
         out.xyz.push_back(R[i].x);
         out.xyz.push_back(R[i].y);
         out.xyz.push_back(R[i].z);

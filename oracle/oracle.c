@@ -116,7 +116,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * direction d of the chunk belongs to run w = d / 64 of W = C / 64 runs; run w
  * covers patch (w / nv, w % nv) of the nu x nv grid of equal-area patches in
  * (u, v) = ((1 - cos theta) / 2, phi / 2 pi), with nu the largest divisor of W
- * with 2 nu^2 <= W and nv = W / nu.  (This build's sampling design: the
+ * with 2 nu^2 <= W and nv = W / nu.  (The HIP path's sampling design: the
  * reference draws unstratified directions from a clock-seeded MTGP32 stream,
  * GPUHybridCalc.cu:196-198, so only the estimator, not the stream, is shared.) */
 void orc_mc_stratify(long long d, int chunk, double *u, double *v) {
