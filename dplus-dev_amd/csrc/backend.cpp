@@ -220,6 +220,72 @@ std::string file_key(const std::string &path) {
     return path + "|" + std::to_string(mt) + "|" + std::to_string(sz);
 }
 
+// the reference's PDB reader holds coordinates and weights in float
+// (PDBReaderOb<float>) and centres in float, in file order (moveCOMToOrigin,
+// PDBReaderLib.cpp:271-292); the same arithmetic here keeps the voxelised
+// layer on the reference's voxels
+std::vector<float> centred_coords(const PdbAtoms &A, bool centered) {
+    const int natoms = (int)A.species.size();
+    std::vector<float> xf(3 * (size_t)natoms);
+    for (size_t i = 0; i < xf.size(); i++) xf[i] = (float)A.xyz[i];
+    if (centered) {
+        float c[3] = {0.f, 0.f, 0.f}, w = 0.f;
+        for (int a = 0; a < natoms; a++) {
+            const float wt = (float)kElements[A.species[a]].weight;
+            for (int k = 0; k < 3; k++) c[k] += xf[3 * a + k] * wt;
+            w += wt;
+        }
+        for (int k = 0; k < 3; k++) c[k] /= w;
+        for (int a = 0; a < natoms; a++)
+            for (int k = 0; k < 3; k++) xf[3 * a + k] -= c[k];
+    }
+    return xf;
+}
+
+// ---- PDB text of a model (GetPDB)
+// EulerD<float> (Amplitude.h:557-573): R = Rx(a) Ry(b) Rz(c) evaluated in float
+void euler_float(float a, float b, float c, float R[9]) {
+    const float c1 = std::cos(a), s1 = std::sin(a), c2 = std::cos(b), s2 = std::sin(b), c3 = std::cos(c),
+                s3 = std::sin(c);
+    const float m[9] = {c2 * c3, -c2 * s3, s2, c1 * s3 + c3 * s1 * s2, c1 * c3 - s1 * s2 * s3, -c2 * s1,
+                        s1 * s3 - c1 * c3 * s2, c3 * s1 + c1 * s2 * s3, c1 * c2};
+    for (int i = 0; i < 9; i++) R[i] = m[i];
+}
+
+struct PdbPoint {
+    float p[3];
+};
+
+void rigid(const float R[9], const float t[3], PdbPoint &x) {
+    float o[3];
+    for (int i = 0; i < 3; i++) o[i] = R[3 * i] * x.p[0] + R[3 * i + 1] * x.p[1] + R[3 * i + 2] * x.p[2] + t[i];
+    for (int i = 0; i < 3; i++) x.p[i] = o[i];
+}
+
+// one ATOM record in the reference's fixed 80-byte layout (PDBAmplitude.cpp:1469-1538):
+// the fields cut to their column widths, occupancy and temperature factor as
+// the first 6 characters of "%6f" each, coordinates in Angstrom as the first 8
+// characters of "%8f"; bytes past the formatted text stay zero, as there
+std::string pdb_record(const PdbAtoms &A, size_t i, const PdbPoint &x) {
+    char occ[40] = {0};
+    snprintf(occ, sizeof occ, "%6f", A.occ[i]);
+    snprintf(occ + 6, sizeof occ - 6, "%6f", A.bfac[i]);
+    const std::string occ12(occ, 12);
+    const std::string place = "-" + std::string(22, 'p') + "-";
+    char buf[160] = {0};
+    snprintf(buf, sizeof buf, "ATOM  %5s %4s %3s %c%4s    %s%s      %s%s", A.serial[i].substr(0, 5).c_str(),
+             A.name[i].substr(0, 4).c_str(), A.res_name[i].substr(0, 3).c_str(), A.chain[i],
+             A.res_no[i].substr(0, 4).c_str(), place.c_str(), occ12.c_str(), A.seg[i].substr(0, 4).c_str(),
+             A.elem4[i].substr(0, 4).c_str());
+    std::string line(buf, 80);
+    for (int k = 0; k < 3; k++) {
+        char t[64] = {0};
+        snprintf(t, sizeof t, "%8f", (double)x.p[k] * 10.);
+        for (int c = 0; c < 8; c++) line[30 + 8 * k + c] = t[c];
+    }
+    return line;
+}
+
 std::string amp_header(const ModelNode &n, const Grid &g) {
     std::ostringstream o;
     o << "# Model: " << (is_pdb(n) ? "PDB" : n.type == "AMP" ? "AMP" : is_space_filling(n) ? "Space-filling Symmetry" : "Manual Symmetry")
@@ -247,6 +313,8 @@ struct Backend {
     std::vector<std::pair<unsigned, std::string>> headers;
     std::map<unsigned, Grid *> model_grid;  // grids of this job's models (get/save amplitude)
     std::string cache_dir;
+    Spec job_spec;  // model tree of the last started job (GetPDB)
+    bool have_job = false;
     // device-resident caches across jobs
     std::map<std::string, PdbAtoms> pdb_cache;
     std::map<std::string, std::unique_ptr<Structure>> structures;
@@ -279,23 +347,7 @@ struct Backend {
             if (pit == pdb_cache.end()) pit = pdb_cache.emplace(fk, read_pdb(n.filename)).first;
             const PdbAtoms &A = pit->second;
             const int natoms = (int)A.species.size();
-            // the reference's PDB reader holds coordinates and weights in float
-            // (PDBReaderOb<float>) and centres in float, in file order
-            // (moveCOMToOrigin, PDBReaderLib.cpp:271-292); the same arithmetic here
-            // keeps the voxelised layer on the reference's voxels
-            std::vector<float> xf(3 * (size_t)natoms);
-            for (size_t i = 0; i < xf.size(); i++) xf[i] = (float)A.xyz[i];
-            if (n.centered) {
-                float c[3] = {0.f, 0.f, 0.f}, w = 0.f;
-                for (int a = 0; a < natoms; a++) {
-                    const float wt = (float)kElements[A.species[a]].weight;
-                    for (int k = 0; k < 3; k++) c[k] += xf[3 * a + k] * wt;
-                    w += wt;
-                }
-                for (int k = 0; k < 3; k++) c[k] /= w;
-                for (int a = 0; a < natoms; a++)
-                    for (int k = 0; k < 3; k++) xf[3 * a + k] -= c[k];
-            }
+            const std::vector<float> xf = centred_coords(A, n.centered);
             std::vector<double> xyz(xf.begin(), xf.end());
             // ion table: elements present, in table order (sorted ion indices)
             std::vector<int> row(kNumElements, -1);
@@ -532,6 +584,8 @@ struct Backend {
         have_results = false;
         stop_flag = false;
         model_ptrs = s.ptrs;
+        job_spec = s;
+        have_job = true;
         headers.clear();
         model_grid.clear();
         job_grids.clear();
@@ -558,6 +612,90 @@ struct Backend {
         o += "],\"Graph\":[";
         for (size_t i = 0; i < graph.size(); i++) o += (i ? "," : "") + json::num(graph[i]);
         return o + "]}";
+    }
+
+    // ---- GetPDB (LocalBackend::HandleGetPDB, LocalBackend.cpp:969-1015)
+    struct AtomAt {
+        const PdbAtoms *A;
+        size_t i;
+        PdbPoint x;
+    };
+
+    static const ModelNode *find_node(const std::vector<ModelNode> &ms, unsigned ptr) {
+        for (auto &m : ms) {
+            if (m.ptr == ptr) return &m;
+            if (const ModelNode *c = find_node(m.children, ptr)) return c;
+        }
+        return nullptr;
+    }
+
+    const PdbAtoms &atoms_of(const ModelNode &n) {
+        const std::string fk = file_key(n.filename);
+        auto it = pdb_cache.find(fk);
+        if (it == pdb_cache.end()) it = pdb_cache.emplace(fk, read_pdb(n.filename)).first;
+        return it->second;
+    }
+
+    // PDBAmplitude::AssemblePDBFile (PDBAmplitude.cpp:1504-1538) and
+    // Symmetry::AssemblePDBFile (Symmetry.cpp:278-351): atoms in the model's own
+    // frame (rotation R(alpha, beta, gamma) in float, then its translation); a
+    // symmetry repeats its children at every sub-location, then applies its own
+    // frame.  Amplitude-file leaves hold no atoms.
+    void assemble(const ModelNode &n, std::vector<AtomAt> &out) {
+        float R[9], t[3] = {(float)n.loc[0], (float)n.loc[1], (float)n.loc[2]};
+        euler_float((float)n.loc[3], (float)n.loc[4], (float)n.loc[5], R);
+        if (is_pdb(n)) {
+            const PdbAtoms &A = atoms_of(n);
+            const std::vector<float> xf = centred_coords(A, n.centered);
+            for (size_t i = 0; i < A.species.size(); i++) {
+                AtomAt a{&A, i, {{xf[3 * i], xf[3 * i + 1], xf[3 * i + 2]}}};
+                rigid(R, t, a.x);
+                out.push_back(a);
+            }
+            return;
+        }
+        if (!is_symmetry(n)) return;
+        const std::vector<SubLoc> subs = sub_locations(n);
+        for (size_t k = 0; k < subs.size(); k++) {
+            float Rs[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+            const float ts[3] = {(float)subs[k].t[0], (float)subs[k].t[1], (float)subs[k].t[2]};
+            if (!is_space_filling(n))
+                euler_float((float)(n.rows[k][3] * kPi / 180.0), (float)(n.rows[k][4] * kPi / 180.0),
+                            (float)(n.rows[k][5] * kPi / 180.0), Rs);
+            const size_t first = out.size();
+            for (auto &c : n.children) assemble(c, out);
+            for (size_t a = first; a < out.size(); a++) {
+                rigid(Rs, ts, out[a].x);
+                rigid(R, t, out[a].x);
+            }
+        }
+    }
+
+    std::string pdb_text(unsigned ptr) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!have_job) throw Error(kErrJobNotFound, "no job has been started");
+        if (running) throw Error(kErrJobRunning, "the job is still running");
+        std::vector<ModelNode> roots;
+        for (auto &p : job_spec.pops)
+            for (auto &m : p.models) roots.push_back(m);
+        const ModelNode *n = find_node(roots, ptr);
+        if (!n || !(is_pdb(*n) || is_symmetry(*n)))
+            throw Error(kErrModelNotFound, "model " + std::to_string(ptr) + " has no atoms in this job");
+        std::string out;
+        if (is_symmetry(*n)) {
+            // Symmetry::SavePDBFile (Symmetry.cpp:234-276): the model header as REMARK lines
+            std::string st = "REMARK This is the header of the D+ state that was used to create the PDB below.\n";
+            for (auto &h : headers)
+                if (h.first == ptr) st += h.second;
+            std::string r;
+            for (char c : st) r += c == '\n' ? std::string("\nREMARK ") : std::string(1, c);
+            out = r + "\n";
+        }
+        std::vector<AtomAt> atoms;
+        assemble(*n, atoms);
+        out.reserve(out.size() + atoms.size() * 81);
+        for (auto &a : atoms) out += pdb_record(*a.A, a.i, a.x) + "\n";
+        return out;
     }
 
     Grid *grid_of(unsigned ptr) {
@@ -681,6 +819,18 @@ int dplus_backend_get_amplitude(dplus_backend *b, unsigned model_ptr, char **byt
         *bytes = (char *)malloc(s.size());
         if (!*bytes) throw Error(kErrOutOfMemory, "out of host memory");
         memcpy(*bytes, s.data(), s.size());
+        *len = (long long)s.size();
+    })
+}
+
+int dplus_backend_get_pdb(dplus_backend *b, unsigned model_ptr, char **text, long long *len) {
+    BACKEND_GUARD({
+        if (!b || !text || !len) throw Error(kErrInvalidArgs, "null argument");
+        const std::string s = ((Backend *)b)->pdb_text(model_ptr);
+        *text = (char *)malloc(s.size() + 1);
+        if (!*text) throw Error(kErrOutOfMemory, "out of host memory");
+        memcpy(*text, s.data(), s.size());
+        (*text)[s.size()] = '\0';
         *len = (long long)s.size();
     })
 }

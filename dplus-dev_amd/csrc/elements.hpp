@@ -70,6 +70,13 @@ struct PdbAtoms {
     std::vector<double> xyz;
     std::vector<int> species;
     int implicit_atoms = 0, implicit_residues = 0;
+    // the record fields the reference keeps for writing the model back out
+    // (PDBReaderLib.cpp:862-931): serial (cols 7-11), name (13-16), residue
+    // name (18-20), chain (22), residue number (23-26), occupancy (55-60),
+    // temperature factor (61-66), segment (73-76), element field (77-80)
+    std::vector<std::string> serial, name, res_name, res_no, seg, elem4;
+    std::vector<char> chain;
+    std::vector<double> occ, bfac;
 };
 PdbAtoms read_pdb(const std::string &path);
 

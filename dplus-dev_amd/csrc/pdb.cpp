@@ -100,8 +100,9 @@ std::string lower(std::string s) {
 }
 
 struct Record {
-    double x, y, z;
-    std::string elem4, name, res_name, res_no;
+    double x, y, z, occ, bfac;
+    std::string elem4, name, res_name, res_no, serial, seg;
+    char chain;
 };
 
 // labels of one residue as implicit groups, or nothing when any label is unknown
@@ -171,6 +172,11 @@ PdbAtoms read_pdb(const std::string &path) {
         r.name = field(12, 4);
         r.res_name = field(17, 3);
         r.res_no = field(22, 4);
+        r.serial = field(6, 5);
+        r.chain = line.size() > 21 ? line[21] : '\0';
+        r.occ = atof(field(54, 6).c_str());
+        r.bfac = atof(field(60, 6).c_str());
+        r.seg = field(72, 4);
         R.push_back(r);
     }
     if (R.empty()) throw Error(kErrInvalidArgs, "The file does not contains any valid ATOM or HETATM lines: " + path);
@@ -207,6 +213,15 @@ PdbAtoms read_pdb(const std::string &path) {
         out.xyz.push_back(R[i].y);
         out.xyz.push_back(R[i].z);
         out.species.push_back(s);
+        out.serial.push_back(R[i].serial);
+        out.name.push_back(R[i].name);
+        out.res_name.push_back(R[i].res_name);
+        out.res_no.push_back(R[i].res_no);
+        out.seg.push_back(R[i].seg);
+        out.elem4.push_back(R[i].elem4);
+        out.chain.push_back(R[i].chain);
+        out.occ.push_back(R[i].occ);
+        out.bfac.push_back(R[i].bfac);
     }
     return out;
 }

@@ -130,6 +130,7 @@ SIGNATURES = {
     "dplus_backend_get_model_ptrs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint), ctypes.c_int, _I]),
     "dplus_backend_get_amplitude": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, _VP,
                                                    ctypes.POINTER(ctypes.c_longlong)]),
+    "dplus_backend_get_pdb": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, _VP, ctypes.POINTER(ctypes.c_longlong)]),
     "dplus_backend_save_amplitude": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint, ctypes.c_char_p]),
     "dplus_backend_stop": (ctypes.c_int, [ctypes.c_void_p]),
     "dplus_engine_create": (ctypes.c_int, [ctypes.POINTER(EngineConfig), ctypes.POINTER(ctypes.c_void_p)]),

@@ -66,6 +66,10 @@ class Backend:
         return self._wrapper.get_amp(modelptr)
 
     @_guard
+    def get_pdb(self, modelptr):
+        return self._wrapper.get_pdb(modelptr)
+
+    @_guard
     def get_model_ptrs(self):
         return self._wrapper.get_model_ptrs()
 
@@ -111,6 +115,9 @@ class EmbeddedLocalRunner:
 
     def get_amp(self, modelptr):
         return self.wrapper.get_amp(modelptr)
+
+    def get_pdb(self, model_ptr):
+        return self.wrapper.get_pdb(model_ptr)
 
     def get_model_ptrs(self):
         return self.wrapper.get_model_ptrs()

@@ -80,6 +80,12 @@ class BackendWrapper:
         _call(L.lib().dplus_backend_get_amplitude(self._h, int(modelptr), ctypes.byref(p), ctypes.byref(n)))
         return L.take_string(p, n.value)
 
+    def get_pdb(self, modelptr):
+        """GetPDB decoded as BackendWrapper.get_pdb does (wrappers/BackendWrapper.pyx:40-43)."""
+        p, n = ctypes.c_void_p(), ctypes.c_longlong()
+        _call(L.lib().dplus_backend_get_pdb(self._h, int(modelptr), ctypes.byref(p), ctypes.byref(n)))
+        return L.take_string(p, n.value).decode("utf-8")
+
     def get_model_ptrs(self):
         n = ctypes.c_int()
         _call(L.lib().dplus_backend_get_model_ptrs(self._h, None, 0, ctypes.byref(n)))

@@ -240,6 +240,14 @@ int dplus_backend_get_model_ptrs(dplus_backend *b, unsigned *ptrs, int capacity,
 /* GetAmplitude (PythonBackendWrapper.cpp:188-198): the model's grid as a legacy
  * .amp byte stream of *len bytes */
 int dplus_backend_get_amplitude(dplus_backend *b, unsigned model_ptr, char **bytes, long long *len);
+/* GetPDB (PythonBackendWrapper.cpp:200-211, LocalBackend.cpp:969-1015): the
+ * atoms of a PDB model in its own frame, or of a symmetry assembled over its
+ * sub-locations after REMARK lines with its header, as 80-byte ATOM records
+ * (PDBAmplitude.cpp:1469-1538, Symmetry.cpp:234-351); *len bytes (records can
+ * hold zero bytes past their text, as the reference's do), released with
+ * dplus_free.  ERROR_JOBNOTFOUND (3) before any job, ERROR_MODELNOTFOUND (14)
+ * for a model without atoms. */
+int dplus_backend_get_pdb(dplus_backend *b, unsigned model_ptr, char **text, long long *len);
 /* SaveAmplitude (PythonBackendWrapper.cpp:176-186): .ampj (or .amp) file */
 int dplus_backend_save_amplitude(dplus_backend *b, unsigned model_ptr, const char *path);
 /* Stop (PythonBackendWrapper.cpp:225-235): the running job ends with ERROR_STOPPED (1) */

@@ -59,6 +59,14 @@ def test_idle_status_and_no_results():
     assert e.value.error_code == 14  # ERROR_MODELNOTFOUND
 
 
+def test_get_pdb_without_a_job_is_code_3():
+    """GetPDB before any job: ERROR_JOBNOTFOUND (LocalBackend.cpp:973-975)."""
+    from dplus_mi355x.backend import Backend, BackendError
+    with pytest.raises(BackendError) as e:
+        Backend().get_pdb(2)
+    assert e.value.error_code == 3
+
+
 def test_metadata_lists_hot_path_models():
     from dplus_mi355x.backend import Backend
     md = Backend().get_all_model_metadata()

@@ -368,3 +368,46 @@ def test_tubulin_symmetry_fixture(name, tmp_path):
         frac = [float(np.mean((d > k * s) & m)) for k in (1, 2, 3)]
         say(f"    outside 1/2/3 sigma: {frac}")
         assert frac[0] < 0.5 and frac[1] < 0.3 and frac[2] < 0.1, frac
+
+
+# ---------------------------------------------------------------- GetPDB
+def test_get_pdb_records_and_assembly(subunit):
+    """GetPDB of a PDB leaf and of a two-level manual symmetry: 80-byte ATOM
+    records whose first 30 columns repeat the input file's, occupancy and
+    temperature factor as the reference prints them, and coordinates equal to
+    the centred atoms moved through every frame of the tree (the independent
+    flatten above), to the 0.001 Angstrom the records carry; REMARK header
+    lines before a symmetry's atoms; unknown models are ERROR_MODELNOTFOUND."""
+    from dplus_mi355x.backend import BackendError
+    say("[case] GetPDB of a leaf and of a symmetry tree")
+    tree = _tree(",26", subunit, True)
+    r, _ = run({"state": state(domain(20), [tree]), "x": list(np.linspace(0.0, 3.0, 12))})
+    w = np.array([WEIGHT[e] for e in subunit["els"]])
+    x32 = np.asarray(subunit["xyz"], np.float32)
+    centred = x32 - ((x32 * w[:, None].astype(np.float32)).sum(0) / np.float32(w.sum())).astype(np.float32)
+    src = [l.rstrip("\n") for l in open(subunit["path"]) if l.startswith("ATOM")]
+
+    def atoms_of(text):
+        recs = [l for l in text.split("\n") if l.startswith("ATOM")]
+        assert all(len(l) == 80 for l in recs)
+        return recs, np.array([[float(l[30:38]), float(l[38:46]), float(l[46:54])] for l in recs])
+
+    leaf = r.get_pdb(12)
+    recs, xyz = atoms_of(leaf)
+    assert len(recs) == len(src)
+    for l, s in zip(recs, src):
+        assert l[:30] == s[:30] and l[54:66] == "1.00000.0000" and l[66:72] == " " * 6 and l[76:80] == s[76:78] + "  "
+    m = tree["Children"][0]["Children"][0]
+    L = m["Location"]
+    want = centred @ euler(L["alpha"], L["beta"], L["gamma"]).T + np.array([L["x"], L["y"], L["z"]])
+    assert np.max(np.abs(xyz - 10.0 * want)) < 2e-3
+    text = r.get_pdb(10)
+    assert text.startswith("REMARK This is the header of the D+ state that was used to create the PDB below.\nREMARK ")
+    recs, xyz = atoms_of(text)
+    flat = flatten(tree)
+    want = np.concatenate([centred @ R.T + t for _, R, t, _ in flat])
+    assert len(recs) == len(flat) * len(src)
+    assert np.max(np.abs(xyz - 10.0 * want)) < 2e-3
+    with pytest.raises(BackendError) as e:
+        r.get_pdb(999)
+    assert e.value.error_code == 14
