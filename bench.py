@@ -99,14 +99,15 @@ def cpu_baseline(W, g64_data, g64_interp, budget_nodes=1500):
                       f"(scaled x{W['iters'] * W['ppb']}); full curve estimate {total:.0f} s"}
 
 
-def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4):
+def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, streams=8, fp64=True):
     """Batched serving stream: n_requests with Poisson arrivals at `rate`/s; each
     request is one of `pool_size` seeded random-walk structures (atoms
     log-uniform in 500..200,000 at 50 atoms/nm^3, dummy-atom solvent), qmax
     U(2, 8) nm^-1, grid size round-up-10(2 qmax L / pi + 3) with L the structure's
     diameter, 200..1,000 q points, MC (65,536 directions) or VEGAS (5 x 4,096
-    points) 50/50.  The mixed engine (4 streams, 200 GB HBM budget) serves the
-    stream; the plain FP64 path then runs the same requests one at a time."""
+    points) 50/50.  The mixed engine (`streams` streams, 200 GB HBM budget)
+    serves the stream; the plain FP64 path then runs the same requests one at a
+    time (fp64=False skips it)."""
     from dplus_mi355x.serving import Engine
     rng = np.random.default_rng(seed)
     t0 = time.time()
@@ -132,7 +133,7 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4):
         reqs.append((q, kw, n))
     log(f"[serving] {n_requests} requests over {pool_size} structures prepared in {time.time() - t0:.1f} s")
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
-    eng = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=4)
+    eng = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=streams)
     tickets = []
     start = time.perf_counter()
     for (q, kw, _), at in zip(reqs, arrivals):
@@ -147,6 +148,9 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4):
     lat = np.array([r["latency_ms"] for _, r in mixed])
     log(f"[serving] mixed: {n_requests / span:.2f} curves/s over {span:.2f} s, p50 {np.percentile(lat, 50):.1f} ms, "
         f"p99 {np.percentile(lat, 99):.1f} ms, max concurrent {st['max_concurrent']}")
+    if not fp64:
+        return {"value": n_requests / span, "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "streams": streams}
     eng64 = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=1)
     t64 = time.perf_counter()
     errs, lat64 = [], []
@@ -164,7 +168,7 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4):
             "p99_ms": float(np.percentile(lat, 99)), "fp64_value": n_requests / span64,
             "fp64_p50_ms": float(np.percentile(lat64, 50)), "fp64_p99_ms": float(np.percentile(lat64, 99)),
             "speedup_vs_fp64": (n_requests / span) / (n_requests / span64), "max_rel_err_vs_fp64": max(errs),
-            "streams": 4, "max_concurrent": st["max_concurrent"], "peak_reserved_bytes": st["peak_reserved"],
+            "streams": streams, "max_concurrent": st["max_concurrent"], "peak_reserved_bytes": st["peak_reserved"],
             "hbm_budget_bytes": st["budget"], "atoms_total": int(sum(n for _, _, n in reqs))}
 
 

@@ -418,6 +418,9 @@ void Structure::build(Grid &g, bool host_copy, hipStream_t s) {
     if (atoms_on)
         for (int i = 0; i < desc.nion; i++) origin += ion_counts[i] * ff[i];
     if (boxes_on) origin += desc.box_contrast * box_volume;
+    // per-build buffers (form factors of this grid's shells, slice partials): a
+    // structure can be built into several grids at once on different streams
+    DevBuf bff, bpart;
     bff.upload(ff.data(), ff.size() * sizeof(double), s);
     const long long nhalf = half_base(g.N + 1);
     const bool phase64 = getenv("DPLUS_PHASE64") != nullptr;
@@ -452,7 +455,7 @@ void Structure::build(Grid &g, bool host_copy, hipStream_t s) {
     }
     launch_set_origin(g.fp64(), g.dev.p, origin * desc.scale, s);
     g.splines(s);
-    last_terms = (double)nhalf * (double)nsources;
+    // the per-build buffers are released after the stream has drained
     if (host_copy)
         g.download(s);
     else

@@ -167,13 +167,13 @@ struct Structure {
     dplus_pdb_desc desc{};
     std::vector<double> coef9, rad;
     std::vector<int> ion_counts;
-    double box_volume = 0.0, last_terms = 0.0;
+    double box_volume = 0.0;
     double rmax = 0.0;  // largest |position| of an atom or box centre (nm)
     bool atoms_on = false, boxes_on = false;
     int ntiles = 0;
     std::vector<int> tile_len;  // terms of each tile
     long long nsources = 0;
-    DevBuf b32, b64, bt, bd, bff, bpart;
+    DevBuf b32, b64, bt, bd;  // immutable after construction
     Structure(const dplus_pdb_desc *d, hipStream_t s);
     void build(Grid &g, bool host_copy, hipStream_t s);
 };

@@ -50,9 +50,31 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
+// 64-bit content hash (multiply-rotate mixing over 8-byte words); two seeds
+// give the 128-bit key of the structure cache
+uint64_t mix64(uint64_t h, uint64_t v) {
+    h ^= v * 0x9E3779B97F4A7C15ULL;
+    h = (h << 29 | h >> 35) * 0xBF58476D1CE4E5B9ULL;
+    return h ^ (h >> 31);
+}
+
+uint64_t hash_bytes(uint64_t h, const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t v;
+        memcpy(&v, b + i, 8);
+        h = mix64(h, v);
+    }
+    uint64_t t = 0;
+    for (size_t k = 0; i + k < n; k++) t |= (uint64_t)b[i + k] << (8 * k);
+    return mix64(h, t ^ ((uint64_t)n << 56));
+}
+
 struct Request {
     long long id = 0;
     bool has_pdb = false;
+    uint64_t key[2] = {0, 0};  // content hash of the PDB description (structure cache)
     std::vector<double> xyz, coef9, rad, box_com;
     std::vector<int> ion, box_dims;
     dplus_pdb_desc desc{};
@@ -129,6 +151,20 @@ struct Engine {
     std::map<long long, std::unique_ptr<Request>> all;
     long long next_id = 1, submitted = 0, completed = 0, max_concurrent = 0;
     size_t budget = 0, reserved = 0, peak_reserved = 0;
+    // structure cache: device-resident atoms/boxes of recent PDB descriptions,
+    // shared by requests with the same content; at most cache_cap bytes, which
+    // admission keeps free of request reservations
+    struct Cached {
+        uint64_t key[2];
+        std::shared_ptr<Structure> s;
+        size_t bytes;
+        unsigned long long last;
+    };
+    std::mutex cache_mu;
+    std::vector<Cached> cache;
+    size_t cache_cap = 0, cache_bytes = 0;
+    unsigned long long cache_tick = 0;
+    long long cache_hits = 0, cache_misses = 0;
     int busy = 0;
     bool stopping = false;
     std::vector<hipStream_t> streams;
@@ -138,6 +174,7 @@ struct Engine {
         Device &d = Device::get();
         d.require();
         budget = budget_bytes > 0 ? (size_t)budget_bytes : (size_t)(0.85 * (double)d.mem);
+        cache_cap = std::min<size_t>(budget / 16, (size_t)8 << 30);
         nstreams = std::max(1, std::min(16, nstreams));
         for (int i = 0; i < nstreams; i++) {
             hipStream_t s;
@@ -155,6 +192,7 @@ struct Engine {
         }
         cv_work.notify_all();
         for (auto &t : workers) t.join();
+        cache.clear();  // frees are ordered on the worker streams: release before destroying them
         for (auto s : streams) {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
@@ -166,7 +204,7 @@ struct Engine {
         while (!pending.empty()) {
             Request *r = pending.front();
             if ((int)admitted.size() + busy >= (int)streams.size()) break;
-            if (reserved + r->need > budget) break;
+            if (reserved + r->need + cache_cap > budget) break;
             pending.pop_front();
             reserved += r->need;
             peak_reserved = std::max(peak_reserved, reserved);
@@ -209,13 +247,47 @@ struct Engine {
         }
     }
 
-    static void run(Request &r, hipStream_t s) {
+    // the resident structure of a request's PDB description: from the cache, or
+    // built on the request's stream and cached (least recently used entries that
+    // no running request holds are dropped past the cap)
+    std::shared_ptr<Structure> structure_for(const Request &r, hipStream_t s) {
+        auto same = [&](const Cached &c) { return c.key[0] == r.key[0] && c.key[1] == r.key[1]; };
+        {
+            std::lock_guard<std::mutex> lk(cache_mu);
+            for (auto &c : cache)
+                if (same(c)) {
+                    c.last = ++cache_tick;
+                    cache_hits++;
+                    return c.s;
+                }
+        }
+        std::shared_ptr<Structure> S(new Structure(&r.desc, s));
+        const size_t bytes = (size_t)(r.desc.natoms + r.desc.nbox) * 48 + (size_t)(S->ntiles + 1) * 16 + 4096;
+        std::lock_guard<std::mutex> lk(cache_mu);
+        cache_misses++;
+        for (auto &c : cache)
+            if (same(c)) return c.s;  // another worker built it meanwhile
+        if (bytes > cache_cap) return S;
+        cache.push_back({{r.key[0], r.key[1]}, S, bytes, ++cache_tick});
+        cache_bytes += bytes;
+        while (cache_bytes > cache_cap) {
+            int victim = -1;
+            for (size_t i = 0; i < cache.size(); i++)
+                if (cache[i].s.use_count() == 1 && (victim < 0 || cache[i].last < cache[victim].last)) victim = (int)i;
+            if (victim < 0) break;
+            cache_bytes -= cache[victim].bytes;
+            cache.erase(cache.begin() + victim);
+        }
+        return S;
+    }
+
+    void run(Request &r, hipStream_t s) {
         try {
-            std::unique_ptr<Structure> S;
+            std::shared_ptr<Structure> S;
             std::unique_ptr<Grid> G;
             Grid *g = r.grid;
             if (r.has_pdb) {
-                S.reset(new Structure(&r.desc, s));
+                S = structure_for(r, s);
                 G.reset(new Grid(r.grid_size, r.qmax, r.precision));
                 S->build(*G, false, s);
                 g = G.get();
@@ -381,7 +453,7 @@ int dplus_engine_create(const dplus_engine_config *cfg, dplus_engine **out) {
     ENGINE_GUARD({
         if (!out) throw Error(kErrInvalidArgs, "null output");
         const long long budget = cfg ? cfg->hbm_budget_bytes : 0;
-        const int streams = cfg && cfg->streams > 0 ? cfg->streams : 4;
+        const int streams = cfg && cfg->streams > 0 ? cfg->streams : 8;
         *out = (dplus_engine *)new Engine(budget, streams);
     })
 }
@@ -413,6 +485,20 @@ int dplus_engine_submit(dplus_engine *eh, const dplus_request *rq, long long *ti
             r->desc.rad = r->rad.data();
             r->desc.box_dims = d.nbox > 0 ? r->box_dims.data() : nullptr;
             r->desc.box_com = d.nbox > 0 ? r->box_com.data() : nullptr;
+            for (int k = 0; k < 2; k++) {
+                uint64_t h = k ? 0x243F6A8885A308D3ULL : 0x13198A2E03707344ULL;
+                const double sc[6] = {d.sol_ed, d.c1, d.scale, d.vox_step, d.box_contrast, (double)d.flags};
+                const long long n[3] = {d.natoms, d.nion, d.nbox};
+                h = hash_bytes(h, n, sizeof n);
+                h = hash_bytes(h, sc, sizeof sc);
+                h = hash_bytes(h, r->xyz.data(), r->xyz.size() * sizeof(double));
+                h = hash_bytes(h, r->ion.data(), r->ion.size() * sizeof(int));
+                h = hash_bytes(h, r->coef9.data(), r->coef9.size() * sizeof(double));
+                h = hash_bytes(h, r->rad.data(), r->rad.size() * sizeof(double));
+                h = hash_bytes(h, r->box_dims.data(), r->box_dims.size() * sizeof(int));
+                h = hash_bytes(h, r->box_com.data(), r->box_com.size() * sizeof(double));
+                r->key[k] = h;
+            }
             r->grid_size = rq->grid_size;
             r->qmax = rq->qmax;
             r->precision = rq->precision;
@@ -446,7 +532,7 @@ int dplus_engine_submit(dplus_engine *eh, const dplus_request *rq, long long *ti
         }
         r->oa = rq->oa;
         r->need = request_bytes(*r);
-        if (r->need > e->budget)
+        if (r->need + e->cache_cap > e->budget)
             throw Error(kErrOutOfMemory, "the request needs " + std::to_string(r->need) +
                                              " device bytes, more than the engine's budget of " +
                                              std::to_string(e->budget));
@@ -513,6 +599,19 @@ int dplus_engine_stats(dplus_engine *eh, long long *submitted, long long *comple
         if (peak_reserved) *peak_reserved = (long long)e->peak_reserved;
         if (budget) *budget = (long long)e->budget;
         if (max_concurrent) *max_concurrent = e->max_concurrent;
+    })
+}
+
+int dplus_engine_cache_stats(dplus_engine *eh, long long *hits, long long *misses, long long *bytes,
+                             long long *capacity) {
+    ENGINE_GUARD({
+        Engine *e = (Engine *)eh;
+        if (!e) throw Error(kErrInvalidArgs, "null engine");
+        std::lock_guard<std::mutex> lk(e->cache_mu);
+        if (hits) *hits = e->cache_hits;
+        if (misses) *misses = e->cache_misses;
+        if (bytes) *bytes = (long long)e->cache_bytes;
+        if (capacity) *capacity = (long long)e->cache_cap;
     })
 }
 

@@ -139,6 +139,7 @@ SIGNATURES = {
     "dplus_engine_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _I]),
     "dplus_engine_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _D, ctypes.POINTER(RequestResult)]),
     "dplus_engine_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 5),
+    "dplus_engine_cache_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
     "dplus_session_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _D, _D, _D,
                                             ctypes.c_ulonglong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)]),
     "dplus_session_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Leaf), ctypes.c_int, ctypes.c_double,

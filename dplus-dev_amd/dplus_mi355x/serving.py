@@ -13,7 +13,7 @@ class Engine:
     """dplus_engine_*: submit() returns a ticket at once; wait(ticket) gives
     (I(q), result dict)."""
 
-    def __init__(self, hbm_budget_bytes=0, streams=4):
+    def __init__(self, hbm_budget_bytes=0, streams=8):
         h = ctypes.c_void_p()
         cfg = L.EngineConfig(int(hbm_budget_bytes), int(streams))
         L.check(L.lib().dplus_engine_create(ctypes.byref(cfg), ctypes.byref(h)))
@@ -74,7 +74,11 @@ class Engine:
     def stats(self):
         v = [ctypes.c_longlong() for _ in range(5)]
         L.check(L.lib().dplus_engine_stats(self._h, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("submitted", "completed", "peak_reserved", "budget", "max_concurrent"), (x.value for x in v)))
+        out = dict(zip(("submitted", "completed", "peak_reserved", "budget", "max_concurrent"), (x.value for x in v)))
+        c = [ctypes.c_longlong() for _ in range(4)]
+        L.check(L.lib().dplus_engine_cache_stats(self._h, *[ctypes.byref(x) for x in c]))
+        out.update(zip(("cache_hits", "cache_misses", "cache_bytes", "cache_capacity"), (x.value for x in c)))
+        return out
 
     def close(self):
         if self._h:

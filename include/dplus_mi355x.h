@@ -262,7 +262,7 @@ typedef struct dplus_engine dplus_engine;
 
 typedef struct {
     long long hbm_budget_bytes; /* admission budget; <= 0: 85 % of device memory */
-    int streams;                /* concurrent requests (1..16); <= 0: 4 */
+    int streams;                /* concurrent requests (1..16); <= 0: 8 */
 } dplus_engine_config;
 
 typedef struct {
@@ -306,6 +306,13 @@ int dplus_engine_poll(dplus_engine *e, long long ticket, int *done);
 int dplus_engine_wait(dplus_engine *e, long long ticket, double *intensity, dplus_request_result *res);
 int dplus_engine_stats(dplus_engine *e, long long *submitted, long long *completed, long long *peak_reserved,
                        long long *budget, long long *max_concurrent);
+
+/* structure cache of the engine (no counterpart in the reference): requests
+ * whose PDB description has the same content (128-bit hash of every array and
+ * scalar) share one device-resident structure; at most budget/16 (<= 8 GiB) is
+ * cached, and admission keeps that much of the budget free of requests */
+int dplus_engine_cache_stats(dplus_engine *e, long long *hits, long long *misses, long long *bytes,
+                             long long *capacity);
 
 /* ------------------------------------------------------------------ fit sessions */
 /* Fitting-loop evaluation over resident subunit grids (the hybrid docking of

@@ -197,3 +197,28 @@ def test_concurrent_sessions_capture_while_others_allocate(dplus):
         eager, _ = dplus.orientation_average(dplus.Docking([(g, rot, nt, tr)]), q, seed=50 + k, max_dirs=2048,
                                              chunk=2048)
         assert np.array_equal(I, eager)
+
+
+def test_engine_structure_cache(dplus):
+    """Requests with the same PDB content share one resident structure (a cache
+    hit), a one-coordinate change is a different structure (a miss), and cached
+    structures give the same curves as a fresh engine."""
+    from dplus_mi355x.serving import Engine
+    say("[case] engine structure cache: hits, misses, results")
+    pdb = make_pdb(700, 41, 1.2)
+    moved = dict(pdb, xyz=pdb["xyz"].copy())
+    moved["xyz"][3, 0] += 0.05
+    q = np.linspace(0.05, 3.0, 40)
+    kw = dict(method=dplus.OA_MC, seed=3, max_dirs=4096, chunk=4096)
+    e = Engine(streams=2)
+    outs = [e.wait(e.submit(q, pdb=p, grid_size=gs, qmax=3.0, **kw))[0]
+            for p, gs in ((pdb, 20), (pdb, 26), (moved, 20), (pdb, 20))]
+    st = e.stats()
+    e.close()
+    say(f"    {st}")
+    assert st["cache_misses"] == 2 and st["cache_hits"] == 2 and 0 < st["cache_bytes"] <= st["cache_capacity"]
+    fresh = Engine(streams=1)
+    for (p, gs), I in zip(((pdb, 20), (pdb, 26), (moved, 20)), outs):
+        assert np.array_equal(fresh.wait(fresh.submit(q, pdb=p, grid_size=gs, qmax=3.0, **kw))[0], I)
+    fresh.close()
+    assert np.array_equal(outs[0], outs[3]) and not np.array_equal(outs[0], outs[2])
