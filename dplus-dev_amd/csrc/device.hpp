@@ -86,6 +86,11 @@ struct GroupDev {
     int layer, q0, m, pad;
 };
 
+// the 15 Kronrod abscissae on [-1, 1] (passed to the integrand kernel by value)
+struct GkNodes {
+    double x[15];
+};
+
 struct LeafGeo {
     int N;
     double step;

@@ -5,8 +5,9 @@
 // phi inside one in theta over the Friedel half sphere theta in [0, pi/2]
 // and divide by 2 pi.  The recursion is replayed level by level here: every
 // open phi interval of every q and every theta node is evaluated in one batch
-// of integrand calls on the device; accept/split decisions, and the sums
-// left + right in the reference's order, are taken on the host.
+// of integrand calls on the device (the host sends one task per interval, the
+// device forms its 15 points); accept/split decisions, the Gauss and Kronrod
+// sums and the sums left + right in the reference's order are taken on the host.
 
 #include <cmath>
 #include <vector>
@@ -17,6 +18,9 @@ namespace dplus {
 
 void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
                         const double4 *trans, int n, const double *qxyz, double *out, hipStream_t s);
+void launch_docked_gk(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
+                      const double4 *trans, long long ntasks, const double4 *tasks, const GkNodes &nodes,
+                      double *out, hipStream_t s);
 
 // Gauss-Kronrod 15/7 tables (Common/mathfuncs.h:100-125)
 static const double kX[15] = {
@@ -140,26 +144,37 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         for (size_t k = 0; k < zero_q.size(); k++) out[zero_q[k]] = vals[k];
     }
 
+    GkNodes nodes;
+    for (int i = 0; i < 15; i++) nodes.x[i] = kX[i];
+    std::vector<double4> tasks;
+    DevBuf btask;
     while (!open.empty()) {
-        // integrand at the 15 Kronrod nodes of every open phi interval
-        pts.resize(open.size() * 15 * 3);
+        // integrand at the 15 Kronrod nodes of every open phi interval: the
+        // interval's task (q sin theta, q cos theta, centre, half width)
+        tasks.resize(open.size());
         for (size_t t = 0; t < open.size(); t++) {
             const PhiTask &P = open[t];
             const ThetaTask &T = tt[P.theta_task];
             const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
             const double th = apb2 + ba2 * kX[P.k];
-            const double sT = std::sin(th), cT = std::cos(th);
-            const double pb = (P.pmax - P.pmin) * 0.5, pa = (P.pmax + P.pmin) * 0.5;
             const double qq = q[T.q];
-            for (int i = 0; i < 15; i++) {
-                const double xi = pa + pb * kX[i];
-                double *o = &pts[(t * 15 + i) * 3];
-                o[0] = qq * sT * std::cos(xi);
-                o[1] = qq * sT * std::sin(xi);
-                o[2] = qq * cT;
-            }
+            tasks[t] = make_double4(qq * std::sin(th), qq * std::cos(th), (P.pmax + P.pmin) * 0.5,
+                                    (P.pmax - P.pmin) * 0.5);
         }
-        evaluate(open.size() * 15);
+        {
+            const size_t npts = open.size() * 15;
+            btask.upload(tasks.data(), tasks.size() * sizeof(double4), s);
+            bo.ensure(npts * sizeof(double));
+            KTimer kt("gk", s);
+            launch_docked_gk(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
+                             (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (long long)open.size(),
+                             (const double4 *)btask.p, nodes, (double *)bo.p, s);
+            check_hip(hipGetLastError(), "gk launch");
+            vals.resize(npts);
+            check_hip(hipMemcpyAsync(vals.data(), bo.p, npts * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check_hip(hipStreamSynchronize(s), "gk batch");
+            evals += (long long)npts;
+        }
         std::vector<PhiTask> next;
         std::vector<int> finished_theta;
         for (size_t t = 0; t < open.size(); t++) {

@@ -604,13 +604,9 @@ __global__ void seeded_samples(const T4 *__restrict__ g, int N, double step, dou
 // with Symmetries.cpp:697-716): the integrand of Gauss-Kronrod
 
 template <typename T, typename T4>
-__global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
-                                  const RotDev *__restrict__ rots, int nrot,
-                                  const double4 *__restrict__ trans, int n,
-                                  const double *__restrict__ qxyz, double *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double q[3] = {qxyz[3 * i], qxyz[3 * i + 1], qxyz[3 * i + 2]};
+__device__ double docked_intensity(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
+                                   const RotDev *__restrict__ rots, int nrot, const double4 *__restrict__ trans,
+                                   const double q[3]) {
     double ax = 0.0, ay = 0.0;
     for (int r = 0; r < nrot; r++) {
         const RotDev rd = rots[r];
@@ -629,7 +625,34 @@ __global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const Lea
         ax += ((double)a.x * sc - (double)a.y * ss) * rd.scale;
         ay += ((double)a.y * sc + (double)a.x * ss) * rd.scale;
     }
-    out[i] = ax * ax + ay * ay;
+    return ax * ax + ay * ay;
+}
+
+template <typename T, typename T4>
+__global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
+                                  const RotDev *__restrict__ rots, int nrot,
+                                  const double4 *__restrict__ trans, int n,
+                                  const double *__restrict__ qxyz, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double q[3] = {qxyz[3 * i], qxyz[3 * i + 1], qxyz[3 * i + 2]};
+    out[i] = docked_intensity<T, T4>(grids, geo, rots, nrot, trans, q);
+}
+
+// Gauss-Kronrod phi intervals: task t = (q sin(theta), q cos(theta), a, b) of
+// one interval; its 15 integrand points are phi = a + b x_k, written to
+// out[15 t + k]
+template <typename T, typename T4>
+__global__ void docked_gk_batch(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
+                                const RotDev *__restrict__ rots, int nrot, const double4 *__restrict__ trans,
+                                long long ntasks, const double4 *__restrict__ tasks, GkNodes nodes,
+                                double *__restrict__ out) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 15 * ntasks) return;
+    const double4 tk = tasks[i / 15];
+    const double xi = tk.z + tk.w * nodes.x[i % 15];
+    const double q[3] = {tk.x * cos(xi), tk.x * sin(xi), tk.y};
+    out[i] = docked_intensity<T, T4>(grids, geo, rots, nrot, trans, q);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -690,6 +713,18 @@ void launch_seeded_samples(bool fp64, const void *g, int N, double step, double 
         seeded_samples<double, double4><<<nb, 256, 0, s>>>((const double4 *)g, N, step, q, n, phi, v2, out);
     else
         seeded_samples<float, float4><<<nb, 256, 0, s>>>((const float4 *)g, N, step, q, n, phi, v2, out);
+}
+
+void launch_docked_gk(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
+                      const double4 *trans, long long ntasks, const double4 *tasks, const GkNodes &nodes,
+                      double *out, hipStream_t s) {
+    const unsigned nb = (unsigned)((15 * ntasks + 255) / 256);
+    if (fp64)
+        docked_gk_batch<double, double4><<<nb, 256, 0, s>>>((const double4 *const *)grids, geo, rots, nrot, trans,
+                                                             ntasks, tasks, nodes, out);
+    else
+        docked_gk_batch<float, float4><<<nb, 256, 0, s>>>((const float4 *const *)grids, geo, rots, nrot, trans,
+                                                           ntasks, tasks, nodes, out);
 }
 
 void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots,
