@@ -68,6 +68,21 @@ def test_grid_build_matches_oracle(dplus, oracle, small, prec):
     assert relmax(g.interp, s["co"]) < (1e-10 if prec == 0 else 2e-5)
 
 
+@pytest.mark.parametrize("gs,natoms", [(2, 1), (4, 3), (6, 7), (10, 1), (16, 40), (40, 5)])
+def test_grid_build_edge_sizes(dplus, oracle, gs, natoms):
+    """The smallest grids (shells 1..4, where a pair unit's rows span the
+    equator and a workgroup spans more rows than the shared-row cache holds),
+    a single atom and a handful of atoms: every node of the mixed and FP64
+    builds against the oracle's direct sum."""
+    say(f"[case] grid build at grid size {gs} with {natoms} atoms")
+    xyz, ion, coef9, rad = structure(natoms, 20 + gs, radius=0.6)
+    ions = oracle_ions(oracle, coef9, rad, ion)
+    ref = oracle.grid_fill_pdb(gs, 3.0, xyz, ion, ions)
+    for prec, tol in ((0, 1e-10), (1, 2e-5)):
+        g = gpu_grid(dplus, gs, 3.0, prec, xyz, ion, coef9, rad)
+        assert relmax(g.data, ref) < tol
+
+
 def test_splines_of_uploaded_data(dplus, oracle):
     say("[case] splines of uploaded random data")
     gs = 40
