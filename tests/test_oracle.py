@@ -375,3 +375,25 @@ def test_docked_point_subunits_equal_debye():
     gk, _ = O.gk_oa(dock, q, 1e-7)
     deb = O.debye(t, [0, 0, 0], O.Ions(coef9, [0.13], [3], 0.0, 1.0, O.CALC_ATOMIC_FORMFACTORS), q)
     assert np.allclose(gk, deb, rtol=2e-5)
+
+
+def test_rows_to_docking_groups_equal_rotations():
+    """Manual-symmetry rows sharing their angles become one rotation with several
+    translations; the docked average is the same as with one rotation per row."""
+    import dplus_mi355x.elements as E
+    import dplus_mi355x.synth as S
+    rows = np.array([[0.5, 0, 0, 10, 20, 30], [0, 0.4, 0, 0, 0, 45], [-0.3, 0.1, 0.2, 10, 20, 30],
+                     [0.1, -0.5, 0.3, 0, 0, 45], [0.2, 0.2, -0.4, 10, 20, 30]])
+    rot, nt, tr = S.rows_to_docking(rows)
+    assert nt.tolist() == [3, 2] and np.allclose(np.rad2deg(rot[:, :3]), [[10, 20, 30], [0, 0, 45]])
+    assert np.array_equal(tr, rows[[0, 2, 4, 1, 3], :3])
+    coef9 = np.array([E.XRAY["O"]])
+    ions = O.Ions(coef9, [0.13], [1], 0.0, 1.0, O.CALC_ATOMIC_FORMFACTORS)
+    gs, qmax = 30, 3.0
+    sub = O.grid_fill_pdb(gs, qmax, [[0.1, 0, 0], [0, 0.2, 0]], [0, 0], ions)
+    co = O.grid_splines(gs, sub)
+    q = np.linspace(0.1, 3.0, 12)
+    flat_rot = np.concatenate([np.deg2rad(rows[:, 3:6]), np.ones((5, 1))], axis=1)
+    grouped, _ = O.gk_oa(O.Docked([(gs, qmax, sub, co, rot, nt, tr)]), q, 1e-6)
+    single, _ = O.gk_oa(O.Docked([(gs, qmax, sub, co, flat_rot, [1] * 5, rows[:, :3])]), q, 1e-6)
+    assert np.allclose(grouped, single, rtol=1e-9)
