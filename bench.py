@@ -99,7 +99,8 @@ def cpu_baseline(W, g64_data, g64_interp, budget_nodes=1500):
                       f"(scaled x{W['iters'] * W['ppb']}); full curve estimate {total:.0f} s"}
 
 
-def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, streams=8, fp64=True):
+def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, streams=8, fp64=True,
+                   batch_builds=False):
     """Batched serving stream: n_requests with Poisson arrivals at `rate`/s; each
     request is one of `pool_size` seeded random-walk structures (atoms
     log-uniform in 500..200,000 at 50 atoms/nm^3, dummy-atom solvent), qmax
@@ -107,33 +108,13 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, stre
     diameter, 200..1,000 q points, MC (65,536 directions) or VEGAS (5 x 4,096
     points) 50/50.  The mixed engine (`streams` streams, 200 GB HBM budget)
     serves the stream; the plain FP64 path then runs the same requests one at a
-    time (fp64=False skips it)."""
+    time (fp64=False skips it).  batch_builds: small mixed builds share grouped
+    builder launches."""
     from dplus_mi355x.serving import Engine
-    rng = np.random.default_rng(seed)
     t0 = time.time()
-    pool = []
-    for k in range(pool_size):
-        n = int(round(np.exp(rng.uniform(np.log(500), np.log(200000)))))
-        R = (3.0 * n / (4.0 * np.pi * 50.0)) ** (1.0 / 3.0)
-        xyz, el = S.walk_structure(5000 + k, n, (R, R, R))
-        coef9, rad, ion, _ = S.structure_tables(el)
-        L = 2.0 * float(np.max(np.linalg.norm(xyz, axis=1)))
-        pool.append((dict(xyz=xyz, ion=ion, coef9=coef9, rad=rad, sol_ed=333.0, c1=1.0, flags=3, scale=1.0), L, n))
-    reqs = []
-    for i in range(n_requests):
-        pdb, L, n = pool[int(rng.integers(pool_size))]
-        qmax = float(rng.uniform(2.0, 8.0))
-        gs = int(np.ceil((2.0 * qmax * L / np.pi + 3.0) / 10.0) * 10)
-        q = np.linspace(0.0, qmax, int(rng.integers(200, 1001)))
-        kw = dict(pdb=pdb, grid_size=gs, qmax=qmax, seed=1000 + i)
-        if rng.random() < 0.5:
-            kw.update(method=D.OA_MC, max_dirs=1 << 16, chunk=8192)
-        else:
-            kw.update(method=D.OA_VEGAS, vegas_iterations=5, vegas_points_per_bin=4)
-        reqs.append((q, kw, n))
+    pool, reqs, arrivals = S.serving_workload(n_requests, pool_size, seed, rate)
     log(f"[serving] {n_requests} requests over {pool_size} structures prepared in {time.time() - t0:.1f} s")
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
-    eng = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=streams)
+    eng = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=streams, batch_builds=batch_builds)
     tickets = []
     start = time.perf_counter()
     for (q, kw, _), at in zip(reqs, arrivals):
@@ -148,9 +129,11 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, stre
     lat = np.array([r["latency_ms"] for _, r in mixed])
     log(f"[serving] mixed: {n_requests / span:.2f} curves/s over {span:.2f} s, p50 {np.percentile(lat, 50):.1f} ms, "
         f"p99 {np.percentile(lat, 99):.1f} ms, max concurrent {st['max_concurrent']}")
+    batching = {"batch_builds": bool(batch_builds), "batch_launches": st["batch_launches"],
+                "batched_builds": st["batched_builds"]}
     if not fp64:
         return {"value": n_requests / span, "p50_ms": float(np.percentile(lat, 50)),
-                "p99_ms": float(np.percentile(lat, 99)), "streams": streams}
+                "p99_ms": float(np.percentile(lat, 99)), "streams": streams, **batching}
     eng64 = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=1)
     t64 = time.perf_counter()
     errs, lat64 = [], []
@@ -169,7 +152,7 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, stre
             "fp64_p50_ms": float(np.percentile(lat64, 50)), "fp64_p99_ms": float(np.percentile(lat64, 99)),
             "speedup_vs_fp64": (n_requests / span) / (n_requests / span64), "max_rel_err_vs_fp64": max(errs),
             "streams": streams, "max_concurrent": st["max_concurrent"], "peak_reserved_bytes": st["peak_reserved"],
-            "hbm_budget_bytes": st["budget"], "atoms_total": int(sum(n for _, _, n in reqs))}
+            "hbm_budget_bytes": st["budget"], "atoms_total": int(sum(n for _, _, n in reqs)), **batching}
 
 
 def fit_sessions(D, S, n_sessions=16, evals=4, directions=8192, nq=600, qmax=5.0, seed=5):
@@ -329,6 +312,10 @@ def main():
         return
     g64 = D.Grid(W["gs"], W["qmax"], D.PRECISION_FP64)
 
+    # builder launch shape: from the shape table the package loads (tuned on MI355X by
+    # python -m dplus_mi355x.tune), else the occupancy model
+    shape = st.build_shape(g32)
+    log(f"[bench] builder launch shape {shape}")
     mb = D.microbench_valu()
     w_t = 2.0 * mb["fma32"] / mb["sin32"]  # flop-equivalents of one v_sin/v_cos
     log(f"[bench] VALU microbenchmark ops/s: {mb}; transcendental weight {w_t:.2f} flop-eq")
@@ -370,7 +357,7 @@ def main():
     terms = float(nhalf) * sources
     # executed per atom and phi / phi + pi node pair (two terms): q_x x + q_y y (3 flops), its sin/cos
     # and four FMAs (8 flops); sin/cos of q_z z come from LDS, evaluated once per (row, atom) per
-    # workgroup of 256 units spanning a few rows (under 2 % of the pair work, not counted)
+    # workgroup, whose units span a few rows (under 2 % of the pair work, not counted)
     flop_eq = 11.0 / 2.0 + w_t
     t_launch = ms_build / max(nb, 1) * 1e-3
     achieved = terms * flop_eq / t_launch / 1e12
@@ -410,7 +397,7 @@ def main():
         "config": {"workload": "50k-atom SAW ellipsoid 8/6/5 nm + 0.3 nm hydration layer; grid 250, qmax 7.5, "
                                "1000 q; VEGAS 10 x 100352; mixed precision",
                    "atoms": len(W["xyz"]), "hydration_boxes": len(W["dims"]), "grid_size": W["gs"],
-                   "q_points": len(W["q"]), "oa": "VEGAS 10x(1024x98)"},
+                   "q_points": len(W["q"]), "oa": "VEGAS 10x(1024x98)", "build_shape": shape},
         "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
         "fp64": {"value": fp64_value, "unit": "curves/s", "p50_ms": 1e3 * float(np.percentile(lat64, 50)),
                  "p99_ms": 1e3 * float(np.percentile(lat64, 99)), "steps": args.fp64_steps},
@@ -418,7 +405,7 @@ def main():
         "max_rel_err_vs_fp64": err,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "build_atoms_pairs", "launch_ms": t_launch * 1e3,
+                     "kernel": f"build_atoms_pairs<false, {shape['wg']}>", "launch_ms": t_launch * 1e3,
                      "terms_per_launch": terms, "flop_eq_per_term": flop_eq,
                      "transcendental_weight": w_t},
         "cpu_baseline": cpu,

@@ -23,11 +23,16 @@
 //    transcendental per term.  Each tile is summed in FP32 and folded into FP64
 //    accumulators with the ion's form factor at the node's shell (or the box
 //    shape's weight): a two-level compensated sum whose FP32 partials never
-//    exceed kTile terms.
+//    exceed kTile terms.  One launch builds up to kMaxBuildJobs grids, with a
+//    workgroup size (64..512 units) and atom-slice count chosen per grid
+//    class by the host's autotuner or occupancy model.
 //  * FP64 mode (build_atoms_fp64): the plain form -- double coordinates,
 //    phases and sincos, one node at a time, K nodes per thread.
 
-#include "common.hpp"
+#include <climits>
+#include <mutex>
+
+#include "device.hpp"
 
 namespace dplus {
 
@@ -189,27 +194,24 @@ __global__ __launch_bounds__(256) void build_atoms_phase64(
 // One unit (thread) owns three such pairs; per atom it takes sin/cos of u once
 // per pair and sums the four products cz cu, sz su, sz cu, cz su: one
 // transcendental per term.  cz, sz depend only on the row and the atom, so a
-// workgroup (whose 256 units span at most kRowsLds rows for all but the
-// smallest shells) evaluates them once per (row, atom) into LDS, chunk by
-// chunk, double-buffered with one barrier per chunk; a workgroup spanning more
-// rows takes them per unit.  Box weights are even in (q_x, q_y), so one per
-// pair.  Rows off the equator also write both antipodes; on the equator
-// (j = 3i/2) the partner is the antipode itself.
+// workgroup (whose kWg units span at most kRows rows for all but the smallest
+// shells) evaluates them once per (row, atom) into LDS, chunk by chunk,
+// double-buffered with one barrier per chunk; a workgroup spanning more rows
+// takes them per unit.  Box weights are even in (q_x, q_y), so one per pair.
+// Rows off the equator also write both antipodes; on the equator (j = 3i/2)
+// the partner is the antipode itself.
 // Every workgroup sums all atoms of its slice, so the atoms are cut into
-// slices (tile ranges, blockIdx.y) to fill the device's last round of
-// resident workgroups; slices end in FP64 partial sums added in order.
+// slices (tile ranges) to fill the device's last round of resident
+// workgroups; slices end in FP64 partial sums added in order.
+// One launch serves up to kMaxBuildJobs grids (of different structures, grid
+// sizes and slice counts): workgroups are numbered job by job, slice by slice,
+// and each finds its job from the launch arguments.
 // With kReduce false the phases (in revolutions) go to v_sin_f32 / v_cos_f32
 // unreduced; the host selects that form only when every |q.r| / 2 pi of the
 // structure is below 128 revolutions (|z| and |u| are both bounded by |q||r|),
 // inside the units' input domain.
 constexpr int kPair = 3;
-constexpr int kRowsLds = 16, kChunk = 64;
-constexpr int kMaxSlices = 8;
-
-// Tile range of each atom slice: slice y sums tiles [begin[y], begin[y + 1]).
-struct SliceTab {
-    int begin[kMaxSlices + 1];
-};
+constexpr int kChunk = 64;
 
 // pair unit u -> its row and first azimuthal index
 struct PairUnit {
@@ -241,17 +243,38 @@ __device__ __forceinline__ void store_pair(float4 *__restrict__ grid, const Pair
     }
 }
 
-// Workgroup (x, y) sums the tiles of atom slice y for units 256 x .. 256 x + 255
-// and leaves its FP64 sums, unscaled, in partial[(y * 12 + 4 p + m) * nunits + u]
-// (m = aR, aI, bR, bI); pairs_finish adds the slices in order and stores.
-template <bool kReduce>
-__global__ __launch_bounds__(256) void build_atoms_pairs(
-    const float4 *__restrict__ atoms, const int4 *__restrict__ tiles, SliceTab slices,
-    const double *__restrict__ ff, int nion, const int4 *__restrict__ bdims, double vox, double contrast,
-    double step, long long nunits, double *__restrict__ partial) {
-    __shared__ float2 rowz[2][kRowsLds][kChunk];
-    __shared__ float rowqz[kRowsLds];
-    const long long u0 = (long long)blockIdx.x * blockDim.x;
+// job of workgroup `wg` (jobs are numbered in order of their first workgroup)
+__device__ __forceinline__ int batch_job(const BuildBatch &b, int wg, bool finishing) {
+    int jb = 0;
+    for (int k = 1; k < b.njobs; k++)
+        if (wg >= (finishing ? b.job[k].fin_begin : b.job[k].wg_begin)) jb = k;
+    return jb;
+}
+
+// LDS rows of a workgroup of kWg units (kWg units of shell i span at most
+// ceil(kWg / i) + 1 rows, so these cover every shell from i = 10 (64 units),
+// 19 (128), 18 (256) or 17 (512) up)
+template <int kWg>
+constexpr int pair_rows() {
+    return kWg <= 128 ? 8 : kWg == 256 ? 16 : 32;
+}
+
+// Workgroup w of job J, slice y = (w - J.wg_begin) / J.xblocks, sums the tiles
+// of slice y for units kWg x .. kWg x + kWg - 1 (x the rest) and leaves its FP64
+// sums, unscaled, in J.partial[(y * 12 + 4 p + m) * nunits + u] (m = aR, aI,
+// bR, bI); pairs_finish adds the slices in order and stores.
+template <bool kReduce, int kWg>
+__global__ __launch_bounds__(kWg) void build_atoms_pairs(BuildBatch batch) {
+    constexpr int kRows = pair_rows<kWg>();
+    __shared__ float2 rowz[2][kRows][kChunk];
+    __shared__ float rowqz[kRows];
+    const BuildJob &J = batch.job[batch_job(batch, (int)blockIdx.x, false)];
+    const float4 *__restrict__ atoms = J.atoms;
+    const int4 *__restrict__ tiles = J.tiles;
+    const long long nunits = J.nunits;
+    const int local = (int)blockIdx.x - J.wg_begin;
+    const int slice = local / J.xblocks;
+    const long long u0 = (long long)(local - slice * J.xblocks) * kWg;
     const long long u = u0 + threadIdx.x;
     const bool live = u < nunits;
     const long long uc = live ? u : nunits - 1;
@@ -263,18 +286,18 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
     {
         const int s0 = shell_of_pair_unit(u0);
         const long long g0 = rows_before(s0 - 1) + (u0 - pair_units_before(s0 - 1)) / s0;
-        const long long ul = min(u0 + (long long)blockDim.x, nunits) - 1;
+        const long long ul = min(u0 + (long long)kWg, nunits) - 1;
         const int s1 = shell_of_pair_unit(ul);
         const long long g1 = rows_before(s1 - 1) + (ul - pair_units_before(s1 - 1)) / s1;
         lrow = (int)(rows_before(shell - 1) + j - g0);
         nrows = (int)(g1 - g0 + 1);
     }
-    const bool shared_z = nrows <= kRowsLds;  // uniform over the workgroup
+    const bool shared_z = nrows <= kRows;  // uniform over the workgroup
     float qx[kPair], qy[kPair], qz;
     double aR[kPair], aI[kPair], bR[kPair], bI[kPair];
     const double inv2pi = 1.0 / k2Pi;
     {
-        const double q = (double)shell * step;
+        const double q = (double)shell * J.step;
         double st, ct;
         sincos(kPi * (double)j / (double)(3 * shell), &st, &ct);
         qz = (float)(q * ct * inv2pi);
@@ -293,11 +316,11 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
         __syncthreads();
     }
     int buf = 0;
-    const int t_end = slices.begin[blockIdx.y + 1];
-    for (int t = slices.begin[blockIdx.y]; t < t_end; t++) {
+    const int t_end = J.slice_begin[slice + 1];
+    for (int t = J.slice_begin[slice]; t < t_end; t++) {
         const int4 tl = tiles[t];
         // the ion's form factor at this shell, loaded now and used after the atom loop
-        const double fion = tl.w == 0 ? ff[(size_t)shell * nion + tl.z] : 0.0;
+        const double fion = tl.w == 0 ? J.ff[(size_t)shell * J.nion + tl.z] : 0.0;
         float cc[kPair], ss[kPair], sc[kPair], cs[kPair];
 #pragma unroll
         for (int p = 0; p < kPair; p++) cc[p] = ss[p] = sc[p] = cs[p] = 0.f;
@@ -306,7 +329,7 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
             for (int c0 = tl.x; c0 < a1; c0 += kChunk) {
                 const int n = min(kChunk, a1 - c0);
                 // cos / sin of q_z z for every (row, atom) of the chunk, once per workgroup
-                for (int e = threadIdx.x; e < nrows * kChunk; e += blockDim.x) {
+                for (int e = threadIdx.x; e < nrows * kChunk; e += kWg) {
                     const int rr = e / kChunk, c = e % kChunk;
                     if (c < n) {
                         const float zz = rowqz[rr] * atoms[c0 + c].z;
@@ -352,17 +375,18 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
         }
         // box tiles: contrast * prod_a L_a sinc(q_a L_a / 2); the z edge is common to the row
         const float k2 = (float)k2Pi;
+        const float vox = (float)J.vox;
         int4 d = make_int4(0, 0, 0, 0);
         float wz = 0.f;
         if (tl.w != 0) {
-            d = bdims[tl.z];
-            wz = edge_weight(qz * k2, d.z, (float)vox);
+            d = J.bdims[tl.z];
+            wz = edge_weight(qz * k2, d.z, vox);
         }
 #pragma unroll
         for (int p = 0; p < kPair; p++) {
             const double f = tl.w == 0 ? fion
-                                       : contrast * (double)(wz * edge_weight(qx[p] * k2, d.x, (float)vox) *
-                                                             edge_weight(qy[p] * k2, d.y, (float)vox));
+                                       : J.contrast * (double)(wz * edge_weight(qx[p] * k2, d.x, vox) *
+                                                               edge_weight(qy[p] * k2, d.y, vox));
             aR[p] = fma(f, (double)(cc[p] - ss[p]), aR[p]);
             aI[p] = fma(f, (double)(sc[p] + cs[p]), aI[p]);
             bR[p] = fma(f, (double)(cc[p] + ss[p]), bR[p]);
@@ -370,7 +394,7 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
         }
     }
     if (!live) return;
-    double *dst = partial + (size_t)blockIdx.y * 12 * nunits + u;
+    double *dst = J.partial + (size_t)slice * 12 * nunits + u;
 #pragma unroll
     for (int p = 0; p < kPair; p++) {
         dst[(4 * p + 0) * nunits] = aR[p];
@@ -380,18 +404,21 @@ __global__ __launch_bounds__(256) void build_atoms_pairs(
     }
 }
 
-// Sum of the atom slices of every pair unit, in slice order, scaled and stored.
-__global__ __launch_bounds__(256) void pairs_finish(const double *__restrict__ partial, int nslices,
-                                                    long long nunits, float4 *__restrict__ grid, double scale) {
-    const long long u = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// Sum of the atom slices of every pair unit of every job, in slice order,
+// scaled and stored; workgroup w serves units 256 (w - J.fin_begin) + .. of job J.
+__global__ __launch_bounds__(256) void pairs_finish(BuildBatch batch) {
+    const BuildJob &J = batch.job[batch_job(batch, (int)blockIdx.x, true)];
+    const long long nunits = J.nunits;
+    const long long u = (long long)((int)blockIdx.x - J.fin_begin) * 256 + threadIdx.x;
     if (u >= nunits) return;
     const PairUnit pu = pair_unit(u);
+    const double *__restrict__ partial = J.partial;
     for (int p = 0; p < kPair; p++) {
         double v[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int y = 0; y < nslices; y++)
+        for (int y = 0; y < J.nslices; y++)
 #pragma unroll
             for (int m = 0; m < 4; m++) v[m] += partial[((size_t)y * 12 + 4 * p + m) * nunits + u];
-        store_pair(grid, pu, p, v[0], v[1], v[2], v[3], scale);
+        store_pair(J.grid, pu, p, v[0], v[1], v[2], v[3], J.scale);
     }
 }
 
@@ -510,22 +537,37 @@ long long pair_units(long long nhalf) {
     return pair_units_before(shell_of_half(nhalf) - 1);
 }
 
-// Number of atom slices of the paired builder.  Every workgroup sums all atoms
-// of its slice, so a launch runs in whole rounds of the device's resident
-// workgroups; S slices turn W workgroups into W S shorter ones.  S minimises
-// ceil(W S / resident) / S, with 1 % per extra slice for its partial sums.
-int pair_slices(long long nunits, int ntiles) {
-    static const int resident = [] {
+// Resident workgroups of the paired builder of workgroup size wg on this device.
+static int pair_resident(int wg) {
+    static int cached[4] = {-1, -1, -1, -1};
+    static std::mutex mu;
+    const int slot = wg == 64 ? 0 : wg == 128 ? 1 : wg == 256 ? 2 : 3;
+    std::lock_guard<std::mutex> lk(mu);
+    if (cached[slot] < 0) {
+        const void *fn = wg == 64    ? (const void *)build_atoms_pairs<false, 64>
+                         : wg == 128 ? (const void *)build_atoms_pairs<false, 128>
+                         : wg == 256 ? (const void *)build_atoms_pairs<false, 256>
+                                     : (const void *)build_atoms_pairs<false, 512>;
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, build_atoms_pairs<false>, 256, 0) !=
-                hipSuccess)
-            return 0;
-        return cus * per_cu;
-    }();
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, wg, 0) != hipSuccess)
+            cached[slot] = 0;
+        else
+            cached[slot] = cus * per_cu;
+    }
+    return cached[slot];
+}
+
+// Number of atom slices of the paired builder (occupancy model).  Every
+// workgroup sums all atoms of its slice, so a launch runs in whole rounds of
+// the device's resident workgroups; S slices turn W workgroups into W S
+// shorter ones.  S minimises ceil(W S / resident) / S, with 1 % per extra
+// slice for its partial sums.
+int pair_slices(long long nunits, int ntiles, int wg) {
+    const int resident = pair_resident(wg);
     if (resident <= 0) return 1;
-    const long long W = blocks_for(nunits, 256);
+    const long long W = blocks_for(nunits, wg);
     int best = 1;
     double best_cost = 1e300;
     for (int S = 1; S <= std::min(kMaxSlices, std::max(ntiles, 1)); S++) {
@@ -538,19 +580,48 @@ int pair_slices(long long nunits, int ntiles) {
     return best;
 }
 
-void launch_build_pairs(bool reduce, const float4 *atoms, const int4 *tiles, const int *slice_begin, int nslices,
-                        const double *ff, int nion, const int4 *bdims, double vox, double contrast, double step,
-                        long long nunits, float4 *grid, double scale, double *partial, hipStream_t s) {
-    SliceTab tab{};
-    for (int y = 0; y <= nslices; y++) tab.begin[y] = slice_begin[y];
-    const dim3 grid_dim(blocks_for(nunits, 256), nslices);
+template <bool kReduce>
+static void launch_pairs(int wg, const BuildBatch &b, unsigned nblocks, hipStream_t s) {
+    switch (wg) {
+        case 64:
+            build_atoms_pairs<kReduce, 64><<<nblocks, 64, 0, s>>>(b);
+            break;
+        case 128:
+            build_atoms_pairs<kReduce, 128><<<nblocks, 128, 0, s>>>(b);
+            break;
+        case 256:
+            build_atoms_pairs<kReduce, 256><<<nblocks, 256, 0, s>>>(b);
+            break;
+        default:
+            build_atoms_pairs<kReduce, 512><<<nblocks, 512, 0, s>>>(b);
+            break;
+    }
+}
+
+void launch_build_batch(bool reduce, int wg, BuildJob *jobs, int njobs, hipStream_t s) {
+    if (njobs < 1 || njobs > kMaxBuildJobs) throw Error(kErrInvalidArgs, "builder batch of 1..16 jobs");
+    if (wg != 64 && wg != 128 && wg != 256 && wg != 512)
+        throw Error(kErrInvalidArgs, "builder workgroup size must be 64, 128, 256 or 512");
+    BuildBatch b{};
+    b.njobs = njobs;
+    long long wgs = 0, fins = 0;
+    for (int k = 0; k < njobs; k++) {
+        BuildJob &J = jobs[k];
+        if (J.nslices < 1 || J.nslices > kMaxSlices || J.nunits < 1)
+            throw Error(kErrInvalidArgs, "builder job with no units or a slice count outside 1..8");
+        J.xblocks = (int)blocks_for(J.nunits, wg);
+        J.wg_begin = (int)wgs;
+        J.fin_begin = (int)fins;
+        wgs += (long long)J.xblocks * J.nslices;
+        fins += blocks_for(J.nunits, 256);
+        b.job[k] = J;
+    }
+    if (wgs > INT_MAX || fins > INT_MAX) throw Error(kErrInvalidArgs, "builder batch too large for one launch");
     if (reduce)
-        build_atoms_pairs<true><<<grid_dim, 256, 0, s>>>(atoms, tiles, tab, ff, nion, bdims, vox, contrast, step,
-                                                        nunits, partial);
+        launch_pairs<true>(wg, b, (unsigned)wgs, s);
     else
-        build_atoms_pairs<false><<<grid_dim, 256, 0, s>>>(atoms, tiles, tab, ff, nion, bdims, vox, contrast, step,
-                                                         nunits, partial);
-    pairs_finish<<<blocks_for(nunits, 256), 256, 0, s>>>(partial, nslices, nunits, grid, scale);
+        launch_pairs<false>(wg, b, (unsigned)wgs, s);
+    pairs_finish<<<(unsigned)fins, 256, 0, s>>>(b);
 }
 
 void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s) {

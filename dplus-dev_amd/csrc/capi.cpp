@@ -272,9 +272,6 @@ void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms,
                         const int4 *tiles, int ntiles, const double *ff, int nion, const int4 *bdims, double vox,
                         double contrast, double step, long long nhalf, void *grid, double scale, hipStream_t s);
 void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s);
-void launch_build_pairs(bool reduce, const float4 *atoms, const int4 *tiles, const int *slice_begin, int nslices,
-                        const double *ff, int nion, const int4 *bdims, double vox, double contrast, double step,
-                        long long nunits, float4 *grid, double scale, double *partial, hipStream_t s);
 
 // Atomic Form Factor.cpp:11-27 (cube-mean radius), 91-115 (dummy-atom
 // solvent), 160-176 (4 Gaussians + c); evaluated in double per shell radius
@@ -410,57 +407,190 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
     check_hip(hipStreamSynchronize(s), "structure upload");
 }
 
-void Structure::build(Grid &g, bool host_copy, hipStream_t s) {
+// ------------------------------------------------------------------ builder shapes
+// Shape table of the paired builder's autotuner: the fastest measured launch
+// shape per (grid shells, tile-count class); the class is the tile count's
+// half-octave, so structures of similar size share an entry.
+namespace {
+std::mutex shape_mu;
+std::map<std::pair<int, int>, BuildShape> shape_table;
+
+int tile_class(int ntiles) { return (int)std::floor(2.0 * std::log2((double)std::max(ntiles, 1))); }
+}  // namespace
+
+bool shape_lookup(int nshells, int ntiles, BuildShape *out) {
+    std::lock_guard<std::mutex> lk(shape_mu);
+    auto it = shape_table.find({nshells, tile_class(ntiles)});
+    if (it == shape_table.end()) return false;
+    *out = it->second;
+    return true;
+}
+
+void shape_record(int nshells, int ntiles, BuildShape shape) {
+    std::lock_guard<std::mutex> lk(shape_mu);
+    shape_table[{nshells, tile_class(ntiles)}] = shape;
+}
+
+BuildShape Structure::shape_for(const Grid &g) const {
+    BuildShape sh;
+    if (shape_lookup(g.N, ntiles, &sh)) return sh;
+    sh.wg = 256;
+    sh.slices = pair_slices(pair_units(half_base(g.N + 1)), ntiles, sh.wg);
+    return sh;
+}
+
+BuildJob Structure::prepare(Grid &g, int slices, Pending &P, hipStream_t s) const {
+    if (g.fp64()) throw Error(kErrInvalidArgs, "the paired builder serves mixed-precision grids");
     std::vector<double> ff;
     form_factor_table(&desc, ion_counts, g.N, g.step, ff);
     // origin: sum of f~(0) over atoms + contrast * total box volume
-    double origin = 0.0;
+    P.origin = 0.0;
     if (atoms_on)
-        for (int i = 0; i < desc.nion; i++) origin += ion_counts[i] * ff[i];
-    if (boxes_on) origin += desc.box_contrast * box_volume;
-    // per-build buffers (form factors of this grid's shells, slice partials): a
-    // structure can be built into several grids at once on different streams
-    DevBuf bff, bpart;
-    bff.upload(ff.data(), ff.size() * sizeof(double), s);
-    const long long nhalf = half_base(g.N + 1);
-    const bool phase64 = getenv("DPLUS_PHASE64") != nullptr;
-    {
-        KTimer kt("build", s);
-        // unreduced hardware sin/cos while every phase stays below 128 revolutions
-        const double qtop = (double)(g.N + 1) * g.step;
-        const bool reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
-        if (!g.fp64() && !phase64) {
-            // atom slices of roughly equal work (a tile costs its terms + 8 for its fold)
-            const long long nunits = pair_units(nhalf);
-            const int S = pair_slices(nunits, ntiles);
-            std::vector<int> begin(S + 1, ntiles);
-            begin[0] = 0;
-            double total = 0.0;
-            for (int t = 0; t < ntiles; t++) total += tile_len[t] + 8.0;
-            double acc = 0.0;
-            for (int t = 0, y = 1; t < ntiles && y < S; t++) {
-                acc += tile_len[t] + 8.0;
-                while (y < S && acc >= total * y / S) begin[y++] = t + 1;
-            }
-            bpart.ensure((size_t)S * 12 * nunits * sizeof(double));
-            launch_build_pairs(reduce, (const float4 *)b32.p, (const int4 *)bt.p, begin.data(), S,
-                               (const double *)bff.p, desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast,
-                               g.step, nunits, (float4 *)g.dev.p, desc.scale, (double *)bpart.p, s);
-        } else {
-            launch_build_atoms(g.fp64(), phase64, reduce, b32.p, b64.p, (const int4 *)bt.p, ntiles,
-                               (const double *)bff.p, desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast,
-                               g.step, nhalf, g.dev.p, desc.scale, s);
-        }
-        check_hip(hipGetLastError(), "build_atoms launch");
+        for (int i = 0; i < desc.nion; i++) P.origin += ion_counts[i] * ff[i];
+    if (boxes_on) P.origin += desc.box_contrast * box_volume;
+    // unreduced hardware sin/cos while every phase stays below 128 revolutions
+    const double qtop = (double)(g.N + 1) * g.step;
+    P.reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
+    P.ff.upload(ff.data(), ff.size() * sizeof(double), s);
+    const long long nunits = pair_units(half_base(g.N + 1));
+    const int S = std::max(1, std::min({slices, kMaxSlices, std::max(ntiles, 1)}));
+    P.part.ensure((size_t)S * 12 * nunits * sizeof(double));
+    BuildJob J{};
+    J.atoms = (const float4 *)b32.p;
+    J.tiles = (const int4 *)bt.p;
+    J.ff = (const double *)P.ff.p;
+    J.bdims = (const int4 *)bd.p;
+    J.partial = (double *)P.part.p;
+    J.grid = (float4 *)g.dev.p;
+    J.nunits = nunits;
+    J.vox = desc.vox_step;
+    J.contrast = desc.box_contrast;
+    J.step = g.step;
+    J.scale = desc.scale;
+    J.nion = desc.nion;
+    J.nslices = S;
+    // atom slices of roughly equal work (a tile costs its terms + 8 for its fold)
+    for (int y = 0; y <= S; y++) J.slice_begin[y] = ntiles;
+    J.slice_begin[0] = 0;
+    double total = 0.0;
+    for (int t = 0; t < ntiles; t++) total += tile_len[t] + 8.0;
+    double acc = 0.0;
+    for (int t = 0, y = 1; t < ntiles && y < S; t++) {
+        acc += tile_len[t] + 8.0;
+        while (y < S && acc >= total * y / S) J.slice_begin[y++] = t + 1;
     }
-    launch_set_origin(g.fp64(), g.dev.p, origin * desc.scale, s);
+    return J;
+}
+
+void Structure::finish(Grid &g, const Pending &P, bool host_copy, hipStream_t s) const {
+    launch_set_origin(g.fp64(), g.dev.p, P.origin * desc.scale, s);
     g.splines(s);
     // the per-build buffers are released after the stream has drained
     if (host_copy)
         g.download(s);
     else
         check_hip(hipStreamSynchronize(s), "pdb build");
+}
+
+void Structure::build(Grid &g, bool host_copy, hipStream_t s) {
+    const bool phase64 = getenv("DPLUS_PHASE64") != nullptr;
+    if (!g.fp64() && !phase64) {
+        // per-build buffers (form factors of this grid's shells, slice partials): a
+        // structure can be built into several grids at once on different streams
+        Pending P;
+        const BuildShape sh = shape_for(g);
+        BuildJob J = prepare(g, sh.slices, P, s);
+        {
+            KTimer kt("build", s);
+            launch_build_batch(P.reduce, sh.wg, &J, 1, s);
+            check_hip(hipGetLastError(), "build_atoms launch");
+        }
+        finish(g, P, host_copy, s);
+        timing_collect();
+        return;
+    }
+    std::vector<double> ff;
+    form_factor_table(&desc, ion_counts, g.N, g.step, ff);
+    double origin = 0.0;
+    if (atoms_on)
+        for (int i = 0; i < desc.nion; i++) origin += ion_counts[i] * ff[i];
+    if (boxes_on) origin += desc.box_contrast * box_volume;
+    DevBuf bff;
+    bff.upload(ff.data(), ff.size() * sizeof(double), s);
+    const long long nhalf = half_base(g.N + 1);
+    {
+        KTimer kt("build", s);
+        const double qtop = (double)(g.N + 1) * g.step;
+        const bool reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
+        launch_build_atoms(g.fp64(), phase64, reduce, b32.p, b64.p, (const int4 *)bt.p, ntiles,
+                           (const double *)bff.p, desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast,
+                           g.step, nhalf, g.dev.p, desc.scale, s);
+        check_hip(hipGetLastError(), "build_atoms launch");
+    }
+    launch_set_origin(g.fp64(), g.dev.p, origin * desc.scale, s);
+    g.splines(s);
+    if (host_copy)
+        g.download(s);
+    else
+        check_hip(hipStreamSynchronize(s), "pdb build");
     timing_collect();
+}
+
+// Autotuner of the paired builder: every workgroup size (64..512 units) with
+// every slice count 1..min(8, tiles) builds grid g once untimed per workgroup
+// size and once timed (HIP events on s); the fastest is recorded in the shape
+// table and builds the grid a last time, so it holds the bits every later
+// build of its class gives (the workgroup size changes no sum; the slice
+// count regroups the FP64 partial sums).
+BuildShape Structure::autotune(Grid &g, hipStream_t s, double *best_ms, double *model_ms) {
+    if (g.fp64()) throw Error(kErrInvalidArgs, "the autotuner serves the mixed-precision builder");
+    const long long nunits = pair_units(half_base(g.N + 1));
+    BuildShape model;
+    model.wg = 256;
+    model.slices = pair_slices(nunits, ntiles, 256);
+    hipEvent_t e0, e1;
+    check_hip(hipEventCreate(&e0), "hipEventCreate");
+    check_hip(hipEventCreate(&e1), "hipEventCreate");
+    BuildShape best = model;
+    double tbest = 1e300, tmodel = 0.0;
+    try {
+        Pending P;
+        const int smax = std::min(kMaxSlices, std::max(ntiles, 1));
+        for (int wg : {64, 128, 256, 512}) {
+            for (int S = 0; S <= smax; S++) {
+                // S = 0: the untimed warm-up of this workgroup size
+                BuildJob J = prepare(g, std::max(S, 1), P, s);
+                check_hip(hipEventRecord(e0, s), "hipEventRecord");
+                launch_build_batch(P.reduce, wg, &J, 1, s);
+                check_hip(hipGetLastError(), "build_atoms launch");
+                check_hip(hipEventRecord(e1, s), "hipEventRecord");
+                check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+                if (S == 0) continue;
+                float ms = 0.f;
+                check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                if (wg == model.wg && S == model.slices) tmodel = ms;
+                if (ms < tbest) {
+                    tbest = ms;
+                    best.wg = wg;
+                    best.slices = S;
+                }
+            }
+        }
+        BuildJob J = prepare(g, best.slices, P, s);
+        launch_build_batch(P.reduce, best.wg, &J, 1, s);
+        check_hip(hipGetLastError(), "build_atoms launch");
+        finish(g, P, false, s);
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    shape_record(g.N, ntiles, best);
+    if (best_ms) *best_ms = tbest;
+    if (model_ms) *model_ms = tmodel;
+    return best;
 }
 
 void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s) {
@@ -1099,6 +1229,134 @@ int dplus_structure_build_grid(dplus_structure *sh, dplus_grid *gh, int host_cop
         if (!st || !g) throw Error(kErrInvalidArgs, "null argument");
         Device::get().require();
         st->build(*g, host_copy != 0, Device::get().stream);
+    })
+}
+
+int dplus_structure_build_grids(int n, dplus_structure *const *sh, dplus_grid *const *gh, int host_copy) {
+    API_GUARD({
+        if (n < 0 || (n > 0 && (!sh || !gh))) throw Error(kErrInvalidArgs, "null argument");
+        for (int i = 0; i < n; i++)
+            if (!sh[i] || !gh[i]) throw Error(kErrInvalidArgs, "null structure or grid");
+        Device::get().require();
+        const hipStream_t s = Device::get().stream;
+        // mixed grids in grouped launches of up to 16 per phase form; FP64 grids one by one
+        std::vector<int> mixed;
+        for (int i = 0; i < n; i++) {
+            Grid *g = (Grid *)gh[i];
+            if (g->fp64() || getenv("DPLUS_PHASE64"))
+                ((Structure *)sh[i])->build(*g, host_copy != 0, s);
+            else
+                mixed.push_back(i);
+        }
+        std::vector<Structure::Pending> P(mixed.size());
+        std::vector<BuildJob> jobs(mixed.size());
+        for (size_t k = 0; k < mixed.size(); k++) {
+            Structure *st = (Structure *)sh[mixed[k]];
+            Grid *g = (Grid *)gh[mixed[k]];
+            jobs[k] = st->prepare(*g, st->shape_for(*g).slices, P[k], s);
+        }
+        {
+            KTimer kt("build", s);
+            for (int red = 0; red < 2; red++) {
+                std::vector<BuildJob> batch;
+                for (size_t k = 0; k < mixed.size(); k++) {
+                    if ((int)P[k].reduce != red) continue;
+                    batch.push_back(jobs[k]);
+                    if ((int)batch.size() == kMaxBuildJobs) {
+                        launch_build_batch(red != 0, 256, batch.data(), (int)batch.size(), s);
+                        batch.clear();
+                    }
+                }
+                if (!batch.empty()) launch_build_batch(red != 0, 256, batch.data(), (int)batch.size(), s);
+            }
+            check_hip(hipGetLastError(), "grouped build launch");
+        }
+        for (size_t k = 0; k < mixed.size(); k++)
+            ((Structure *)sh[mixed[k]])->finish(*(Grid *)gh[mixed[k]], P[k], host_copy != 0, s);
+        timing_collect();
+    })
+}
+
+int dplus_structure_build_shape(dplus_structure *sh, dplus_grid *gh, int *wg, int *slices, int *tuned) {
+    API_GUARD({
+        Structure *st = (Structure *)sh;
+        Grid *g = (Grid *)gh;
+        if (!st || !g) throw Error(kErrInvalidArgs, "null argument");
+        BuildShape b;
+        const bool t = shape_lookup(g->N, st->ntiles, &b);
+        if (!t) b = st->shape_for(*g);
+        if (wg) *wg = b.wg;
+        if (slices) *slices = b.slices;
+        if (tuned) *tuned = t ? 1 : 0;
+    })
+}
+
+int dplus_build_shapes_clear(void) {
+    API_GUARD({
+        std::lock_guard<std::mutex> lk(shape_mu);
+        shape_table.clear();
+    })
+}
+
+int dplus_structure_autotune(dplus_structure *sh, dplus_grid *gh, int *wg, int *slices, double *best_ms,
+                             double *model_ms) {
+    API_GUARD({
+        Structure *st = (Structure *)sh;
+        Grid *g = (Grid *)gh;
+        if (!st || !g) throw Error(kErrInvalidArgs, "null argument");
+        Device::get().require();
+        double tb = 0.0, tm = 0.0;
+        const BuildShape b = st->autotune(*g, Device::get().stream, &tb, &tm);
+        if (wg) *wg = b.wg;
+        if (slices) *slices = b.slices;
+        if (best_ms) *best_ms = tb;
+        if (model_ms) *model_ms = tm;
+    })
+}
+
+int dplus_build_shapes_save(char **text, long long *len) {
+    API_GUARD({
+        if (!text || !len) throw Error(kErrInvalidArgs, "null argument");
+        std::string out;
+        {
+            std::lock_guard<std::mutex> lk(shape_mu);
+            for (auto &kv : shape_table)
+                out += std::to_string(kv.first.first) + " " + std::to_string(kv.first.second) + " " +
+                       std::to_string(kv.second.wg) + " " + std::to_string(kv.second.slices) + "\n";
+        }
+        *text = (char *)malloc(out.size() + 1);
+        if (!*text) throw Error(kErrOutOfMemory, "out of host memory");
+        memcpy(*text, out.data(), out.size());
+        (*text)[out.size()] = '\0';
+        *len = (long long)out.size();
+    })
+}
+
+int dplus_build_shapes_load(const char *text) {
+    API_GUARD({
+        if (!text) throw Error(kErrInvalidArgs, "null argument");
+        std::vector<std::pair<std::pair<int, int>, BuildShape>> rows;
+        const char *p = text;
+        while (*p) {
+            while (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t') p++;
+            if (!*p) break;
+            if (*p == '#') {  // comment line
+                while (*p && *p != '\n') p++;
+                continue;
+            }
+            int n = 0, c = 0, w = 0, sl = 0, used = 0;
+            if (sscanf(p, "%d %d %d %d%n", &n, &c, &w, &sl, &used) != 4)
+                throw Error(kErrInvalidArgs, "shape table line is not \"grid_shells tile_class wg slices\"");
+            if (n < 1 || (w != 64 && w != 128 && w != 256 && w != 512) || sl < 1 || sl > kMaxSlices)
+                throw Error(kErrInvalidArgs, "shape table entry out of range");
+            BuildShape b;
+            b.wg = w;
+            b.slices = sl;
+            rows.push_back({{n, c}, b});
+            p += used;
+        }
+        std::lock_guard<std::mutex> lk(shape_mu);
+        for (auto &r : rows) shape_table[r.first] = r.second;
     })
 }
 

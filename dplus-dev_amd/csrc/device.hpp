@@ -168,6 +168,37 @@ void microbench_valu(hipStream_t s, double *fma32, double *sin32, double *cos32,
 void copy_probe(hipStream_t s, long long bytes, double *ms);
 
 // atoms and solvent boxes of one PDB leaf, sorted, tiled and resident in HBM
+// One grid's share of a paired-builder launch (read by the kernels from the
+// launch arguments): its resident atoms/tiles, per-shell form factors, slice
+// partials and output grid, with the tile range of each atom slice.
+constexpr int kMaxSlices = 8;
+constexpr int kMaxBuildJobs = 16;
+struct BuildJob {
+    const float4 *atoms;
+    const int4 *tiles;
+    const double *ff;
+    const int4 *bdims;
+    double *partial;
+    float4 *grid;
+    long long nunits;
+    double vox, contrast, step, scale;
+    int nion, nslices;
+    int xblocks;    // workgroups per slice (set by the launcher)
+    int wg_begin;   // first builder workgroup of the job (set by the launcher)
+    int fin_begin;  // first finishing workgroup of the job (set by the launcher)
+    int slice_begin[kMaxSlices + 1];
+};
+struct BuildBatch {
+    int njobs;
+    BuildJob job[kMaxBuildJobs];
+};
+// launch shape of the paired builder: workgroup size (64, 128, 256 or 512
+// units) and atom slices (1..kMaxSlices)
+struct BuildShape {
+    int wg = 256;
+    int slices = 1;
+};
+
 struct Structure {
     dplus_pdb_desc desc{};
     std::vector<double> coef9, rad;
@@ -181,13 +212,37 @@ struct Structure {
     DevBuf b32, b64, bt, bd;  // immutable after construction
     Structure(const dplus_pdb_desc *d, hipStream_t s);
     void build(Grid &g, bool host_copy, hipStream_t s);
+    // The mixed build in three steps, for callers that launch the builder of
+    // several grids together: prepare uploads this grid's form factors and
+    // sizes the slice partials (buffers owned by `bufs`, used on stream s) and
+    // returns the builder's job; after the builder ran, finish sets the origin,
+    // takes the splines and drains s.
+    struct Pending {
+        DevBuf ff, part;
+        double origin = 0.0;
+        bool reduce = false;
+    };
+    BuildJob prepare(Grid &g, int slices, Pending &bufs, hipStream_t s) const;
+    void finish(Grid &g, const Pending &bufs, bool host_copy, hipStream_t s) const;
+    // launch shape of the paired builder for grid g: the tuned one when the
+    // shape table holds it, else the occupancy model's
+    BuildShape shape_for(const Grid &g) const;
+    // times every launch shape on grid g, records the fastest in the shape
+    // table and returns it (ms: its time and the occupancy model's)
+    BuildShape autotune(Grid &g, hipStream_t s, double *best_ms, double *model_ms);
 };
 
 std::vector<GroupDev> make_groups(const double *q, int nq, double step);
-// paired builder: work units of a grid with nhalf canonical nodes, and the
-// number of atom slices a launch over nunits units is split into
+// paired builder: work units of a grid with nhalf canonical nodes, the number
+// of atom slices the occupancy model splits a launch of workgroup size wg over
+// nunits units into, and one launch over several grids' jobs (njobs <=
+// kMaxBuildJobs, the same phase form `reduce` and workgroup size)
 long long pair_units(long long nhalf);
-int pair_slices(long long nunits, int ntiles);
+int pair_slices(long long nunits, int ntiles, int wg);
+void launch_build_batch(bool reduce, int wg, BuildJob *jobs, int njobs, hipStream_t s);
+// shape table of the autotuner, keyed by (grid shells, tile-count class)
+bool shape_lookup(int nshells, int ntiles, BuildShape *out);
+void shape_record(int nshells, int ntiles, BuildShape shape);
 void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s);
 // optional goodness of fit of an average: chi2 = sum ((scale I + constant - y) / sigma)^2
 // over q with sigma > 0; computed on the device from the device-resident

@@ -11,6 +11,13 @@
 // allocations, so one request's frees never stall the others.  A request's
 // result does not depend on what else runs: every kernel reads only that
 // request's buffers and its own seeded random streams.
+// Grouped builds (optional).  Small mixed PDB builds (up to batch_terms
+// terms) are handed to a batcher thread with its own stream: whatever builds
+// are ready when the previous grouped launch has been issued go out together
+// as one builder launch over up to 16 grids of different structures, grid
+// sizes and slice counts (each job waits on its request's upload event, each
+// request's stream waits on the launch's event), so small requests fill the
+// device together instead of launching one small grid each.
 //
 // Fit session.  Resident subunit grids plus fixed q, data and sigma; each
 // evaluation re-docks the subunits with new placements (symmetry parameters of
@@ -112,11 +119,11 @@ size_t request_bytes(const Request &r) {
         const long long P = total_points(gs);
         fp64 = r.precision == DPLUS_PRECISION_FP64;
         b += (size_t)P * (fp64 ? 32 : 16);
-        // FP64 partial sums of the paired builder's atom slices
+        // FP64 partial sums of the paired builder's atom slices (as many as a
+        // tuned launch shape may take)
         if (!fp64) {
             const long long units = pair_units(half_base(gs / 2 + 4));
-            const int S = pair_slices(units, 1 << 20);
-            b += (size_t)S * 12 * units * sizeof(double);
+            b += (size_t)kMaxSlices * 12 * units * sizeof(double);
         }
         b += (size_t)(r.desc.natoms + r.desc.nbox) * 48;
         step = r.qmax / (double)(gs / 2);
@@ -169,17 +176,43 @@ struct Engine {
     bool stopping = false;
     std::vector<hipStream_t> streams;
     std::vector<std::thread> workers;
+    // grouped builds: tickets of ready builds, the batcher's thread and stream
+    struct BuildTicket {
+        BuildJob job;
+        bool reduce = false;
+        hipEvent_t ready = nullptr, done = nullptr;
+        bool launched = false;
+        int code = 0;
+        std::string message;
+    };
+    bool batch_builds = false, autotune = false;
+    double batch_terms = 1e10;
+    std::mutex bmu;
+    std::condition_variable bcv;
+    std::deque<BuildTicket *> bq;
+    bool bstop = false;
+    hipStream_t bstream = nullptr;
+    std::thread bthread;
+    long long batch_launches = 0, batch_jobs = 0;
+    std::mutex tune_mu;  // one autotuning sweep at a time
 
-    Engine(long long budget_bytes, int nstreams) {
+    Engine(long long budget_bytes, int nstreams, bool batch, long long bterms, bool tune) {
         Device &d = Device::get();
         d.require();
         budget = budget_bytes > 0 ? (size_t)budget_bytes : (size_t)(0.85 * (double)d.mem);
         cache_cap = std::min<size_t>(budget / 16, (size_t)8 << 30);
+        batch_builds = batch;
+        autotune = tune;
+        if (bterms > 0) batch_terms = (double)bterms;
         nstreams = std::max(1, std::min(16, nstreams));
         for (int i = 0; i < nstreams; i++) {
             hipStream_t s;
             check_hip(hipStreamCreate(&s), "hipStreamCreate");
             streams.push_back(s);
+        }
+        if (batch_builds) {
+            check_hip(hipStreamCreate(&bstream), "hipStreamCreate");
+            bthread = std::thread([this] { batcher(); });
         }
         for (int i = 0; i < nstreams; i++) workers.emplace_back([this, i] { work(i); });
     }
@@ -192,11 +225,136 @@ struct Engine {
         }
         cv_work.notify_all();
         for (auto &t : workers) t.join();
+        if (bthread.joinable()) {
+            {
+                std::lock_guard<std::mutex> lk(bmu);
+                bstop = true;
+            }
+            bcv.notify_all();
+            bthread.join();
+        }
         cache.clear();  // frees are ordered on the worker streams: release before destroying them
         for (auto s : streams) {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
         }
+        if (bstream) {
+            (void)hipStreamSynchronize(bstream);
+            (void)hipStreamDestroy(bstream);
+        }
+    }
+
+    // The batcher: takes every ready build (up to 16), launches them as one
+    // grouped builder launch per phase form on its stream after each job's
+    // upload event, and records each job's completion event.  It keeps at
+    // most two grouped launches in flight, so builds that become ready while
+    // the device works on one launch gather into the next.
+    void batcher() {
+        hipEvent_t prev = nullptr;
+        for (;;) {
+            std::vector<BuildTicket *> take;
+            {
+                std::unique_lock<std::mutex> lk(bmu);
+                bcv.wait(lk, [&] { return bstop || !bq.empty(); });
+                if (bq.empty()) break;
+                while (!bq.empty() && (int)take.size() < kMaxBuildJobs) {
+                    take.push_back(bq.front());
+                    bq.pop_front();
+                }
+            }
+            hipEvent_t pace = nullptr;
+            for (int red = 0; red < 2; red++) {
+                std::vector<BuildJob> jobs;
+                std::vector<BuildTicket *> ts;
+                for (BuildTicket *t : take)
+                    if ((int)t->reduce == red) {
+                        jobs.push_back(t->job);
+                        ts.push_back(t);
+                    }
+                if (jobs.empty()) continue;
+                try {
+                    for (BuildTicket *t : ts) check_hip(hipStreamWaitEvent(bstream, t->ready, 0), "hipStreamWaitEvent");
+                    launch_build_batch(red != 0, 256, jobs.data(), (int)jobs.size(), bstream);
+                    check_hip(hipGetLastError(), "grouped build launch");
+                    for (BuildTicket *t : ts) check_hip(hipEventRecord(t->done, bstream), "hipEventRecord");
+                } catch (const Error &e) {
+                    // nothing of this launch may still read the jobs' buffers when they hear of it
+                    (void)hipStreamSynchronize(bstream);
+                    for (BuildTicket *t : ts) {
+                        t->code = e.code;
+                        t->message = e.what();
+                    }
+                }
+            }
+            if (hipEventCreateWithFlags(&pace, hipEventDisableTiming) == hipSuccess &&
+                hipEventRecord(pace, bstream) != hipSuccess) {
+                (void)hipEventDestroy(pace);
+                pace = nullptr;
+            }
+            {
+                std::lock_guard<std::mutex> lk(bmu);
+                for (BuildTicket *t : take) t->launched = true;
+                batch_launches++;
+                batch_jobs += (long long)take.size();
+            }
+            bcv.notify_all();
+            if (prev) {
+                (void)hipEventSynchronize(prev);
+                (void)hipEventDestroy(prev);
+            }
+            prev = pace;
+        }
+        if (prev) {
+            (void)hipEventSynchronize(prev);
+            (void)hipEventDestroy(prev);
+        }
+    }
+
+    // mixed build of a request's grid: autotuned on first sight of its class
+    // when asked, grouped with other small builds when batching, else alone
+    void build(Structure &S, Grid &g, hipStream_t s) {
+        BuildShape tuned;
+        const bool mixed = !g.fp64() && getenv("DPLUS_PHASE64") == nullptr;
+        if (mixed && autotune && !shape_lookup(g.N, S.ntiles, &tuned)) {
+            std::lock_guard<std::mutex> lk(tune_mu);
+            if (!shape_lookup(g.N, S.ntiles, &tuned)) {
+                S.autotune(g, s, nullptr, nullptr);
+                return;
+            }
+        }
+        const double terms = (double)half_base(g.N + 1) * (double)S.nsources;
+        if (!mixed || !batch_builds || terms > batch_terms) {
+            S.build(g, false, s);
+            return;
+        }
+        Structure::Pending P;
+        BuildTicket t;
+        t.job = S.prepare(g, S.shape_for(g).slices, P, s);
+        t.reduce = P.reduce;
+        check_hip(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming), "hipEventCreate");
+        if (hipEventCreateWithFlags(&t.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipEventDestroy(t.ready);
+            throw Error(kErrGeneral, "hipEventCreate failed");
+        }
+        struct Events {
+            BuildTicket &t;
+            hipStream_t s;
+            ~Events() {
+                (void)hipStreamSynchronize(s);
+                (void)hipEventDestroy(t.ready);
+                (void)hipEventDestroy(t.done);
+            }
+        } ev{t, s};
+        check_hip(hipEventRecord(t.ready, s), "hipEventRecord");
+        {
+            std::unique_lock<std::mutex> lk(bmu);
+            bq.push_back(&t);
+            bcv.notify_all();
+            bcv.wait(lk, [&] { return t.launched; });
+        }
+        if (t.code) throw Error(t.code, t.message);
+        check_hip(hipStreamWaitEvent(s, t.done, 0), "hipStreamWaitEvent");
+        S.finish(g, P, false, s);
     }
 
     // admit queued requests in arrival order while a slot is free and the HBM budget holds
@@ -289,7 +447,7 @@ struct Engine {
             if (r.has_pdb) {
                 S = structure_for(r, s);
                 G.reset(new Grid(r.grid_size, r.qmax, r.precision));
-                S->build(*G, false, s);
+                build(*S, *G, s);
                 g = G.get();
             }
             const double single[4] = {0, 0, 0, 1}, origin[3] = {0, 0, 0};
@@ -454,7 +612,8 @@ int dplus_engine_create(const dplus_engine_config *cfg, dplus_engine **out) {
         if (!out) throw Error(kErrInvalidArgs, "null output");
         const long long budget = cfg ? cfg->hbm_budget_bytes : 0;
         const int streams = cfg && cfg->streams > 0 ? cfg->streams : 8;
-        *out = (dplus_engine *)new Engine(budget, streams);
+        *out = (dplus_engine *)new Engine(budget, streams, cfg && cfg->batch_builds != 0, cfg ? cfg->batch_terms : 0,
+                                          cfg && cfg->autotune != 0);
     })
 }
 
@@ -612,6 +771,16 @@ int dplus_engine_cache_stats(dplus_engine *eh, long long *hits, long long *misse
         if (misses) *misses = e->cache_misses;
         if (bytes) *bytes = (long long)e->cache_bytes;
         if (capacity) *capacity = (long long)e->cache_cap;
+    })
+}
+
+int dplus_engine_batch_stats(dplus_engine *eh, long long *launches, long long *builds) {
+    ENGINE_GUARD({
+        Engine *e = (Engine *)eh;
+        if (!e) throw Error(kErrInvalidArgs, "null engine");
+        std::lock_guard<std::mutex> lk(e->bmu);
+        if (launches) *launches = e->batch_launches;
+        if (builds) *builds = e->batch_jobs;
     })
 }
 

@@ -4,5 +4,19 @@ symmetry docking and orientation averages) behind a C ABI
 from ._lib import (CALC_ATOMIC_FORMFACTORS, CALC_DUMMY_SOLVENT, OA_GK, OA_MC, OA_VEGAS, PRECISION_FP64,
                    PRECISION_MIXED, DplusError, device_info, lib)
 from ._lib import copy_probe, microbench_valu, timing_enable, timing_get, timing_reset
-from .grid import (Docking, Grid, Structure, dock_grid, dock_grid_space_filling, hydration_boxes, index_from_indices,
-                   indices_from_index, orientation_average, synth_walk)
+from .grid import (Docking, Grid, Structure, build_grids, build_shapes_clear, build_shapes_load, build_shapes_save,
+                   dock_grid, dock_grid_space_filling, hydration_boxes, index_from_indices, indices_from_index,
+                   orientation_average, synth_walk)
+
+
+def _load_shipped_shapes():
+    """Loads the builder launch shapes tuned on MI355X (build_shapes_mi355x.txt,
+    written by python -m dplus_mi355x.tune) into the library's shape table."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "build_shapes_mi355x.txt")
+    if os.path.exists(path) and os.path.exists(_lib.LIB_PATH):
+        with open(path) as f:
+            build_shapes_load(f.read())
+
+
+_load_shipped_shapes()

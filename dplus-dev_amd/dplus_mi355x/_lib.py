@@ -59,7 +59,8 @@ class OAStats(ctypes.Structure):
 
 
 class EngineConfig(ctypes.Structure):
-    _fields_ = [("hbm_budget_bytes", ctypes.c_longlong), ("streams", ctypes.c_int)]
+    _fields_ = [("hbm_budget_bytes", ctypes.c_longlong), ("streams", ctypes.c_int), ("batch_builds", ctypes.c_int),
+                ("batch_terms", ctypes.c_longlong), ("autotune", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -111,6 +112,13 @@ SIGNATURES = {
     "dplus_structure_create": (ctypes.c_int, [ctypes.POINTER(PdbDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "dplus_structure_destroy": (None, [ctypes.c_void_p]),
     "dplus_structure_build_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),
+    "dplus_structure_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _I, _I, _D, _D]),
+    "dplus_structure_build_grids": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                                   ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
+    "dplus_build_shapes_clear": (ctypes.c_int, []),
+    "dplus_structure_build_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, _I, _I, _I]),
+    "dplus_build_shapes_save": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_longlong)]),
+    "dplus_build_shapes_load": (ctypes.c_int, [ctypes.c_char_p]),
     "dplus_timing_enable": (ctypes.c_int, [ctypes.c_int]),
     "dplus_timing_get": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_longlong), _D]),
     "dplus_timing_reset": (ctypes.c_int, []),
@@ -140,6 +148,7 @@ SIGNATURES = {
     "dplus_engine_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, _D, ctypes.POINTER(RequestResult)]),
     "dplus_engine_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 5),
     "dplus_engine_cache_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
+    "dplus_engine_batch_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2),
     "dplus_session_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _D, _D, _D,
                                             ctypes.c_ulonglong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)]),
     "dplus_session_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Leaf), ctypes.c_int, ctypes.c_double,

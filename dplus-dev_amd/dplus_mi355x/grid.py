@@ -115,6 +115,23 @@ class Structure:
     def build(self, grid, host_copy=False):
         L.check(L.lib().dplus_structure_build_grid(self._h, grid.handle, 1 if host_copy else 0))
 
+    def build_shape(self, grid):
+        """dplus_structure_build_shape: {"wg", "slices", "tuned"} of a mixed build of grid."""
+        wg, sl, tu = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.check(L.lib().dplus_structure_build_shape(self._h, grid.handle, ctypes.byref(wg), ctypes.byref(sl),
+                                                    ctypes.byref(tu)))
+        return {"wg": wg.value, "slices": sl.value, "tuned": bool(tu.value)}
+
+    def autotune(self, grid):
+        """dplus_structure_autotune: times every builder launch shape on this grid,
+        records the fastest for the grid's class and builds the grid; returns
+        {"wg", "slices", "best_ms", "model_ms"}."""
+        wg, sl = ctypes.c_int(), ctypes.c_int()
+        tb, tm = ctypes.c_double(), ctypes.c_double()
+        L.check(L.lib().dplus_structure_autotune(self._h, grid.handle, ctypes.byref(wg), ctypes.byref(sl),
+                                                 ctypes.byref(tb), ctypes.byref(tm)))
+        return {"wg": wg.value, "slices": sl.value, "best_ms": tb.value, "model_ms": tm.value}
+
     def __del__(self):
         try:
             if self._h:
@@ -194,3 +211,33 @@ def synth_walk(seed, n, step=0.38, min_sep=0.12, axes=(3.0, 3.0, 3.0)):
     L.check(L.lib().dplus_synth_walk(int(seed), int(n), float(step), float(min_sep), float(axes[0]),
                                      float(axes[1]), float(axes[2]), L.dp(out)))
     return out.reshape(-1, 3)
+
+
+def build_shapes_save():
+    """The autotuner's shape table as text ("grid_shells tile_class wg slices" lines)."""
+    p, n = ctypes.c_void_p(), ctypes.c_longlong()
+    L.check(L.lib().dplus_build_shapes_save(ctypes.byref(p), ctypes.byref(n)))
+    try:
+        return ctypes.string_at(p, n.value).decode()
+    finally:
+        L.lib().dplus_free(p)
+
+
+def build_grids(structures, grids, host_copy=False):
+    """dplus_structure_build_grids: grid i from structure i, mixed grids in grouped launches."""
+    n = len(structures)
+    if len(grids) != n:
+        raise ValueError("one grid per structure")
+    S = (ctypes.c_void_p * max(n, 1))(*[s._h.value for s in structures])
+    G = (ctypes.c_void_p * max(n, 1))(*[g.handle.value for g in grids])
+    L.check(L.lib().dplus_structure_build_grids(n, S, G, 1 if host_copy else 0))
+
+
+def build_shapes_clear():
+    """Empties the autotuner's shape table (builds use the occupancy model again)."""
+    L.check(L.lib().dplus_build_shapes_clear())
+
+
+def build_shapes_load(text):
+    """Adds the lines of a saved shape table to this process's table."""
+    L.check(L.lib().dplus_build_shapes_load(text.encode()))

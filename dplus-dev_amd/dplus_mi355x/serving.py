@@ -11,11 +11,15 @@ from .grid import _pdb_desc
 
 class Engine:
     """dplus_engine_*: submit() returns a ticket at once; wait(ticket) gives
-    (I(q), result dict)."""
+    (I(q), result dict).  batch_builds groups small mixed builds (at most
+    batch_terms canonical-node x source terms; 0: 1e10) into shared builder
+    launches; autotune tunes each new (grid, tile class) build shape on first
+    sight."""
 
-    def __init__(self, hbm_budget_bytes=0, streams=8):
+    def __init__(self, hbm_budget_bytes=0, streams=8, batch_builds=False, batch_terms=0, autotune=False):
         h = ctypes.c_void_p()
-        cfg = L.EngineConfig(int(hbm_budget_bytes), int(streams))
+        cfg = L.EngineConfig(int(hbm_budget_bytes), int(streams), 1 if batch_builds else 0, int(batch_terms),
+                             1 if autotune else 0)
         L.check(L.lib().dplus_engine_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
         self._nq = {}
@@ -78,6 +82,9 @@ class Engine:
         c = [ctypes.c_longlong() for _ in range(4)]
         L.check(L.lib().dplus_engine_cache_stats(self._h, *[ctypes.byref(x) for x in c]))
         out.update(zip(("cache_hits", "cache_misses", "cache_bytes", "cache_capacity"), (x.value for x in c)))
+        b = [ctypes.c_longlong() for _ in range(2)]
+        L.check(L.lib().dplus_engine_batch_stats(self._h, *[ctypes.byref(x) for x in b]))
+        out.update(zip(("batch_launches", "batched_builds"), (x.value for x in b)))
         return out
 
     def close(self):

@@ -112,6 +112,32 @@ int dplus_structure_create(const dplus_pdb_desc *d, dplus_structure **out);
 void dplus_structure_destroy(dplus_structure *s);
 /* build the amplitude grid of a resident structure (+ CalculateSplines) */
 int dplus_structure_build_grid(dplus_structure *s, dplus_grid *g, int host_copy);
+/* Grid i of n from structure i (+ CalculateSplines), the mixed-precision ones
+ * as grouped builder launches of up to 16 grids each (structures, grid sizes
+ * and atom counts may all differ), the same values as n single builds.  The
+ * reference builds one leaf at a time (Amplitude.cpp:1348-1405). */
+int dplus_structure_build_grids(int n, dplus_structure *const *s, dplus_grid *const *g, int host_copy);
+
+/* Autotuner of the mixed builder's launch shape (no counterpart in the
+ * reference, whose kernels have fixed launch shapes, PDBAmplitudeCUDA.cu:
+ * 320-410): times every workgroup size (64, 128, 256, 512 units) with every
+ * atom-slice count (1..8) on this structure and grid, records the fastest in
+ * the process's shape table under (grid size, tile-count class) and builds the
+ * grid (+ splines).  Later builds of that class use the recorded shape; others
+ * use the occupancy model.  Writes the chosen shape, its time and the
+ * occupancy model's (ms). */
+int dplus_structure_autotune(dplus_structure *s, dplus_grid *g, int *wg, int *slices, double *best_ms,
+                             double *model_ms);
+/* the shape table as text, one "grid_shells tile_class wg slices" line per
+ * entry (callee-allocated, free with dplus_free); loading such text (lines
+ * starting with '#' are comments) adds or replaces entries; clearing empties
+ * the table */
+/* the launch shape a mixed build of grid g from structure s takes, and whether
+ * it comes from the shape table (1) or the occupancy model (0) */
+int dplus_structure_build_shape(dplus_structure *s, dplus_grid *g, int *wg, int *slices, int *tuned);
+int dplus_build_shapes_save(char **text, long long *len);
+int dplus_build_shapes_load(const char *text);
+int dplus_build_shapes_clear(void);
 
 /* ------------------------------------------------------------------ measurement */
 /* Per-kernel HIP-event timing on the library's stream: enable/disable, read the
@@ -263,6 +289,12 @@ typedef struct dplus_engine dplus_engine;
 typedef struct {
     long long hbm_budget_bytes; /* admission budget; <= 0: 85 % of device memory */
     int streams;                /* concurrent requests (1..16); <= 0: 8 */
+    int batch_builds;           /* 1: mixed PDB builds of at most batch_terms terms that are
+                                   ready together run as one grouped builder launch (up to
+                                   16 grids of different structures and sizes) */
+    long long batch_terms;      /* canonical nodes x (atoms + boxes); <= 0: 1e10 */
+    int autotune;               /* 1: a mixed build whose (grid, tile class) the shape table
+                                   lacks is autotuned on first sight (dplus_structure_autotune) */
 } dplus_engine_config;
 
 typedef struct {
@@ -313,6 +345,9 @@ int dplus_engine_stats(dplus_engine *e, long long *submitted, long long *complet
  * cached, and admission keeps that much of the budget free of requests */
 int dplus_engine_cache_stats(dplus_engine *e, long long *hits, long long *misses, long long *bytes,
                              long long *capacity);
+/* grouped builder launches of the engine (no counterpart in the reference):
+ * launches made and builds they carried */
+int dplus_engine_batch_stats(dplus_engine *e, long long *launches, long long *builds);
 
 /* ------------------------------------------------------------------ fit sessions */
 /* Fitting-loop evaluation over resident subunit grids (the hybrid docking of

@@ -33,3 +33,27 @@ def test_binding_matches_header():
     import dplus_mi355x._lib as L
     assert set(L.SIGNATURES) == declared()
     L.lib()  # every signature binds
+
+
+def test_build_shape_table_text_round_trip():
+    """The autotuner's shape table (host-only state): load, save, clear, and
+    rejection of malformed or out-of-range lines with ERROR_INVALID_ARGS."""
+    import dplus_mi355x as D
+    from dplus_mi355x._lib import DplusError
+    D.build_shapes_clear()
+    try:
+        assert D.build_shapes_save() == ""
+        D.build_shapes_load("# grid_shells tile_class wg slices\n28 20 256 4\n53 27 512 2\n")
+        assert D.build_shapes_save() == "28 20 256 4\n53 27 512 2\n"
+        D.build_shapes_load("28 20 128 3")  # replaces the (28, 20) entry
+        assert D.build_shapes_save() == "28 20 128 3\n53 27 512 2\n"
+        for bad in ("28 20 96 4", "28 20 256 9", "0 1 256 1", "28 20 256"):
+            try:
+                D.build_shapes_load(bad)
+            except DplusError as e:
+                assert e.error_code == 21
+            else:
+                raise AssertionError(bad)
+        assert D.build_shapes_save() == "28 20 128 3\n53 27 512 2\n"
+    finally:
+        D.build_shapes_clear()

@@ -299,3 +299,67 @@ def test_rebuild_is_idempotent(dplus):
         st.build(g, host_copy=True)
         st.build(g, host_copy=True)
         assert np.array_equal(g.data, first)
+
+
+def test_grouped_build_matches_single_builds(dplus, oracle):
+    """One grouped builder launch over grids of different structures, atom
+    counts, hydration boxes, grid sizes and phase forms gives every grid the
+    bits of its own single build, and the oracle's values."""
+    say("[case] grouped build: 6 grids (5 mixed + 1 FP64) in one call vs one by one")
+    specs = [(120, 3, 1.0, 16, 4.0, 1), (900, 4, 1.6, 40, 6.0, 1), (2500, 5, 2.2, 30, 5.0, 1),
+             (60, 6, 0.8, 24, 7.5, 0), (700, 7, 1.4, 50, 3.0, 1), (400, 8, 1.5, 44, 7.0, 1)]
+    sts, grids, singles, refs = [], [], [], []
+    for i, (n, seed, radius, gs, qmax, prec) in enumerate(specs):
+        xyz, ion, coef9, rad = structure(n, seed, radius)
+        if i == 5:
+            xyz = xyz + np.array([100.0, 0.0, 0.0])  # phases beyond 128 revolutions: the reduced form
+        kw = {}
+        if i == 2:
+            dims, com = dplus.hydration_boxes(xyz, rad[ion], probe=0.14, thickness=0.3, vox=0.1)
+            kw = dict(box_dims=dims, box_com=com, vox_step=0.1, box_contrast=33.3)
+        st = dplus.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=3, **kw)
+        g1 = dplus.Grid(gs, qmax, prec)
+        st.build(g1, host_copy=True)
+        sts.append(st)
+        grids.append(dplus.Grid(gs, qmax, prec))
+        singles.append(g1.data.copy())
+        if i == 1:
+            ions = oracle_ions(oracle, coef9, rad, ion, sol=333.0)
+            refs.append((i, oracle.grid_fill_pdb(gs, qmax, xyz, ion, ions)))
+    dplus.build_grids(sts, grids, host_copy=True)
+    for i, (g, one) in enumerate(zip(grids, singles)):
+        assert np.array_equal(g.data, one), i
+    for i, ref in refs:
+        assert relmax(grids[i].data, ref) < 2e-5, i
+
+
+def test_autotune_shape_table(dplus):
+    """The autotuner times every launch shape, records the fastest for the
+    grid's class, and leaves the grid with the values of a default build."""
+    say("[case] autotune: 3,000 atoms at grid 60")
+    xyz, ion, coef9, rad = structure(3000, 9, 2.0)
+    st = dplus.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=3)
+    dplus.build_shapes_clear()
+    try:
+        g0 = dplus.Grid(60, 6.0, 1)
+        st.build(g0, host_copy=True)
+        g = dplus.Grid(60, 6.0, 1)
+        shape = st.autotune(g)
+        say("   ", shape)
+        assert shape["wg"] in (64, 128, 256, 512) and 1 <= shape["slices"] <= 8
+        assert 0 < shape["best_ms"] <= shape["model_ms"]
+        g.download()
+        assert relmax(g.data, g0.data) < 1e-6
+        table = dplus.build_shapes_save()
+        lines = table.split()
+        assert len(lines) == 4 and int(lines[0]) == g.actual_size
+        assert int(lines[2]) == shape["wg"] and int(lines[3]) == shape["slices"]
+        # a build of the same class takes the recorded shape: same values as the default build
+        g2 = dplus.Grid(60, 6.0, 1)
+        st.build(g2, host_copy=True)
+        assert relmax(g2.data, g0.data) < 1e-6
+        dplus.build_shapes_clear()
+        dplus.build_shapes_load(table)
+        assert dplus.build_shapes_save() == table
+    finally:
+        dplus.build_shapes_clear()

@@ -222,3 +222,52 @@ def test_engine_structure_cache(dplus):
         assert np.array_equal(fresh.wait(fresh.submit(q, pdb=p, grid_size=gs, qmax=3.0, **kw))[0], I)
     fresh.close()
     assert np.array_equal(outs[0], outs[3]) and not np.array_equal(outs[0], outs[2])
+
+
+def test_engine_grouped_builds_same_as_alone(dplus):
+    """With grouped builds on, every mixed PDB build goes through the batcher's
+    shared launches and each request keeps the bits it has alone."""
+    from dplus_mi355x.serving import Engine
+    say("[case] engine: grouped builder launches vs one request at a time")
+    reqs = requests(dplus)
+    e1 = Engine(streams=1)
+    alone = [e1.wait(e1.submit(q, **kw)) for q, kw in reqs]
+    e1.close()
+    eb = Engine(streams=8, batch_builds=True)
+    tickets = [eb.submit(q, **kw) for q, kw in reqs]
+    mix = [eb.wait(t) for t in tickets]
+    st = eb.stats()
+    eb.close()
+    say(f"    engine stats {st}")
+    nmixed = sum(1 for _, kw in reqs if kw["precision"] == dplus.PRECISION_MIXED)
+    assert st["batched_builds"] == nmixed and 1 <= st["batch_launches"] <= nmixed
+    for (I1, r1), (Ib, rb) in zip(alone, mix):
+        assert np.array_equal(I1, Ib)
+        assert r1["chi2"] == rb["chi2"]
+
+
+def test_engine_autotunes_new_build_classes(dplus):
+    """autotune=True: the first mixed build of a (grid, tile class) runs the
+    autotuner and records its shape; results agree with the untuned engine."""
+    from dplus_mi355x.serving import Engine
+    say("[case] engine: autotune on first sight")
+    pdb = make_pdb(1500, 21, 1.8)
+    q = np.linspace(0.05, 5.0, 80)
+    kw = dict(pdb=pdb, grid_size=40, qmax=5.0, method=dplus.OA_MC, seed=3, max_dirs=8192, chunk=4096)
+    dplus.build_shapes_clear()
+    try:
+        e0 = Engine(streams=1)
+        I0, _ = e0.wait(e0.submit(q, **kw))
+        e0.close()
+        assert dplus.build_shapes_save() == ""
+        et = Engine(streams=2, autotune=True)
+        I1, _ = et.wait(et.submit(q, **kw))
+        I2, _ = et.wait(et.submit(q, **kw))
+        et.close()
+        table = dplus.build_shapes_save().split()
+        say(f"    shape table {table}")
+        assert len(table) == 4 and int(table[0]) == 40 // 2 + 3
+        assert np.max(np.abs(I1 - I0) / I0) < 1e-6
+        assert np.array_equal(I1, I2)
+    finally:
+        dplus.build_shapes_clear()
