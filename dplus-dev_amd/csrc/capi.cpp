@@ -1235,8 +1235,11 @@ int dplus_structure_build_grid(dplus_structure *sh, dplus_grid *gh, int host_cop
 int dplus_structure_build_grids(int n, dplus_structure *const *sh, dplus_grid *const *gh, int host_copy) {
     API_GUARD({
         if (n < 0 || (n > 0 && (!sh || !gh))) throw Error(kErrInvalidArgs, "null argument");
-        for (int i = 0; i < n; i++)
+        for (int i = 0; i < n; i++) {
             if (!sh[i] || !gh[i]) throw Error(kErrInvalidArgs, "null structure or grid");
+            for (int k = 0; k < i; k++)
+                if (gh[k] == gh[i]) throw Error(kErrInvalidArgs, "a grid appears twice in one grouped build");
+        }
         Device::get().require();
         const hipStream_t s = Device::get().stream;
         // mixed grids in grouped launches of up to 16 per phase form; FP64 grids one by one

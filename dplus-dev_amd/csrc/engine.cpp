@@ -57,25 +57,36 @@ double ms_between(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-// 64-bit content hash (multiply-rotate mixing over 8-byte words); two seeds
-// give the 128-bit key of the structure cache
+// 128-bit content hash (multiply-rotate mixing over 8-byte words): two seeds
+// times four interleaved lanes, eight independent chains per pass over the
+// data, folded at the end of each array
 uint64_t mix64(uint64_t h, uint64_t v) {
     h ^= v * 0x9E3779B97F4A7C15ULL;
     h = (h << 29 | h >> 35) * 0xBF58476D1CE4E5B9ULL;
     return h ^ (h >> 31);
 }
 
-uint64_t hash_bytes(uint64_t h, const void *p, size_t n) {
+void hash_bytes(uint64_t key[2], const void *p, size_t n) {
     const unsigned char *b = (const unsigned char *)p;
+    uint64_t h[2][4];
+    for (int k = 0; k < 2; k++)
+        for (int l = 0; l < 4; l++) h[k][l] = mix64(key[k], (uint64_t)l + 1);
     size_t i = 0;
-    for (; i + 8 <= n; i += 8) {
-        uint64_t v;
-        memcpy(&v, b + i, 8);
-        h = mix64(h, v);
+    for (; i + 32 <= n; i += 32) {
+        uint64_t v[4];
+        memcpy(v, b + i, 32);
+        for (int l = 0; l < 4; l++) {
+            h[0][l] = mix64(h[0][l], v[l]);
+            h[1][l] = mix64(h[1][l], v[l]);
+        }
     }
-    uint64_t t = 0;
-    for (size_t k = 0; i + k < n; k++) t |= (uint64_t)b[i + k] << (8 * k);
-    return mix64(h, t ^ ((uint64_t)n << 56));
+    uint64_t t[4] = {0, 0, 0, 0};
+    for (size_t k = 0; i + k < n; k++) t[k / 8] |= (uint64_t)b[i + k] << (8 * (k % 8));
+    for (int k = 0; k < 2; k++) {
+        uint64_t a = key[k];
+        for (int l = 0; l < 4; l++) a = mix64(a, mix64(h[k][l], t[l]));
+        key[k] = mix64(a, (uint64_t)n);
+    }
 }
 
 struct Request {
@@ -405,10 +416,29 @@ struct Engine {
         }
     }
 
+    // content key of a request's PDB description: every array and scalar
+    static void structure_key(Request &r) {
+        const dplus_pdb_desc &d = r.desc;
+        uint64_t key[2] = {0x13198A2E03707344ULL, 0x243F6A8885A308D3ULL};
+        const double sc[6] = {d.sol_ed, d.c1, d.scale, d.vox_step, d.box_contrast, (double)d.flags};
+        const long long n[3] = {d.natoms, d.nion, d.nbox};
+        hash_bytes(key, n, sizeof n);
+        hash_bytes(key, sc, sizeof sc);
+        hash_bytes(key, r.xyz.data(), r.xyz.size() * sizeof(double));
+        hash_bytes(key, r.ion.data(), r.ion.size() * sizeof(int));
+        hash_bytes(key, r.coef9.data(), r.coef9.size() * sizeof(double));
+        hash_bytes(key, r.rad.data(), r.rad.size() * sizeof(double));
+        hash_bytes(key, r.box_dims.data(), r.box_dims.size() * sizeof(int));
+        hash_bytes(key, r.box_com.data(), r.box_com.size() * sizeof(double));
+        r.key[0] = key[0];
+        r.key[1] = key[1];
+    }
+
     // the resident structure of a request's PDB description: from the cache, or
     // built on the request's stream and cached (least recently used entries that
     // no running request holds are dropped past the cap)
-    std::shared_ptr<Structure> structure_for(const Request &r, hipStream_t s) {
+    std::shared_ptr<Structure> structure_for(Request &r, hipStream_t s) {
+        structure_key(r);
         auto same = [&](const Cached &c) { return c.key[0] == r.key[0] && c.key[1] == r.key[1]; };
         {
             std::lock_guard<std::mutex> lk(cache_mu);
@@ -644,20 +674,7 @@ int dplus_engine_submit(dplus_engine *eh, const dplus_request *rq, long long *ti
             r->desc.rad = r->rad.data();
             r->desc.box_dims = d.nbox > 0 ? r->box_dims.data() : nullptr;
             r->desc.box_com = d.nbox > 0 ? r->box_com.data() : nullptr;
-            for (int k = 0; k < 2; k++) {
-                uint64_t h = k ? 0x243F6A8885A308D3ULL : 0x13198A2E03707344ULL;
-                const double sc[6] = {d.sol_ed, d.c1, d.scale, d.vox_step, d.box_contrast, (double)d.flags};
-                const long long n[3] = {d.natoms, d.nion, d.nbox};
-                h = hash_bytes(h, n, sizeof n);
-                h = hash_bytes(h, sc, sizeof sc);
-                h = hash_bytes(h, r->xyz.data(), r->xyz.size() * sizeof(double));
-                h = hash_bytes(h, r->ion.data(), r->ion.size() * sizeof(int));
-                h = hash_bytes(h, r->coef9.data(), r->coef9.size() * sizeof(double));
-                h = hash_bytes(h, r->rad.data(), r->rad.size() * sizeof(double));
-                h = hash_bytes(h, r->box_dims.data(), r->box_dims.size() * sizeof(int));
-                h = hash_bytes(h, r->box_com.data(), r->box_com.size() * sizeof(double));
-                r->key[k] = h;
-            }
+            // the structure cache's key is hashed by the worker that runs the request
             r->grid_size = rq->grid_size;
             r->qmax = rq->qmax;
             r->precision = rq->precision;

@@ -115,7 +115,8 @@ int dplus_structure_build_grid(dplus_structure *s, dplus_grid *g, int host_copy)
 /* Grid i of n from structure i (+ CalculateSplines), the mixed-precision ones
  * as grouped builder launches of up to 16 grids each (structures, grid sizes
  * and atom counts may all differ), the same values as n single builds.  The
- * reference builds one leaf at a time (Amplitude.cpp:1348-1405). */
+ * grids must be distinct (ERROR_INVALID_ARGS otherwise).  The reference
+ * builds one leaf at a time (Amplitude.cpp:1348-1405). */
 int dplus_structure_build_grids(int n, dplus_structure *const *s, dplus_grid *const *g, int host_copy);
 
 /* Autotuner of the mixed builder's launch shape (no counterpart in the
