@@ -365,6 +365,19 @@ def main():
     err = float(np.max(np.abs(I32 - I64) / np.abs(I64)))
     log(f"[bench] fp64: {np.mean(lat64):.3f} s/curve; max rel I(q) mixed vs fp64 = {err:.2e}")
 
+    # ---- the same grids averaged by adaptive Gauss-Kronrod (relative tolerance 1e-4, depth 8)
+    gk = {}
+    for name, g in (("mixed", g32), ("fp64", g64)):
+        t0 = time.perf_counter()
+        Igk, stg = D.orientation_average(D.Docking.single(g), W["q"], method=D.OA_GK, eps=1e-4, gk_max_depth=8)
+        gk[name] = (Igk, time.perf_counter() - t0, stg.evaluations)
+    gk_err = float(np.max(np.abs(gk["mixed"][0] - gk["fp64"][0]) / np.abs(gk["fp64"][0])))
+    gk_out = {"tolerance": 1e-4, "max_depth": 8, "mixed_s": gk["mixed"][1], "fp64_s": gk["fp64"][1],
+              "evaluations": gk["mixed"][2], "curve_s_mixed": gk["mixed"][1] + ms_build / max(nb, 1) * 1e-3,
+              "max_rel_err_vs_fp64": gk_err}
+    log(f"[bench] GK (tol 1e-4): mixed {gk['mixed'][1]:.3f} s, fp64 {gk['fp64'][1]:.3f} s per average "
+        f"({gk['mixed'][2]} integrand evaluations); max rel I(q) mixed vs fp64 = {gk_err:.2e}")
+
     # ---- roofline of the dominant kernel (the amplitude builder, VALU-bound)
     N = W["gs"] // 2 + 3
     nhalf = 3 * (N + 1) ** 2 * N
@@ -418,6 +431,7 @@ def main():
                  "p99_ms": 1e3 * float(np.percentile(lat64, 99)), "steps": args.fp64_steps},
         "speedup_vs_fp64": value / fp64_value,
         "max_rel_err_vs_fp64": err,
+        "gauss_kronrod": gk_out,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": f"build_atoms_pairs<false, {shape['wg']}>", "launch_ms": t_launch * 1e3,

@@ -105,16 +105,25 @@ __global__ __launch_bounds__(64) void splines_kernel(T4 *__restrict__ g, int nsh
     double bx = 0.0, by = 0.0;  // bottom-row accumulation of bl[i] * as[i]
     bx += bl[0] * x0;
     by += bl[0] * y0;
+    // the node values two apart are carried in registers; unrolled so that the
+    // loads of later nodes and multipliers (independent of the recurrence)
+    // are in flight while earlier steps compute
+    T vxm = p[0].x, vym = p[0].y, vx0 = p[1].x, vy0 = p[1].y;
+#pragma unroll 8
     for (int k = 1; k < n - 1; k++) {
-        T4 a = p[k + 1], b = p[k - 1];
-        double ax = 3.0 * ((double)a.x - (double)b.x) - lx * fwd[k];
-        double ay = 3.0 * ((double)a.y - (double)b.y) - ly * fwd[k];
+        const T vx1 = p[k + 1].x, vy1 = p[k + 1].y;
+        double ax = 3.0 * ((double)vx1 - (double)vxm) - lx * fwd[k];
+        double ay = 3.0 * ((double)vy1 - (double)vym) - ly * fwd[k];
         p[k].z = (T)ax;
         p[k].w = (T)ay;
         bx += bl[k] * ax;
         by += bl[k] * ay;
         lx = ax;
         ly = ay;
+        vxm = vx0;
+        vym = vy0;
+        vx0 = vx1;
+        vy0 = vy1;
     }
     // last row
     double nx, ny;
@@ -141,6 +150,7 @@ __global__ __launch_bounds__(64) void splines_kernel(T4 *__restrict__ g, int nsh
     q.z = (T)(cx / dg[n - 2]);
     q.w = (T)(cy / dg[n - 2]);
     p[n - 2] = q;
+#pragma unroll 8
     for (int k = n - 3; k >= 0; k--) {
         T4 a = p[k];
         double ax = (double)a.z - nx * rr[k] - cx * uu[k];
