@@ -8,8 +8,14 @@
 // of integrand calls on the device (the host sends one task per interval, the
 // device forms its 15 points); accept/split decisions, the Gauss and Kronrod
 // sums and the sums left + right in the reference's order are taken on the host.
+// The host work of a level is split over threads at theta-task boundaries (a
+// theta task's open phi intervals are contiguous and no other task touches its
+// trees), and the threads' new intervals are concatenated in order, so the
+// result does not depend on the thread count.
 
+#include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "device.hpp"
@@ -148,19 +154,47 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
     for (int i = 0; i < 15; i++) nodes.x[i] = kX[i];
     std::vector<double4> tasks;
     DevBuf btask;
+    const int hw = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    // [begin, end) ranges of `open` for the host threads, cut at theta-task boundaries
+    auto ranges = [&]() {
+        const size_t n = open.size();
+        const int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)hw, n / 8192));
+        std::vector<size_t> cut(1, 0);
+        for (int k = 1; k < nt; k++) {
+            size_t b = std::max(cut.back(), n * (size_t)k / (size_t)nt);
+            while (b > 0 && b < n && open[b].theta_task == open[b - 1].theta_task) b++;
+            cut.push_back(b);
+        }
+        cut.push_back(n);
+        return cut;
+    };
+    auto parallel = [&](const std::vector<size_t> &cut, const auto &fn) {
+        const int nt = (int)cut.size() - 1;
+        if (nt == 1) {
+            fn(0, cut[0], cut[1]);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (int k = 1; k < nt; k++) th.emplace_back([&, k] { fn(k, cut[k], cut[k + 1]); });
+        fn(0, cut[0], cut[1]);
+        for (auto &t : th) t.join();
+    };
     while (!open.empty()) {
+        const std::vector<size_t> cut = ranges();
         // integrand at the 15 Kronrod nodes of every open phi interval: the
         // interval's task (q sin theta, q cos theta, centre, half width)
         tasks.resize(open.size());
-        for (size_t t = 0; t < open.size(); t++) {
-            const PhiTask &P = open[t];
-            const ThetaTask &T = tt[P.theta_task];
-            const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
-            const double th = apb2 + ba2 * kX[P.k];
-            const double qq = q[T.q];
-            tasks[t] = make_double4(qq * std::sin(th), qq * std::cos(th), (P.pmax + P.pmin) * 0.5,
-                                    (P.pmax - P.pmin) * 0.5);
-        }
+        parallel(cut, [&](int, size_t b, size_t e) {
+            for (size_t t = b; t < e; t++) {
+                const PhiTask &P = open[t];
+                const ThetaTask &T = tt[P.theta_task];
+                const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
+                const double th = apb2 + ba2 * kX[P.k];
+                const double qq = q[T.q];
+                tasks[t] = make_double4(qq * std::sin(th), qq * std::cos(th), (P.pmax + P.pmin) * 0.5,
+                                        (P.pmax - P.pmin) * 0.5);
+            }
+        });
         {
             const size_t npts = open.size() * 15;
             btask.upload(tasks.data(), tasks.size() * sizeof(double4), s);
@@ -175,33 +209,44 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
             check_hip(hipStreamSynchronize(s), "gk batch");
             evals += (long long)npts;
         }
+        const int nt = (int)cut.size() - 1;
+        std::vector<std::vector<PhiTask>> next_part(nt);
+        std::vector<std::vector<int>> fin_part(nt);
+        parallel(cut, [&](int part, size_t b, size_t e) {
+            std::vector<PhiTask> &next = next_part[part];
+            std::vector<int> &finished_theta = fin_part[part];
+            for (size_t t = b; t < e; t++) {
+                const PhiTask &P = open[t];
+                const double ba2 = (P.pmax - P.pmin) * 0.5, apb2 = (P.pmax + P.pmin) * 0.5;
+                double kv = 0.0, gv = 0.0;
+                for (int i = 0; i < 15; i++) {
+                    const double I = vals[t * 15 + i];
+                    kv += I * ba2 * kW[i];
+                    if (i % 2 == 1) gv += I * ba2 * gW[i / 2];
+                }
+                Tree &pt = phiTrees[P.tree];
+                if (P.mind < 0 && (P.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv)) {
+                    const int r = pt.resolve(P.node, kv);
+                    if (r >= 0) {
+                        ThetaTask &T = tt[P.theta_task];
+                        T.phival[P.k] = pt.n[r].value;
+                        if (--T.left == 0) finished_theta.push_back(P.theta_task);
+                    }
+                } else {
+                    const int l = pt.add(P.node), rgt = pt.add(P.node);
+                    pt.n[P.node].child[0] = l;
+                    pt.n[P.node].child[1] = rgt;
+                    pt.n[P.node].pending = 2;
+                    next.push_back({P.theta_task, P.k, P.pmin, apb2, P.maxd - 1, P.mind - 1, l, P.tree});
+                    next.push_back({P.theta_task, P.k, apb2, P.pmax, P.maxd - 1, P.mind - 1, rgt, P.tree});
+                }
+            }
+        });
         std::vector<PhiTask> next;
         std::vector<int> finished_theta;
-        for (size_t t = 0; t < open.size(); t++) {
-            const PhiTask &P = open[t];
-            const double ba2 = (P.pmax - P.pmin) * 0.5, apb2 = (P.pmax + P.pmin) * 0.5;
-            double kv = 0.0, gv = 0.0;
-            for (int i = 0; i < 15; i++) {
-                const double I = vals[t * 15 + i];
-                kv += I * ba2 * kW[i];
-                if (i % 2 == 1) gv += I * ba2 * gW[i / 2];
-            }
-            Tree &pt = phiTrees[P.tree];
-            if (P.mind < 0 && (P.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv)) {
-                const int r = pt.resolve(P.node, kv);
-                if (r >= 0) {
-                    ThetaTask &T = tt[P.theta_task];
-                    T.phival[P.k] = pt.n[r].value;
-                    if (--T.left == 0) finished_theta.push_back(P.theta_task);
-                }
-            } else {
-                const int l = pt.add(P.node), rgt = pt.add(P.node);
-                pt.n[P.node].child[0] = l;
-                pt.n[P.node].child[1] = rgt;
-                pt.n[P.node].pending = 2;
-                next.push_back({P.theta_task, P.k, P.pmin, apb2, P.maxd - 1, P.mind - 1, l, P.tree});
-                next.push_back({P.theta_task, P.k, apb2, P.pmax, P.maxd - 1, P.mind - 1, rgt, P.tree});
-            }
+        for (int k = 0; k < nt; k++) {
+            next.insert(next.end(), next_part[k].begin(), next_part[k].end());
+            finished_theta.insert(finished_theta.end(), fin_part[k].begin(), fin_part[k].end());
         }
         open.swap(next);
         // theta intervals whose 15 phi integrals are complete
