@@ -86,9 +86,12 @@ struct GroupDev {
     int layer, q0, m, pad;
 };
 
-// the 15 Kronrod abscissae on [-1, 1] (passed to the integrand kernel by value)
+// the 15 Kronrod abscissae on [-1, 1] with the Kronrod and Gauss weights
+// (passed to the integrand kernel by value)
 struct GkNodes {
     double x[15];
+    double kw[15];
+    double gw[7];
 };
 
 struct LeafGeo {

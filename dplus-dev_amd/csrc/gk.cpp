@@ -6,8 +6,9 @@
 // and divide by 2 pi.  The recursion is replayed level by level here: every
 // open phi interval of every q and every theta node is evaluated in one batch
 // of integrand calls on the device (the host sends one task per interval, the
-// device forms its 15 points); accept/split decisions, the Gauss and Kronrod
-// sums and the sums left + right in the reference's order are taken on the host.
+// device forms its 15 points and their Kronrod and Gauss sums, in the
+// reference's arithmetic); accept/split decisions and the sums left + right in
+// the reference's order are taken on the host.
 // The host work of a level is split over threads at theta-task boundaries (a
 // theta task's open phi intervals are contiguous and no other task touches its
 // trees), and the threads' new intervals are concatenated in order, so the
@@ -151,7 +152,11 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
     }
 
     GkNodes nodes;
-    for (int i = 0; i < 15; i++) nodes.x[i] = kX[i];
+    for (int i = 0; i < 15; i++) {
+        nodes.x[i] = kX[i];
+        nodes.kw[i] = kW[i];
+    }
+    for (int i = 0; i < 7; i++) nodes.gw[i] = gW[i];
     std::vector<double4> tasks;
     DevBuf btask;
     const int hw = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
@@ -196,16 +201,17 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
             }
         });
         {
-            const size_t npts = open.size() * 15;
+            // the device returns each interval's Kronrod and Gauss sums
+            const size_t npts = open.size() * 15, nout = open.size() * 2;
             btask.upload(tasks.data(), tasks.size() * sizeof(double4), s);
-            bo.ensure(npts * sizeof(double));
+            bo.ensure(nout * sizeof(double));
             KTimer kt("gk", s);
             launch_docked_gk(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
                              (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (long long)open.size(),
                              (const double4 *)btask.p, nodes, (double *)bo.p, s);
             check_hip(hipGetLastError(), "gk launch");
-            vals.resize(npts);
-            check_hip(hipMemcpyAsync(vals.data(), bo.p, npts * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            vals.resize(nout);
+            check_hip(hipMemcpyAsync(vals.data(), bo.p, nout * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
             check_hip(hipStreamSynchronize(s), "gk batch");
             evals += (long long)npts;
         }
@@ -217,13 +223,8 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
             std::vector<int> &finished_theta = fin_part[part];
             for (size_t t = b; t < e; t++) {
                 const PhiTask &P = open[t];
-                const double ba2 = (P.pmax - P.pmin) * 0.5, apb2 = (P.pmax + P.pmin) * 0.5;
-                double kv = 0.0, gv = 0.0;
-                for (int i = 0; i < 15; i++) {
-                    const double I = vals[t * 15 + i];
-                    kv += I * ba2 * kW[i];
-                    if (i % 2 == 1) gv += I * ba2 * gW[i / 2];
-                }
+                const double apb2 = (P.pmax + P.pmin) * 0.5;
+                const double kv = vals[2 * t], gv = vals[2 * t + 1];
                 Tree &pt = phiTrees[P.tree];
                 if (P.mind < 0 && (P.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv)) {
                     const int r = pt.resolve(P.node, kv);

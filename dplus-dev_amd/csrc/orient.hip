@@ -640,19 +640,45 @@ __global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const Lea
 }
 
 // Gauss-Kronrod phi intervals: task t = (q sin(theta), q cos(theta), a, b) of
-// one interval; its 15 integrand points are phi = a + b x_k, written to
-// out[15 t + k]
+// one interval; its 15 integrand points are phi = a + b x_k.  A workgroup of
+// 240 threads evaluates the points of 16 tasks, then one thread per task forms
+// the Kronrod and Gauss sums
+//   out[2t] = sum_k I_k b kw_k,  out[2t + 1] = sum_{k odd} I_k b gw_{k/2}
+// in k order with every product and sum rounded as written (no fused
+// multiply-add), the arithmetic of DomainModel::GaussLegPhi
+// (Amplitude.cpp:1443-1467), so accept/split decisions on the host follow it.
+constexpr int kGkTasksPerBlock = 16;
+
 template <typename T, typename T4>
-__global__ void docked_gk_batch(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
-                                const RotDev *__restrict__ rots, int nrot, const double4 *__restrict__ trans,
-                                long long ntasks, const double4 *__restrict__ tasks, GkNodes nodes,
-                                double *__restrict__ out) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 15 * ntasks) return;
-    const double4 tk = tasks[i / 15];
-    const double xi = tk.z + tk.w * nodes.x[i % 15];
-    const double q[3] = {tk.x * cos(xi), tk.x * sin(xi), tk.y};
-    out[i] = docked_intensity<T, T4>(grids, geo, rots, nrot, trans, q);
+__global__ __launch_bounds__(15 * kGkTasksPerBlock) void docked_gk_batch(
+    const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo, const RotDev *__restrict__ rots,
+    int nrot, const double4 *__restrict__ trans, long long ntasks, const double4 *__restrict__ tasks,
+    GkNodes nodes, double *__restrict__ out) {
+    __shared__ double val[15 * kGkTasksPerBlock];
+    const long long t0 = (long long)blockIdx.x * kGkTasksPerBlock;
+    const long long t = t0 + threadIdx.x / 15;
+    double I = 0.0;
+    if (t < ntasks) {
+        const double4 tk = tasks[t];
+        const double xi = tk.z + tk.w * nodes.x[threadIdx.x % 15];
+        const double q[3] = {tk.x * cos(xi), tk.x * sin(xi), tk.y};
+        I = docked_intensity<T, T4>(grids, geo, rots, nrot, trans, q);
+    }
+    val[threadIdx.x] = I;
+    __syncthreads();
+    const long long ts = t0 + threadIdx.x;
+    if (threadIdx.x < kGkTasksPerBlock && ts < ntasks) {
+        const double hw = tasks[ts].w;
+        const double *v = val + 15 * threadIdx.x;
+        double kv = 0.0, gv = 0.0;
+        for (int k = 0; k < 15; k++) {
+            const double p = __dmul_rn(v[k], hw);
+            kv = __dadd_rn(kv, __dmul_rn(p, nodes.kw[k]));
+            if (k % 2 == 1) gv = __dadd_rn(gv, __dmul_rn(p, nodes.gw[k / 2]));
+        }
+        out[2 * ts] = kv;
+        out[2 * ts + 1] = gv;
+    }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -718,13 +744,14 @@ void launch_seeded_samples(bool fp64, const void *g, int N, double step, double 
 void launch_docked_gk(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
                       const double4 *trans, long long ntasks, const double4 *tasks, const GkNodes &nodes,
                       double *out, hipStream_t s) {
-    const unsigned nb = (unsigned)((15 * ntasks + 255) / 256);
+    const unsigned nb = (unsigned)((ntasks + kGkTasksPerBlock - 1) / kGkTasksPerBlock);
+    const int bs = 15 * kGkTasksPerBlock;
     if (fp64)
-        docked_gk_batch<double, double4><<<nb, 256, 0, s>>>((const double4 *const *)grids, geo, rots, nrot, trans,
-                                                             ntasks, tasks, nodes, out);
+        docked_gk_batch<double, double4><<<nb, bs, 0, s>>>((const double4 *const *)grids, geo, rots, nrot, trans,
+                                                            ntasks, tasks, nodes, out);
     else
-        docked_gk_batch<float, float4><<<nb, 256, 0, s>>>((const float4 *const *)grids, geo, rots, nrot, trans,
-                                                           ntasks, tasks, nodes, out);
+        docked_gk_batch<float, float4><<<nb, bs, 0, s>>>((const float4 *const *)grids, geo, rots, nrot, trans,
+                                                          ntasks, tasks, nodes, out);
 }
 
 void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots,
