@@ -87,7 +87,7 @@ struct GroupDev {
 };
 
 // the 15 Kronrod abscissae on [-1, 1] with the Kronrod and Gauss weights
-// (passed to the integrand kernel by value)
+// (in device memory for the integrand kernel)
 struct GkNodes {
     double x[15];
     double kw[15];

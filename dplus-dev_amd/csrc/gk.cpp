@@ -26,7 +26,7 @@ namespace dplus {
 void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
                         const double4 *trans, int n, const double *qxyz, double *out, hipStream_t s);
 void launch_docked_gk(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
-                      const double4 *trans, long long ntasks, const double4 *tasks, const GkNodes &nodes,
+                      const double4 *trans, long long ntasks, const double4 *tasks, const GkNodes *nodes,
                       double *out, hipStream_t s);
 
 // Gauss-Kronrod 15/7 tables (Common/mathfuncs.h:100-125)
@@ -157,6 +157,8 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         nodes.kw[i] = kW[i];
     }
     for (int i = 0; i < 7; i++) nodes.gw[i] = gW[i];
+    DevBuf bnodes;  // in device memory: the kernel indexes it per lane
+    bnodes.upload(&nodes, sizeof nodes, s);
     std::vector<double4> tasks;
     DevBuf btask;
     const int hw = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
@@ -208,7 +210,7 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
             KTimer kt("gk", s);
             launch_docked_gk(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
                              (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (long long)open.size(),
-                             (const double4 *)btask.p, nodes, (double *)bo.p, s);
+                             (const double4 *)btask.p, (const GkNodes *)bnodes.p, (double *)bo.p, s);
             check_hip(hipGetLastError(), "gk launch");
             vals.resize(nout);
             check_hip(hipMemcpyAsync(vals.data(), bo.p, nout * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");

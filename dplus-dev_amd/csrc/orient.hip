@@ -532,7 +532,7 @@ __global__ __launch_bounds__(1024) void vegas_reduce(const GroupDev *__restrict_
 // ------------------------------------------------------------------ single points
 // GetSphr (Grid.cpp:487-548): the CPU-path lookup used by getAmplitude and GK
 template <typename T, typename T4>
-__device__ cplx<T> get_sphr(const T4 *__restrict__ g, int N, double step, double rr, double th,
+__device__ __forceinline__ cplx<T> get_sphr(const T4 *__restrict__ g, int N, double step, double rr, double th,
                             double ph) {
     const T4 o = g[0];
     if (rr == 0.0) return {o.x, o.y};
@@ -575,7 +575,7 @@ __device__ __forceinline__ cplx<T> get_cart(const T4 *__restrict__ g, int N, dou
 
 // batch GetSphr at (q, theta, phi) triples
 template <typename T, typename T4>
-__global__ void sphr_batch(const T4 *__restrict__ g, int N, double step, int n,
+__global__ __launch_bounds__(256) void sphr_batch(const T4 *__restrict__ g, int N, double step, int n,
                            const double *__restrict__ q, const double *__restrict__ th,
                            const double *__restrict__ ph, double2 *__restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -587,7 +587,7 @@ __global__ void sphr_batch(const T4 *__restrict__ g, int N, double step, int n,
 // |A|^2 of JacobianSphereGrid::CalculateIntensity's samples (Grid.cpp:421-466):
 // phi[i] and v2[i] in [0,2), theta = acos(v2-1), lookup through GetCart
 template <typename T, typename T4>
-__global__ void seeded_samples(const T4 *__restrict__ g, int N, double step, double q, long long n,
+__global__ __launch_bounds__(256) void seeded_samples(const T4 *__restrict__ g, int N, double step, double q, long long n,
                                const double *__restrict__ phi, const double *__restrict__ v2,
                                double *__restrict__ out) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -629,7 +629,7 @@ __device__ double docked_intensity(const T4 *const *__restrict__ grids, const Le
 }
 
 template <typename T, typename T4>
-__global__ void docked_cart_batch(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
+__global__ __launch_bounds__(256) void docked_cart_batch(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
                                   const RotDev *__restrict__ rots, int nrot,
                                   const double4 *__restrict__ trans, int n,
                                   const double *__restrict__ qxyz, double *__restrict__ out) {
@@ -653,14 +653,14 @@ template <typename T, typename T4>
 __global__ __launch_bounds__(15 * kGkTasksPerBlock) void docked_gk_batch(
     const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo, const RotDev *__restrict__ rots,
     int nrot, const double4 *__restrict__ trans, long long ntasks, const double4 *__restrict__ tasks,
-    GkNodes nodes, double *__restrict__ out) {
+    const GkNodes *__restrict__ nodes, double *__restrict__ out) {
     __shared__ double val[15 * kGkTasksPerBlock];
     const long long t0 = (long long)blockIdx.x * kGkTasksPerBlock;
     const long long t = t0 + threadIdx.x / 15;
     double I = 0.0;
     if (t < ntasks) {
         const double4 tk = tasks[t];
-        const double xi = tk.z + tk.w * nodes.x[threadIdx.x % 15];
+        const double xi = tk.z + tk.w * nodes->x[threadIdx.x % 15];
         const double q[3] = {tk.x * cos(xi), tk.x * sin(xi), tk.y};
         I = docked_intensity<T, T4>(grids, geo, rots, nrot, trans, q);
     }
@@ -673,8 +673,8 @@ __global__ __launch_bounds__(15 * kGkTasksPerBlock) void docked_gk_batch(
         double kv = 0.0, gv = 0.0;
         for (int k = 0; k < 15; k++) {
             const double p = __dmul_rn(v[k], hw);
-            kv = __dadd_rn(kv, __dmul_rn(p, nodes.kw[k]));
-            if (k % 2 == 1) gv = __dadd_rn(gv, __dmul_rn(p, nodes.gw[k / 2]));
+            kv = __dadd_rn(kv, __dmul_rn(p, nodes->kw[k]));
+            if (k % 2 == 1) gv = __dadd_rn(gv, __dmul_rn(p, nodes->gw[k / 2]));
         }
         out[2 * ts] = kv;
         out[2 * ts + 1] = gv;
@@ -742,7 +742,7 @@ void launch_seeded_samples(bool fp64, const void *g, int N, double step, double 
 }
 
 void launch_docked_gk(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
-                      const double4 *trans, long long ntasks, const double4 *tasks, const GkNodes &nodes,
+                      const double4 *trans, long long ntasks, const double4 *tasks, const GkNodes *nodes,
                       double *out, hipStream_t s) {
     const unsigned nb = (unsigned)((ntasks + kGkTasksPerBlock - 1) / kGkTasksPerBlock);
     const int bs = 15 * kGkTasksPerBlock;
@@ -817,7 +817,7 @@ __device__ inline void lattice_mul(double d, double n, double &re, double &im) {
 }
 
 template <typename T, typename T4>
-__global__ void dock_grid_kernel(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
+__global__ __launch_bounds__(256) void dock_grid_kernel(const T4 *const *__restrict__ grids, const LeafGeo *__restrict__ geo,
                                  const RotDev *__restrict__ rots, int nrot, const double4 *__restrict__ trans,
                                  T4 *__restrict__ out, double step_out, long long P, double scale,
                                  LatticeDev lat) {
