@@ -599,11 +599,11 @@ void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s) 
 }
 
 // ------------------------------------------------------------------ orientation averages
-void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
+void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
                const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
                double step, uint64_t seed, long long dir0, int nblk, int chunk, double *partial, double *out,
                hipStream_t s);
-void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
+void launch_vegas_iter(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
                        const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
                        double step, uint64_t seed, int iter, int ppb, double *bounds, double *binsum,
                        double *binvar, double *hist, int nq, hipStream_t s);
@@ -761,6 +761,9 @@ void DockSet::finish(std::vector<Grid *> &gl, std::vector<RotDev> &rots, std::ve
     }
     if (tr.empty()) tr.push_back(make_double4(0, 0, 0, 0));
     nrot = (int)rots.size();
+    static const double eye[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    single = nrot == 1 && rots[0].nt == 1 && rots[0].t0 == 0 && tr.size() == 1 && tr[0].x == 0.0 &&
+             tr[0].y == 0.0 && tr[0].z == 0.0 && std::equal(eye, eye + 9, rots[0].m);
     grids.upload(gp.data(), gp.size() * sizeof(void *), s);
     geos.upload(geo.data(), geo.size() * sizeof(LeafGeo), s);
     rotb.upload(rots.data(), std::max<size_t>(1, rots.size()) * sizeof(RotDev), s);
@@ -822,7 +825,7 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
             bout.ensure((size_t)ng * mstride * sizeof(double));
             {
                 KTimer kt("mc", s);
-                launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                launch_mc(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
                           (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
                           (const double *)bq.p, D.step, p->seed, 0, nblk, chunk, (double *)bpart.p, (double *)bout.p,
                           s);
@@ -848,7 +851,7 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                 bact.upload(active.data(), active.size() * sizeof(int), s);
                 {
                     KTimer kt("mc", s);
-                    launch_mc(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                    launch_mc(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
                               (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
                               (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k * chunk, nblk,
                               chunk, (double *)bpart.p, (double *)bout.p, s);
@@ -901,7 +904,7 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
             if (k > 0) bact.upload(active.data(), active.size() * sizeof(int), s);
             {
                 KTimer kt("vegas", s);
-                launch_vegas_iter(D.fp64, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                launch_vegas_iter(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
                                   (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
                                   (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k, ppb,
                                   (double *)bb.p, (double *)bsum.p, (double *)bvar.p, (double *)bh.p, nq, s);

@@ -128,6 +128,9 @@ struct DockSet {
     int grid_size = 0, nrot = 0;
     double qmax = 0, step = 0;
     bool fp64 = false;
+    // one leaf placed once, unrotated, at the origin: the orientation-average
+    // kernels look up the drawn angles directly
+    bool single = false;
     DevBuf grids, geos, rotb, trb;
     DockSet(int nleaf, const dplus_leaf *leaves, hipStream_t s);
     // placements sharing grid, rotation and scale become one rotation record

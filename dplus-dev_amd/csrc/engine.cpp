@@ -40,7 +40,7 @@
 
 namespace dplus {
 
-void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
+void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
                const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
                double step, uint64_t seed, long long dir0, int nblk, int chunk, double *partial, double *out,
                hipStream_t s);
@@ -600,7 +600,7 @@ struct Session {
         check_hip(hipMemcpyAsync(brot.p, h_rot, cap_rot * sizeof(RotDev), hipMemcpyHostToDevice, s), "H2D");
         check_hip(hipMemcpyAsync(btr.p, h_tr, cap_tr * sizeof(double4), hipMemcpyHostToDevice, s), "H2D");
         check_hip(hipMemcpyAsync(bparam.p, h_param, 2 * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
-        launch_mc(fp64, (const void *const *)bgrids.p, (const RotDev *)brot.p, nrot, (const double4 *)btr.p,
+        launch_mc(fp64, false, (const void *const *)bgrids.p, (const RotDev *)brot.p, nrot, (const double4 *)btr.p,
                   (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride, (const double *)bq.p, step, seed, 0, nblk,
                   chunk, (double *)bpart.p, (double *)bout.p, s);
         launch_mc_finish((const GroupDev *)bg.p, ng, mstride, (const double *)bout.p, (double)total, (double *)bI.p, s);

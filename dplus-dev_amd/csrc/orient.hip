@@ -199,6 +199,27 @@ __device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ g
     }
 }
 
+// the same for one leaf placed once, unrotated, at the origin: the lookup takes
+// the direction's angles as drawn (no rotation back to angles, no translation
+// phases), a = scale * A(q, theta, phi)
+template <typename T, typename T4, int M = kMaxM>
+__device__ __forceinline__ void single_amplitude(const T4 *__restrict__ grid, T sc, int layer, double step,
+                                                 double th, double ph, const double *__restrict__ qs, int m,
+                                                 cplx<T> acc[M]) {
+    const double inv_step = 1.0 / step;
+    cplx<T> amps[4], d1, d2;
+    group_planes<T, T4>(grid, layer, th, ph, amps, d1, d2);
+#pragma unroll
+    for (int j = 0; j < M; j++) {
+        if (j < m) {
+            const T t = (T)((qs[j] - layer * step) * inv_step);
+            cplx<T> a = hermite<T>(amps[1], amps[2], d1, d2, t);
+            acc[j].x += a.x * sc;
+            acc[j].y += a.y * sc;
+        }
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -208,7 +229,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ------------------------------------------------------------------ MC
 // grid: x = direction block, y = active group.  mstride >= every group's m.
 // partial[(gi * nblk + bx) * mstride + j] = sum over the block's directions of |A_j|^2
-template <typename T, typename T4>
+template <typename T, typename T4, bool kSingle>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T4 *const *__restrict__ grids,
                                                  const RotDev *__restrict__ rots, int nrot,
                                                  const double4 *__restrict__ trans,
@@ -224,10 +245,15 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T
     uniform_pair(seed, (uint32_t)d, (uint32_t)(d >> 32), (uint32_t)G.layer, kStreamMC, u, v);
     mc_stratify(d, chunk, nu, nv, u, v);
     const double th = acos(2.0 * u - 1.0), ph = k2Pi * v;
-    double st, ct, sp, cp;
-    sincos(th, &st, &ct);
-    sincos(ph, &sp, &cp);
-    const double qv[3] = {cp * st, sp * st, ct};
+    double qv[3] = {0.0, 0.0, 0.0};
+    if (!kSingle) {
+        double st, ct, sp, cp;
+        sincos(th, &st, &ct);
+        sincos(ph, &sp, &cp);
+        qv[0] = cp * st;
+        qv[1] = sp * st;
+        qv[2] = ct;
+    }
     __shared__ double red[4][kMaxM];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int jb = 0; jb < G.m; jb += kMaxM) {
@@ -235,7 +261,11 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T
         cplx<T> acc[kMaxM];
 #pragma unroll
         for (int j = 0; j < kMaxM; j++) acc[j] = {(T)0, (T)0};
-        docked_amplitude<T, T4>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, acc);
+        if (kSingle)
+            single_amplitude<T, T4>(grids[rots[0].leaf], (T)rots[0].scale, G.layer, step, th, ph, qs + G.q0 + jb, m,
+                                    acc);
+        else
+            docked_amplitude<T, T4>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, acc);
 #pragma unroll
         for (int j = 0; j < kMaxM; j++) {
             double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
@@ -354,7 +384,7 @@ __device__ __forceinline__ double lanes_sum(double v, int L) {
     return v;
 }
 
-template <typename T, typename T4, int kL>  // kL: lanes per bin fixed at compile time, or 0 for L
+template <typename T, typename T4, int kL, bool kSingle>  // kL: lanes per bin fixed at compile time, or 0 for L
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(const T4 *const *__restrict__ grids,
                                                     const RotDev *__restrict__ rots, int nrot,
                                                     const double4 *__restrict__ trans,
@@ -385,31 +415,69 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
         if (p < ppb) uniform_pair(seed, (uint32_t)(bin * ppb + p), (uint32_t)iter, (uint32_t)G.layer, kStreamVegas, u, v);
         const double th = acos(1. - 2. * (u * (bt[ti + 1] - bt[ti]) + bt[ti]));
         const double ph = (v * (bp[pj + 1] - bp[pj]) + bp[pj]) * 2. * kPi;
-        double st, ct, sp, cp;
-        sincos(th, &st, &ct);
-        sincos(ph, &sp, &cp);
-        qv[k][0] = cp * st;
-        qv[k][1] = sp * st;
-        qv[k][2] = ct;
+        if (kSingle) {  // the lookup takes the angles themselves
+            qv[k][0] = th;
+            qv[k][1] = ph;
+            qv[k][2] = 0.0;
+        } else {
+            double st, ct, sp, cp;
+            sincos(th, &st, &ct);
+            sincos(ph, &sp, &cp);
+            qv[k][0] = cp * st;
+            qv[k][1] = sp * st;
+            qv[k][2] = ct;
+        }
     }
     for (int jb = 0; jb < G.m; jb += kVM) {
         const int m = min(kVM, G.m - jb);
         double e[kVM];
 #pragma unroll
         for (int j = 0; j < kVM; j++) e[j] = 0.0;
+        if (kSingle) {
+            // one point at a time (a rolled loop: unrolled, the points' gathers would be
+            // interleaved past the register budget); its angles and sum by selection
+#pragma unroll 1
+            for (int k = 0; k < kVPts; k++) {
+                if (pl + L * k >= ppb) break;
+                double th = qv[0][0], ph = qv[0][1];
 #pragma unroll
-        for (int k = 0; k < kVPts; k++) {
-            if (pl + L * k >= ppb) continue;
-            cplx<T> acc[kVM];
+                for (int kk = 1; kk < kVPts; kk++)
+                    if (kk == k) {
+                        th = qv[kk][0];
+                        ph = qv[kk][1];
+                    }
+                cplx<T> acc[kVM];
 #pragma unroll
-            for (int j = 0; j < kVM; j++) acc[j] = {(T)0, (T)0};
-            docked_amplitude<T, T4, kVM>(grids, rots, nrot, trans, G.layer, step, qv[k], qs + G.q0 + jb, m, acc);
+                for (int j = 0; j < kVM; j++) acc[j] = {(T)0, (T)0};
+                single_amplitude<T, T4, kVM>(grids[rots[0].leaf], (T)rots[0].scale, G.layer, step, th, ph,
+                                             qs + G.q0 + jb, m, acc);
 #pragma unroll
-            for (int j = 0; j < kVM; j++) {
-                const double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
-                if (j < m) {
-                    e[j] += I;
-                    so[k] += I;
+                for (int j = 0; j < kVM; j++) {
+                    const double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
+                    if (j < m) {
+                        e[j] += I;
+#pragma unroll
+                        for (int kk = 0; kk < kVPts; kk++)
+                            if (kk == k) so[kk] += I;
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kVPts; k++) {
+                if (pl + L * k >= ppb) continue;
+                cplx<T> acc[kVM];
+#pragma unroll
+                for (int j = 0; j < kVM; j++) acc[j] = {(T)0, (T)0};
+                docked_amplitude<T, T4, kVM>(grids, rots, nrot, trans, G.layer, step, qv[k], qs + G.q0 + jb, m,
+                                             acc);
+#pragma unroll
+                for (int j = 0; j < kVM; j++) {
+                    const double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
+                    if (j < m) {
+                        e[j] += I;
+                        so[k] += I;
+                    }
                 }
             }
         }
@@ -682,25 +750,29 @@ __global__ __launch_bounds__(15 * kGkTasksPerBlock) void docked_gk_batch(
 }
 
 // ------------------------------------------------------------------ launchers
-void launch_mc(bool fp64, const void *const *grids, const RotDev *rots, int nrot,
+void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot,
                const double4 *trans, const GroupDev *groups, const int *active, int nactive,
                int mstride, const double *qs, double step, uint64_t seed, long long dir0, int nblk, int chunk,
                double *partial, double *out, hipStream_t s) {
     dim3 grid(nblk, nactive);
     int nu, nv;
     mc_strata(chunk, nu, nv);
-    if (fp64)
-        mc_kernel<double, double4><<<grid, 256, 0, s>>>((const double4 *const *)grids, rots, nrot, trans,
-                                                         groups, active, mstride, qs, step, seed, dir0,
-                                                         nblk, chunk, nu, nv, partial);
+#define DPLUS_MC(T, T4, S)                                                                                      \
+    mc_kernel<T, T4, S><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups, active, mstride, \
+                                             qs, step, seed, dir0, nblk, chunk, nu, nv, partial)
+    if (fp64 && single)
+        DPLUS_MC(double, double4, true);
+    else if (fp64)
+        DPLUS_MC(double, double4, false);
+    else if (single)
+        DPLUS_MC(float, float4, true);
     else
-        mc_kernel<float, float4><<<grid, 256, 0, s>>>((const float4 *const *)grids, rots, nrot, trans,
-                                                       groups, active, mstride, qs, step, seed, dir0,
-                                                       nblk, chunk, nu, nv, partial);
+        DPLUS_MC(float, float4, false);
+#undef DPLUS_MC
     mc_reduce<<<nactive, 64, 0, s>>>(groups, active, partial, nblk, mstride, out);
 }
 
-void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, int nrot,
+void launch_vegas_iter(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot,
                        const double4 *trans, const GroupDev *groups, const int *active, int nactive,
                        int mstride, const double *qs, double step, uint64_t seed, int iter, int ppb,
                        double *bounds, double *binsum, double *binvar, double *hist, int nq, hipStream_t s) {
@@ -708,17 +780,24 @@ void launch_vegas_iter(bool fp64, const void *const *grids, const RotDev *rots, 
     int L = 1;
     while (L < ppb && L < 64) L <<= 1;
     dim3 grid(kVB * kVB / (4 * (64 / L)), nactive);
-#define DPLUS_VEGAS(T, T4, KL)                                                                                 \
-    vegas_sample<T, T4, KL><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups, active, \
-                                                 mstride, qs, step, seed, iter, ppb, L, bounds, binsum, binvar)
-    if (fp64 && L == 64)
-        DPLUS_VEGAS(double, double4, 64);
-    else if (fp64)
-        DPLUS_VEGAS(double, double4, 0);
-    else if (L == 64)
-        DPLUS_VEGAS(float, float4, 64);
-    else
-        DPLUS_VEGAS(float, float4, 0);
+#define DPLUS_VEGAS(T, T4, KL, S)                                                                           \
+    vegas_sample<T, T4, KL, S><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups, active, \
+                                                    mstride, qs, step, seed, iter, ppb, L, bounds, binsum, binvar)
+#define DPLUS_VEGAS_L(T, T4, S)       \
+    if (L == 64)                      \
+        DPLUS_VEGAS(T, T4, 64, S);    \
+    else                              \
+        DPLUS_VEGAS(T, T4, 0, S);
+    if (fp64 && single) {
+        DPLUS_VEGAS_L(double, double4, true)
+    } else if (fp64) {
+        DPLUS_VEGAS_L(double, double4, false)
+    } else if (single) {
+        DPLUS_VEGAS_L(float, float4, true)
+    } else {
+        DPLUS_VEGAS_L(float, float4, false)
+    }
+#undef DPLUS_VEGAS_L
 #undef DPLUS_VEGAS
     vegas_reduce<<<nactive, 1024, 0, s>>>(groups, active, mstride, ppb, iter, binsum, binvar, bounds, hist, nq);
 }
