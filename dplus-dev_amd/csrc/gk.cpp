@@ -16,6 +16,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
 #include <thread>
 #include <vector>
 
@@ -81,6 +86,7 @@ struct ThetaTask {
     long long maxd, mind;
     int node;          // node in the theta tree of q
     double phival[15]; // phi integrals of the 15 theta nodes
+    double qs[15], qc[15];  // q sin(theta_k), q cos(theta_k) of the 15 theta nodes
     int left = 15;
 };
 
@@ -91,6 +97,111 @@ struct PhiTask {
     int node;  // node in the phi tree
     int tree;  // which phi tree (theta_task * 15 + k)
 };
+
+// growable array of phi tasks without value-initialisation (millions per level)
+struct TaskBuf {
+    std::unique_ptr<PhiTask[]> p;
+    size_t n = 0, cap = 0;
+    void reserve(size_t c) {
+        if (c <= cap) return;
+        c = std::max(c, 2 * cap);
+        std::unique_ptr<PhiTask[]> q(new PhiTask[c]);
+        if (n) std::copy(p.get(), p.get() + n, q.get());
+        p.swap(q);
+        cap = c;
+    }
+    void push_back(const PhiTask &t) {
+        reserve(n + 1);
+        p[n++] = t;
+    }
+    PhiTask &operator[](size_t i) { return p[i]; }
+    const PhiTask &operator[](size_t i) const { return p[i]; }
+    bool empty() const { return n == 0; }
+    size_t size() const { return n; }
+};
+
+// pinned host staging buffer (grows; DMA without a pageable bounce)
+template <typename V>
+struct Pinned {
+    V *p = nullptr;
+    size_t cap = 0;
+    V *get(size_t n) {
+        if (n > cap) {
+            std::shared_lock<std::shared_mutex> lk(capture_mutex());
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            cap = 0;
+            check_hip(hipHostMalloc((void **)&p, std::max<size_t>(n, 1) * sizeof(V), hipHostMallocDefault),
+                      "hipHostMalloc");
+            cap = n;
+        }
+        return p;
+    }
+    ~Pinned() {
+        if (p) {
+            std::shared_lock<std::shared_mutex> lk(capture_mutex());
+            (void)hipHostFree(p);
+        }
+    }
+};
+
+// worker threads that run fn(0..n-1) with the caller taking part
+class Pool {
+   public:
+    explicit Pool(int n) {
+        for (int i = 1; i < n; i++) th_.emplace_back([this] { loop(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    int size() const { return (int)th_.size() + 1; }
+    void run(int n, const std::function<void(int)> &fn) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &fn;
+            njob_ = n;
+            next_ = 0;
+            remaining_ = n;
+        }
+        cv_.notify_all();
+        std::unique_lock<std::mutex> lk(mu_);
+        while (next_ < njob_) {
+            const int k = next_++;
+            lk.unlock();
+            fn(k);
+            lk.lock();
+            --remaining_;
+        }
+        done_.wait(lk, [&] { return remaining_ == 0; });
+        job_ = nullptr;
+    }
+
+   private:
+    void loop() {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [&] { return stop_ || (job_ && next_ < njob_); });
+            if (stop_) return;
+            const int k = next_++;
+            const std::function<void(int)> *fn = job_;
+            lk.unlock();
+            (*fn)(k);
+            lk.lock();
+            if (--remaining_ == 0) done_.notify_all();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *job_ = nullptr;
+    int njob_ = 0, next_ = 0, remaining_ = 0;
+    bool stop_ = false;
+};
 }  // namespace
 
 void gk_average(const DockSet &D, int nq, const double *q, double eps, long long max_depth, double *out,
@@ -98,7 +209,7 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
     std::vector<Tree> thetaTrees(nq);
     std::vector<ThetaTask> tt;
     std::vector<Tree> phiTrees;
-    std::vector<PhiTask> open;
+    TaskBuf open, next;
     long long evals = 0;
     std::vector<int> zero_q;
 
@@ -110,6 +221,12 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         t.maxd = maxd;
         t.mind = mind;
         t.node = node;
+        const double ba2 = (b - a) * 0.5, apb2 = (b + a) * 0.5;
+        for (int k = 0; k < 15; k++) {
+            const double th = apb2 + ba2 * kX[k];
+            t.qs[k] = q[qi] * std::sin(th);
+            t.qc[k] = q[qi] * std::cos(th);
+        }
         tt.push_back(t);
         const int id = (int)tt.size() - 1;
         for (int k = 0; k < 15; k++) {
@@ -129,25 +246,19 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         open_theta(i, 0.0, kPi / 2.0, max_depth - 1, -1, root);
     }
     // q = 0: |A(0)|^2 (Amplitude.cpp:1479-1484)
-    std::vector<double> pts;
-    std::vector<double> vals;
-    DevBuf bq, bo;
-    auto evaluate = [&](size_t npts) {
-        bq.upload(pts.data(), npts * 3 * sizeof(double), s);
-        bo.ensure(npts * sizeof(double));
+    if (!zero_q.empty()) {
+        std::vector<double> pts(3 * zero_q.size(), 0.0), vals(zero_q.size());
+        DevBuf bq, bo;
+        bq.upload(pts.data(), pts.size() * sizeof(double), s);
+        bo.ensure(vals.size() * sizeof(double));
         KTimer kt("gk", s);
         launch_docked_cart(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
-                           (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (int)npts,
+                           (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (int)vals.size(),
                            (const double *)bq.p, (double *)bo.p, s);
         check_hip(hipGetLastError(), "gk launch");
-        vals.resize(npts);
-        check_hip(hipMemcpyAsync(vals.data(), bo.p, npts * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        check_hip(hipMemcpyAsync(vals.data(), bo.p, vals.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         check_hip(hipStreamSynchronize(s), "gk batch");
-        evals += (long long)npts;
-    };
-    if (!zero_q.empty()) {
-        pts.assign(3 * zero_q.size(), 0.0);
-        evaluate(zero_q.size());
+        evals += (long long)vals.size();
         for (size_t k = 0; k < zero_q.size(); k++) out[zero_q[k]] = vals[k];
     }
 
@@ -157,15 +268,16 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         nodes.kw[i] = kW[i];
     }
     for (int i = 0; i < 7; i++) nodes.gw[i] = gW[i];
-    DevBuf bnodes;  // in device memory: the kernel indexes it per lane
+    DevBuf bnodes, btask, bo;  // nodes in device memory: the kernel indexes them per lane
     bnodes.upload(&nodes, sizeof nodes, s);
-    std::vector<double4> tasks;
-    DevBuf btask;
-    const int hw = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    Pinned<double4> htask;
+    Pinned<double> hval;
+    Pool pool(std::max(1, std::min(16, (int)std::thread::hardware_concurrency())));
+    std::vector<unsigned char> split;
     // [begin, end) ranges of `open` for the host threads, cut at theta-task boundaries
     auto ranges = [&]() {
         const size_t n = open.size();
-        const int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)hw, n / 8192));
+        const int nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)pool.size(), n / 8192));
         std::vector<size_t> cut(1, 0);
         for (int k = 1; k < nt; k++) {
             size_t b = std::max(cut.back(), n * (size_t)k / (size_t)nt);
@@ -175,107 +287,106 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         cut.push_back(n);
         return cut;
     };
-    auto parallel = [&](const std::vector<size_t> &cut, const auto &fn) {
-        const int nt = (int)cut.size() - 1;
-        if (nt == 1) {
-            fn(0, cut[0], cut[1]);
-            return;
-        }
-        std::vector<std::thread> th;
-        for (int k = 1; k < nt; k++) th.emplace_back([&, k] { fn(k, cut[k], cut[k + 1]); });
-        fn(0, cut[0], cut[1]);
-        for (auto &t : th) t.join();
-    };
     while (!open.empty()) {
         const std::vector<size_t> cut = ranges();
+        const int nt = (int)cut.size() - 1;
+        const size_t n = open.size();
         // integrand at the 15 Kronrod nodes of every open phi interval: the
         // interval's task (q sin theta, q cos theta, centre, half width)
-        tasks.resize(open.size());
-        parallel(cut, [&](int, size_t b, size_t e) {
-            for (size_t t = b; t < e; t++) {
+        double4 *tasks = htask.get(n);
+        pool.run(nt, [&](int part) {
+            for (size_t t = cut[part]; t < cut[part + 1]; t++) {
                 const PhiTask &P = open[t];
                 const ThetaTask &T = tt[P.theta_task];
-                const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
-                const double th = apb2 + ba2 * kX[P.k];
-                const double qq = q[T.q];
-                tasks[t] = make_double4(qq * std::sin(th), qq * std::cos(th), (P.pmax + P.pmin) * 0.5,
-                                        (P.pmax - P.pmin) * 0.5);
+                tasks[t] = make_double4(T.qs[P.k], T.qc[P.k], (P.pmax + P.pmin) * 0.5, (P.pmax - P.pmin) * 0.5);
             }
         });
+        // the device returns each interval's Kronrod and Gauss sums
+        double *vals = hval.get(2 * n);
         {
-            // the device returns each interval's Kronrod and Gauss sums
-            const size_t npts = open.size() * 15, nout = open.size() * 2;
-            btask.upload(tasks.data(), tasks.size() * sizeof(double4), s);
-            bo.ensure(nout * sizeof(double));
+            btask.ensure(n * sizeof(double4));
+            bo.ensure(2 * n * sizeof(double));
+            check_hip(hipMemcpyAsync(btask.p, tasks, n * sizeof(double4), hipMemcpyHostToDevice, s), "H2D");
             KTimer kt("gk", s);
             launch_docked_gk(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
-                             (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (long long)open.size(),
+                             (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (long long)n,
                              (const double4 *)btask.p, (const GkNodes *)bnodes.p, (double *)bo.p, s);
             check_hip(hipGetLastError(), "gk launch");
-            vals.resize(nout);
-            check_hip(hipMemcpyAsync(vals.data(), bo.p, nout * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check_hip(hipMemcpyAsync(vals, bo.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
             check_hip(hipStreamSynchronize(s), "gk batch");
-            evals += (long long)npts;
+            evals += 15 * (long long)n;
         }
-        const int nt = (int)cut.size() - 1;
-        std::vector<std::vector<PhiTask>> next_part(nt);
-        std::vector<std::vector<int>> fin_part(nt);
-        parallel(cut, [&](int part, size_t b, size_t e) {
-            std::vector<PhiTask> &next = next_part[part];
-            std::vector<int> &finished_theta = fin_part[part];
-            for (size_t t = b; t < e; t++) {
+        // pass 1: accept or split each interval, and the split count of every range
+        split.resize(n);
+        std::vector<size_t> nsplit(nt + 1, 0);
+        pool.run(nt, [&](int part) {
+            size_t c = 0;
+            for (size_t t = cut[part]; t < cut[part + 1]; t++) {
                 const PhiTask &P = open[t];
-                const double apb2 = (P.pmax + P.pmin) * 0.5;
                 const double kv = vals[2 * t], gv = vals[2 * t + 1];
+                const bool accept = P.mind < 0 && (P.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv);
+                split[t] = accept ? 0 : 1;
+                c += accept ? 0 : 1;
+            }
+            nsplit[part + 1] = c;
+        });
+        for (int k = 0; k < nt; k++) nsplit[k + 1] += nsplit[k];
+        // pass 2: resolve accepted intervals up their trees; write the halves of
+        // split ones at their place in the next level (range offsets keep the order)
+        next.n = 0;
+        next.reserve(2 * nsplit[nt]);
+        next.n = 2 * nsplit[nt];
+        std::vector<std::vector<int>> fin_part(nt);
+        pool.run(nt, [&](int part) {
+            size_t w = 2 * nsplit[part];
+            std::vector<int> &finished_theta = fin_part[part];
+            for (size_t t = cut[part]; t < cut[part + 1]; t++) {
+                const PhiTask &P = open[t];
                 Tree &pt = phiTrees[P.tree];
-                if (P.mind < 0 && (P.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv)) {
-                    const int r = pt.resolve(P.node, kv);
+                if (!split[t]) {
+                    const int r = pt.resolve(P.node, vals[2 * t]);
                     if (r >= 0) {
                         ThetaTask &T = tt[P.theta_task];
                         T.phival[P.k] = pt.n[r].value;
                         if (--T.left == 0) finished_theta.push_back(P.theta_task);
                     }
                 } else {
+                    const double apb2 = (P.pmax + P.pmin) * 0.5;
                     const int l = pt.add(P.node), rgt = pt.add(P.node);
                     pt.n[P.node].child[0] = l;
                     pt.n[P.node].child[1] = rgt;
                     pt.n[P.node].pending = 2;
-                    next.push_back({P.theta_task, P.k, P.pmin, apb2, P.maxd - 1, P.mind - 1, l, P.tree});
-                    next.push_back({P.theta_task, P.k, apb2, P.pmax, P.maxd - 1, P.mind - 1, rgt, P.tree});
+                    next[w++] = {P.theta_task, P.k, P.pmin, apb2, P.maxd - 1, P.mind - 1, l, P.tree};
+                    next[w++] = {P.theta_task, P.k, apb2, P.pmax, P.maxd - 1, P.mind - 1, rgt, P.tree};
                 }
             }
         });
-        std::vector<PhiTask> next;
-        std::vector<int> finished_theta;
-        for (int k = 0; k < nt; k++) {
-            next.insert(next.end(), next_part[k].begin(), next_part[k].end());
-            finished_theta.insert(finished_theta.end(), fin_part[k].begin(), fin_part[k].end());
-        }
-        open.swap(next);
-        // theta intervals whose 15 phi integrals are complete
-        for (int id : finished_theta) {
-            const ThetaTask T = tt[id];
-            const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
-            double kv = 0.0, gv = 0.0;
-            for (int i = 0; i < 15; i++) {
-                const double sT = std::sin(apb2 + ba2 * kX[i]);
-                kv += T.phival[i] * kW[i] * ba2 * sT;
-                if (i % 2 == 1) gv += T.phival[i] * gW[i / 2] * ba2 * sT;
+        std::swap(open, next);
+        // theta intervals whose 15 phi integrals are complete, in interval order
+        for (int part = 0; part < nt; part++)
+            for (int id : fin_part[part]) {
+                const ThetaTask T = tt[id];
+                const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
+                double kv = 0.0, gv = 0.0;
+                for (int i = 0; i < 15; i++) {
+                    const double sT = std::sin(apb2 + ba2 * kX[i]);
+                    kv += T.phival[i] * kW[i] * ba2 * sT;
+                    if (i % 2 == 1) gv += T.phival[i] * gW[i / 2] * ba2 * sT;
+                }
+                Tree &tr = thetaTrees[T.q];
+                if (T.mind < 0 && (T.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv)) {
+                    const int r = tr.resolve(T.node, kv);
+                    if (r >= 0) out[T.q] = tr.n[r].value / (2.0 * kPi);
+                } else {
+                    const double mid = (T.tmax + T.tmin) / 2.0;
+                    const int l = tr.add(T.node), rgt = tr.add(T.node);
+                    tr.n[T.node].child[0] = l;
+                    tr.n[T.node].child[1] = rgt;
+                    tr.n[T.node].pending = 2;
+                    open_theta(T.q, T.tmin, mid, T.maxd - 1, T.mind - 1, l);
+                    open_theta(T.q, mid, T.tmax, T.maxd - 1, T.mind - 1, rgt);
+                }
             }
-            Tree &tr = thetaTrees[T.q];
-            if (T.mind < 0 && (T.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv)) {
-                const int r = tr.resolve(T.node, kv);
-                if (r >= 0) out[T.q] = tr.n[r].value / (2.0 * kPi);
-            } else {
-                const double mid = (T.tmax + T.tmin) / 2.0;
-                const int l = tr.add(T.node), rgt = tr.add(T.node);
-                tr.n[T.node].child[0] = l;
-                tr.n[T.node].child[1] = rgt;
-                tr.n[T.node].pending = 2;
-                open_theta(T.q, T.tmin, mid, T.maxd - 1, T.mind - 1, l);
-                open_theta(T.q, mid, T.tmax, T.maxd - 1, T.mind - 1, rgt);
-            }
-        }
     }
     if (st) st->evaluations = evals;
 }
