@@ -95,7 +95,7 @@ struct PhiTask {
     double pmin, pmax;
     long long maxd, mind;
     int node;  // node in the phi tree
-    int tree;  // which phi tree (theta_task * 15 + k)
+    int tree;  // node pool of the phi trees (one per theta task, holding its 15 trees)
 };
 
 // growable array of phi tasks without value-initialisation (millions per level)
@@ -229,9 +229,10 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         }
         tt.push_back(t);
         const int id = (int)tt.size() - 1;
+        phiTrees.push_back(Tree());
+        Tree &pt = phiTrees.back();
+        pt.n.reserve(64);
         for (int k = 0; k < 15; k++) {
-            phiTrees.push_back(Tree());
-            Tree &pt = phiTrees.back();
             const int root = pt.add(-1);
             open.push_back({id, k, 0.0, k2Pi, maxd, mind, root, (int)phiTrees.size() - 1});
         }
@@ -318,10 +319,11 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         }
         // pass 1: accept or split each interval, and the split count of every range
         split.resize(n);
-        std::vector<size_t> nsplit(nt + 1, 0);
+        std::vector<size_t> nsplit(nt + 1, 0), nthetas(nt, 0);
         pool.run(nt, [&](int part) {
             size_t c = 0;
             for (size_t t = cut[part]; t < cut[part + 1]; t++) {
+                if (t == cut[part] || open[t].theta_task != open[t - 1].theta_task) nthetas[part]++;
                 const PhiTask &P = open[t];
                 const double kv = vals[2 * t], gv = vals[2 * t + 1];
                 const bool accept = P.mind < 0 && (P.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv);
@@ -333,8 +335,12 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         for (int k = 0; k < nt; k++) nsplit[k + 1] += nsplit[k];
         // pass 2: resolve accepted intervals up their trees; write the halves of
         // split ones at their place in the next level (range offsets keep the order)
+        // room for the new intervals of theta tasks that split after this level (30 each)
+        // so that appending them does not move the level's intervals
+        size_t nthe = 0;
+        for (int k = 0; k < nt; k++) nthe += nthetas[k];
         next.n = 0;
-        next.reserve(2 * nsplit[nt]);
+        next.reserve(2 * nsplit[nt] + 30 * nthe);
         next.n = 2 * nsplit[nt];
         std::vector<std::vector<int>> fin_part(nt);
         pool.run(nt, [&](int part) {
