@@ -27,7 +27,8 @@ sys.path.insert(0, os.path.join(ROOT, "dplus-dev_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "I(q) curves/s + p50/p99 latency, mixed vs FP64 GPU at ≤1e-4 rel err"
-PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector/matrix peak
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E  # MI355X published FP32 vector/matrix peak
 
 
 def log(*a):
@@ -436,7 +437,11 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": f"build_atoms_pairs<false, {shape['wg']}>", "launch_ms": t_launch * 1e3,
                      "terms_per_launch": terms, "flop_eq_per_term": flop_eq,
-                     "transcendental_weight": w_t},
+                     "transcendental_weight": w_t,
+                     # the same launch against the HBM roofline: PMC bytes per launch over its time
+                     "hbm": None if traffic is None else {
+                         "achieved": traffic / t_launch / 1e9, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
+                         "frac": traffic / t_launch / 1e9 / PEAK_HBM_GBPS}},
         "cpu_baseline": cpu,
         "serving": serving,
         "fit_sessions": sessions,
