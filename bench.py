@@ -27,8 +27,8 @@ sys.path.insert(0, os.path.join(ROOT, "dplus-dev_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 METRIC = "I(q) curves/s + p50/p99 latency, mixed vs FP64 GPU at ≤1e-4 rel err"
-PEAK_FP32_TFLOPS = 157.3
-PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E  # MI355X published FP32 vector/matrix peak
+PEAK_FP32_TFLOPS = 157.3  # MI355X published FP32 vector/matrix peak
+PEAK_HBM_GBPS = 8000.0  # MI355X HBM3E
 
 
 def log(*a):
