@@ -141,10 +141,10 @@ struct DockSet {
     void finish(std::vector<Grid *> &gl, std::vector<RotDev> &rots, std::vector<double4> &tr, hipStream_t s);
 };
 
-// run fn(i) for i in [0, n) on the host's cores (dynamic chunks)
+// run fn(i) for i in [0, n) on up to 16 host threads (dynamic chunks)
 template <typename F>
 void parallel_for(long long n, F fn) {
-    unsigned nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     if (n < 2 || nt == 1) {
         for (long long i = 0; i < n; i++) fn(i);
         return;
