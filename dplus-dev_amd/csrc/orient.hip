@@ -256,16 +256,41 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T
     }
     __shared__ double red[4][kMaxM];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (kSingle) {
+        // one lookup per direction serves every q of the group, each q reduced as it is formed
+        cplx<T> amps[4], d1, d2;
+        group_planes<T, T4>(grids[rots[0].leaf], G.layer, th, ph, amps, d1, d2);
+        const T sc = (T)rots[0].scale;
+        const double inv_step = 1.0 / step;
+        for (int jb = 0; jb < G.m; jb += kMaxM) {
+            const int m = min(kMaxM, G.m - jb);
+#pragma unroll
+            for (int j = 0; j < kMaxM; j++) {
+                if (j < m) {
+                    const T t = (T)((qs[G.q0 + jb + j] - G.layer * step) * inv_step);
+                    const cplx<T> a = hermite<T>(amps[1], amps[2], d1, d2, t);
+                    const T ax = a.x * sc, ay = a.y * sc;
+                    double I = (double)ax * (double)ax + (double)ay * (double)ay;
+                    I = wave_sum(I);
+                    if (lane == 0) red[w][j] = I;
+                }
+            }
+            __syncthreads();
+            if (threadIdx.x < m) {
+                const int j = threadIdx.x;
+                partial[((size_t)gi * nblk + blockIdx.x) * mstride + jb + j] =
+                    red[0][j] + red[1][j] + red[2][j] + red[3][j];
+            }
+            __syncthreads();
+        }
+        return;
+    }
     for (int jb = 0; jb < G.m; jb += kMaxM) {
         const int m = min(kMaxM, G.m - jb);
         cplx<T> acc[kMaxM];
 #pragma unroll
         for (int j = 0; j < kMaxM; j++) acc[j] = {(T)0, (T)0};
-        if (kSingle)
-            single_amplitude<T, T4>(grids[rots[0].leaf], (T)rots[0].scale, G.layer, step, th, ph, qs + G.q0 + jb, m,
-                                    acc);
-        else
-            docked_amplitude<T, T4>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, acc);
+        docked_amplitude<T, T4>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, acc);
 #pragma unroll
         for (int j = 0; j < kMaxM; j++) {
             double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
