@@ -453,37 +453,62 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
             qv[k][2] = ct;
         }
     }
+    // single leaf: each point's lookup (the two inner planes and the q-spline slopes)
+    // once, held for every q chunk of the group; points in a rolled loop, one
+    // lookup in flight (unrolled, the points' gathers would be interleaved past
+    // the register budget), their results kept by selection
+    T hold[kVPts][8];
+    if (kSingle) {
+#pragma unroll 1
+        for (int k = 0; k < kVPts; k++) {
+            if (pl + L * k >= ppb) break;
+            double th = qv[0][0], ph = qv[0][1];
+#pragma unroll
+            for (int kk = 1; kk < kVPts; kk++)
+                if (kk == k) {
+                    th = qv[kk][0];
+                    ph = qv[kk][1];
+                }
+            cplx<T> amps[4], d1, d2;
+            group_planes<T, T4>(grids[rots[0].leaf], G.layer, th, ph, amps, d1, d2);
+#pragma unroll
+            for (int kk = 0; kk < kVPts; kk++)
+                if (kk == k) {
+                    hold[kk][0] = amps[1].x;
+                    hold[kk][1] = amps[1].y;
+                    hold[kk][2] = amps[2].x;
+                    hold[kk][3] = amps[2].y;
+                    hold[kk][4] = d1.x;
+                    hold[kk][5] = d1.y;
+                    hold[kk][6] = d2.x;
+                    hold[kk][7] = d2.y;
+                }
+        }
+    }
     for (int jb = 0; jb < G.m; jb += kVM) {
         const int m = min(kVM, G.m - jb);
         double e[kVM];
 #pragma unroll
         for (int j = 0; j < kVM; j++) e[j] = 0.0;
         if (kSingle) {
-            // one point at a time (a rolled loop: unrolled, the points' gathers would be
-            // interleaved past the register budget); its angles and sum by selection
-#pragma unroll 1
+            const T sc = (T)rots[0].scale;
+            const double inv_step = 1.0 / step;
+#pragma unroll
             for (int k = 0; k < kVPts; k++) {
-                if (pl + L * k >= ppb) break;
-                double th = qv[0][0], ph = qv[0][1];
-#pragma unroll
-                for (int kk = 1; kk < kVPts; kk++)
-                    if (kk == k) {
-                        th = qv[kk][0];
-                        ph = qv[kk][1];
-                    }
-                cplx<T> acc[kVM];
-#pragma unroll
-                for (int j = 0; j < kVM; j++) acc[j] = {(T)0, (T)0};
-                single_amplitude<T, T4, kVM>(grids[rots[0].leaf], (T)rots[0].scale, G.layer, step, th, ph,
-                                             qs + G.q0 + jb, m, acc);
+                if (pl + L * k >= ppb) continue;
+                const cplx<T> p1 = {hold[k][0], hold[k][1]}, p2 = {hold[k][2], hold[k][3]};
+                const cplx<T> d1 = {hold[k][4], hold[k][5]}, d2 = {hold[k][6], hold[k][7]};
 #pragma unroll
                 for (int j = 0; j < kVM; j++) {
-                    const double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
                     if (j < m) {
+                        const T t = (T)((qs[G.q0 + jb + j] - G.layer * step) * inv_step);
+                        const cplx<T> a = hermite<T>(p1, p2, d1, d2, t);
+                        cplx<T> acc = {(T)0, (T)0};
+                        acc.x += a.x * sc;
+                        acc.y += a.y * sc;
+                        const double I = (double)acc.x * (double)acc.x + (double)acc.y * (double)acc.y;
                         e[j] += I;
-#pragma unroll
-                        for (int kk = 0; kk < kVPts; kk++)
-                            if (kk == k) so[kk] += I;
+                        so[k] += I;
                     }
                 }
             }
