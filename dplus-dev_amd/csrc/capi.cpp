@@ -431,9 +431,9 @@ void shape_record(int nshells, int ntiles, BuildShape shape) {
     shape_table[{nshells, tile_class(ntiles)}] = shape;
 }
 
-BuildShape Structure::shape_for(const Grid &g) const {
+BuildShape Structure::shape_for(const Grid &g, bool use_table) const {
     BuildShape sh;
-    if (shape_lookup(g.N, ntiles, &sh)) return sh;
+    if (use_table && shape_lookup(g.N, ntiles, &sh)) return sh;
     sh.wg = 256;
     sh.slices = pair_slices(pair_units(half_base(g.N + 1)), ntiles, sh.wg);
     return sh;
@@ -492,13 +492,13 @@ void Structure::finish(Grid &g, const Pending &P, bool host_copy, hipStream_t s)
         check_hip(hipStreamSynchronize(s), "pdb build");
 }
 
-void Structure::build(Grid &g, bool host_copy, hipStream_t s) {
+void Structure::build(Grid &g, bool host_copy, hipStream_t s, bool use_table) {
     const bool phase64 = getenv("DPLUS_PHASE64") != nullptr;
     if (!g.fp64() && !phase64) {
         // per-build buffers (form factors of this grid's shells, slice partials): a
         // structure can be built into several grids at once on different streams
         Pending P;
-        const BuildShape sh = shape_for(g);
+        const BuildShape sh = shape_for(g, use_table);
         BuildJob J = prepare(g, sh.slices, P, s);
         {
             KTimer kt("build", s);

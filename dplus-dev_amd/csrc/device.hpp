@@ -217,7 +217,9 @@ struct Structure {
     long long nsources = 0;
     DevBuf b32, b64, bt, bd;  // immutable after construction
     Structure(const dplus_pdb_desc *d, hipStream_t s);
-    void build(Grid &g, bool host_copy, hipStream_t s);
+    // use_table: the launch shape from the autotuner's table where it has the grid's
+    // class (shapes timed on a device to themselves), else the occupancy model's
+    void build(Grid &g, bool host_copy, hipStream_t s, bool use_table = true);
     // The mixed build in three steps, for callers that launch the builder of
     // several grids together: prepare uploads this grid's form factors and
     // sizes the slice partials (buffers owned by `bufs`, used on stream s) and
@@ -232,7 +234,7 @@ struct Structure {
     void finish(Grid &g, const Pending &bufs, bool host_copy, hipStream_t s) const;
     // launch shape of the paired builder for grid g: the tuned one when the
     // shape table holds it, else the occupancy model's
-    BuildShape shape_for(const Grid &g) const;
+    BuildShape shape_for(const Grid &g, bool use_table = true) const;
     // times every launch shape on grid g, records the fastest in the shape
     // table and returns it (ms: its time and the occupancy model's)
     BuildShape autotune(Grid &g, hipStream_t s, double *best_ms, double *model_ms);

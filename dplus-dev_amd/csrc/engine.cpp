@@ -322,7 +322,11 @@ struct Engine {
     }
 
     // mixed build of a request's grid: autotuned on first sight of its class
-    // when asked, grouped with other small builds when batching, else alone
+    // when asked, grouped with other small builds when batching, else alone.
+    // Without autotuning the engine takes the occupancy model's shapes: the
+    // shape table's entries are timed on an otherwise idle device, and on the
+    // concurrent request stream they were slower than the model (a 512-request
+    // stream 619-637 against 660-672 curves/s).
     void build(Structure &S, Grid &g, hipStream_t s) {
         BuildShape tuned;
         const bool mixed = !g.fp64() && getenv("DPLUS_PHASE64") == nullptr;
@@ -335,12 +339,12 @@ struct Engine {
         }
         const double terms = (double)half_base(g.N + 1) * (double)S.nsources;
         if (!mixed || !batch_builds || terms > batch_terms) {
-            S.build(g, false, s);
+            S.build(g, false, s, autotune);
             return;
         }
         Structure::Pending P;
         BuildTicket t;
-        t.job = S.prepare(g, S.shape_for(g).slices, P, s);
+        t.job = S.prepare(g, S.shape_for(g, autotune).slices, P, s);
         t.reduce = P.reduce;
         check_hip(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming), "hipEventCreate");
         if (hipEventCreateWithFlags(&t.done, hipEventDisableTiming) != hipSuccess) {

@@ -124,8 +124,10 @@ int dplus_structure_build_grids(int n, dplus_structure *const *s, dplus_grid *co
  * 320-410): times every workgroup size (64, 128, 256, 512 units) with every
  * atom-slice count (1..8) on this structure and grid, records the fastest in
  * the process's shape table under (grid size, tile-count class) and builds the
- * grid (+ splines).  Later builds of that class use the recorded shape; others
- * use the occupancy model.  Writes the chosen shape, its time and the
+ * grid (+ splines).  Later builds of that class use the recorded shape (in the
+ * request engine only with its autotune flag: shapes timed on an idle device
+ * do not carry over to a device shared by concurrent requests); others use
+ * the occupancy model.  Writes the chosen shape, its time and the
  * occupancy model's (ms). */
 int dplus_structure_autotune(dplus_structure *s, dplus_grid *g, int *wg, int *slices, double *best_ms,
                              double *model_ms);
@@ -294,8 +296,10 @@ typedef struct {
                                    ready together run as one grouped builder launch (up to
                                    16 grids of different structures and sizes) */
     long long batch_terms;      /* canonical nodes x (atoms + boxes); <= 0: 1e10 */
-    int autotune;               /* 1: a mixed build whose (grid, tile class) the shape table
-                                   lacks is autotuned on first sight (dplus_structure_autotune) */
+    int autotune;               /* 1: builds take the shape table's shapes, and a mixed build
+                                   whose (grid, tile class) the table lacks is autotuned on
+                                   first sight (dplus_structure_autotune); 0: the occupancy
+                                   model's shapes */
 } dplus_engine_config;
 
 typedef struct {
