@@ -121,7 +121,7 @@ int dplus_structure_build_grids(int n, dplus_structure *const *s, dplus_grid *co
 
 /* Autotuner of the mixed builder's launch shape (no counterpart in the
  * reference, whose kernels have fixed launch shapes, PDBAmplitudeCUDA.cu:
- * 320-410): times every workgroup size (64, 128, 256, 512 units) with every
+ * 242-247, 304-399): times every workgroup size (64, 128, 256, 512 units) with every
  * atom-slice count (1..8) on this structure and grid, records the fastest in
  * the process's shape table under (grid size, tile-count class) and builds the
  * grid (+ splines).  Later builds of that class use the recorded shape (in the
