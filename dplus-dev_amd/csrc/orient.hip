@@ -409,7 +409,7 @@ __device__ __forceinline__ double lanes_sum(double v, int L) {
     return v;
 }
 
-template <typename T, typename T4, int kL, bool kSingle>  // kL: lanes per bin fixed at compile time, or 0 for L
+template <typename T, typename T4, int kL, bool kSingle, int kP>  // kL: lanes per bin fixed at compile time, or 0 for L
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(const T4 *const *__restrict__ grids,
                                                     const RotDev *__restrict__ rots, int nrot,
                                                     const double4 *__restrict__ trans,
@@ -430,10 +430,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
     const int ti = bin / kVB, pj = bin % kVB;
     const double *bt = bounds + (size_t)g * 66, *bp = bt + 33;
     const double vol = (bt[ti + 1] - bt[ti]) * (bp[pj + 1] - bp[pj]);
-    double qv[kVPts][3];
-    double so[kVPts];
+    double qv[kP][3];
+    double so[kP];
 #pragma unroll
-    for (int k = 0; k < kVPts; k++) {
+    for (int k = 0; k < kP; k++) {
         so[k] = 0.0;
         const int p = pl + L * k;
         double u = 0.5, v = 0.5;
@@ -457,14 +457,14 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
     // once, held for every q chunk of the group; points in a rolled loop, one
     // lookup in flight (unrolled, the points' gathers would be interleaved past
     // the register budget), their results kept by selection
-    T hold[kVPts][8];
+    T hold[kP][8];
     if (kSingle) {
 #pragma unroll 1
-        for (int k = 0; k < kVPts; k++) {
+        for (int k = 0; k < kP; k++) {
             if (pl + L * k >= ppb) break;
             double th = qv[0][0], ph = qv[0][1];
 #pragma unroll
-            for (int kk = 1; kk < kVPts; kk++)
+            for (int kk = 1; kk < kP; kk++)
                 if (kk == k) {
                     th = qv[kk][0];
                     ph = qv[kk][1];
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
             cplx<T> amps[4], d1, d2;
             group_planes<T, T4>(grids[rots[0].leaf], G.layer, th, ph, amps, d1, d2);
 #pragma unroll
-            for (int kk = 0; kk < kVPts; kk++)
+            for (int kk = 0; kk < kP; kk++)
                 if (kk == k) {
                     hold[kk][0] = amps[1].x;
                     hold[kk][1] = amps[1].y;
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
             const T sc = (T)rots[0].scale;
             const double inv_step = 1.0 / step;
 #pragma unroll
-            for (int k = 0; k < kVPts; k++) {
+            for (int k = 0; k < kP; k++) {
                 if (pl + L * k >= ppb) continue;
                 const cplx<T> p1 = {hold[k][0], hold[k][1]}, p2 = {hold[k][2], hold[k][3]};
                 const cplx<T> d1 = {hold[k][4], hold[k][5]}, d2 = {hold[k][6], hold[k][7]};
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
             }
         } else {
 #pragma unroll
-            for (int k = 0; k < kVPts; k++) {
+            for (int k = 0; k < kP; k++) {
                 if (pl + L * k >= ppb) continue;
                 cplx<T> acc[kVM];
 #pragma unroll
@@ -540,12 +540,12 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
     // variance of vol * S over the bin's points, two-pass (equal to the reference's Welford sum)
     double sx = 0.0;
 #pragma unroll
-    for (int k = 0; k < kVPts; k++)
+    for (int k = 0; k < kP; k++)
         if (pl + L * k < ppb) sx += vol * so[k];
     const double mean = lanes_sum(sx, L) / (double)ppb;
     double sq = 0.0;
 #pragma unroll
-    for (int k = 0; k < kVPts; k++)
+    for (int k = 0; k < kP; k++)
         if (pl + L * k < ppb) sq += (vol * so[k] - mean) * (vol * so[k] - mean);
     sq = lanes_sum(sq, L);
     if (pl == 0) binvar[(size_t)gi * (kVB * kVB) + bin] = sq * ppb;
@@ -830,24 +830,32 @@ void launch_vegas_iter(bool fp64, bool single, const void *const *grids, const R
     int L = 1;
     while (L < ppb && L < 64) L <<= 1;
     dim3 grid(kVB * kVB / (4 * (64 / L)), nactive);
-#define DPLUS_VEGAS(T, T4, KL, S)                                                                           \
-    vegas_sample<T, T4, KL, S><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups, active, \
-                                                    mstride, qs, step, seed, iter, ppb, L, bounds, binsum, binvar)
-#define DPLUS_VEGAS_L(T, T4, S)       \
-    if (L == 64)                      \
-        DPLUS_VEGAS(T, T4, 64, S);    \
-    else                              \
-        DPLUS_VEGAS(T, T4, 0, S);
-    if (fp64 && single) {
-        DPLUS_VEGAS_L(double, double4, true)
+#define DPLUS_VEGAS(T, T4, KL, S, P)                                                                          \
+    vegas_sample<T, T4, KL, S, P><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups,      \
+                                                       active, mstride, qs, step, seed, iter, ppb, L, bounds,   \
+                                                       binsum, binvar)
+    // points per lane: the single-leaf instances are sized to the bin's points
+    // (fewer registers), the docked ones take up to kVPts
+    const int per_lane = (ppb + L - 1) / L;
+#define DPLUS_VEGAS_P(T, T4, KL)                 \
+    if (!single)                                 \
+        DPLUS_VEGAS(T, T4, KL, false, kVPts);    \
+    else if (per_lane <= 1)                      \
+        DPLUS_VEGAS(T, T4, KL, true, 1);         \
+    else if (per_lane <= 2)                      \
+        DPLUS_VEGAS(T, T4, KL, true, 2);         \
+    else                                         \
+        DPLUS_VEGAS(T, T4, KL, true, kVPts);
+    if (fp64 && L == 64) {
+        DPLUS_VEGAS_P(double, double4, 64)
     } else if (fp64) {
-        DPLUS_VEGAS_L(double, double4, false)
-    } else if (single) {
-        DPLUS_VEGAS_L(float, float4, true)
+        DPLUS_VEGAS_P(double, double4, 0)
+    } else if (L == 64) {
+        DPLUS_VEGAS_P(float, float4, 64)
     } else {
-        DPLUS_VEGAS_L(float, float4, false)
+        DPLUS_VEGAS_P(float, float4, 0)
     }
-#undef DPLUS_VEGAS_L
+#undef DPLUS_VEGAS_P
 #undef DPLUS_VEGAS
     vegas_reduce<<<nactive, 1024, 0, s>>>(groups, active, mstride, ppb, iter, binsum, binvar, bounds, hist, nq);
 }
