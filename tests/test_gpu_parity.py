@@ -211,7 +211,7 @@ def test_single_atom_fixture_on_gpu(dplus, oracle):
 
 
 def test_friedel_and_debye_at_larger_size(dplus, oracle):
-    """2,000 atoms (configuration 1 size), gs 100: the builder output is
+    """2,000 atoms in a 3 nm sphere, gs 100: the builder output is
     Friedel-symmetric and the MC curve follows the exact Debye sum within the
     grid's interpolation error."""
     say("[case] 2000 atoms gs 100 Debye")
@@ -225,14 +225,14 @@ def test_friedel_and_debye_at_larger_size(dplus, oracle):
     assert np.median(err) < 0.02 and err.max() < 0.2
 
 
-def test_helix_of_1000_copies_at_configuration3_shape(dplus):
-    """Configuration-3 shape: a 5,000-atom subunit on a grid of size 200 (qmax 6),
+def test_helix_of_1000_copies_of_a_5000_atom_grid200(dplus):
+    """A 5,000-atom subunit on a grid of size 200 (qmax 6),
     docked as a 1,000-copy helix (radius 12 nm, rise 0.9 nm, twist 27.7 deg; the
     translations reach 900 nm), 800 q, MC with 8,192 directions.  Mixed and FP64
     agree within the mixed bar, and at q = 0 every copy adds the subunit's
     forward amplitude in phase: I(0) = (1000 A(0))^2 exactly (FP64 1e-12)."""
     import dplus_mi355x.synth as S
-    say("[case] configuration-3 shape: 5000 atoms gs 200, helix of 1000 copies, 800 q")
+    say("[case] docking: 5000 atoms gs 200, helix of 1000 copies, 800 q")
     xyz, ion, coef9, rad = structure(5000, 3, radius=2.5)
     rot, nt, tr = S.rows_to_docking(S.helix_rows(1000, 12.0, 0.9, 27.7))
     q = np.linspace(0.0, 6.0, 800)
@@ -365,12 +365,12 @@ def test_autotune_shape_table(dplus):
         dplus.build_shapes_clear()
 
 
-def test_configuration1_at_full_size_vs_debye(dplus, oracle):
+def test_2000_atoms_grid100_500q_mc_and_gk_vs_debye(dplus, oracle):
     """The 2,000-atom workload as specified (3 nm sphere, grid 100, qmax 5,
     500 q, MC over 2^20 directions, fixed seed) against the FP64 Debye sum:
     the MC curve within its sampling error, Gauss-Kronrod (no sampling error)
     within the grid's interpolation error, and mixed within 1e-4 of FP64."""
-    say("[case] configuration 1: 2000 atoms, grid 100, 500 q, MC 2^20 directions and GK vs Debye")
+    say("[case] 2000 atoms, grid 100, 500 q, MC 2^20 directions and GK vs Debye")
     xyz, ion, coef9, rad = structure(2000, 1, radius=3.0)
     q = np.linspace(0.0, 5.0, 500)
     deb = oracle.debye(xyz, ion, oracle_ions(oracle, coef9, rad, ion), q)
