@@ -221,13 +221,7 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         t.maxd = maxd;
         t.mind = mind;
         t.node = node;
-        const double ba2 = (b - a) * 0.5, apb2 = (b + a) * 0.5;
-        for (int k = 0; k < 15; k++) {
-            const double th = apb2 + ba2 * kX[k];
-            t.qs[k] = q[qi] * std::sin(th);
-            t.qc[k] = q[qi] * std::cos(th);
-        }
-        tt.push_back(t);
+        tt.push_back(t);  // its q sin / q cos of the theta nodes: fill_nodes
         const int id = (int)tt.size() - 1;
         phiTrees.push_back(Tree());
         Tree &pt = phiTrees.back();
@@ -238,6 +232,23 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         }
     };
 
+    Pool pool(std::max(1, std::min(16, (int)std::thread::hardware_concurrency())));
+    // q sin(theta_k), q cos(theta_k) of the 15 nodes of theta tasks [from, end)
+    auto fill_nodes = [&](size_t from) {
+        const size_t n = tt.size() - from;
+        const int parts = (int)std::max<size_t>(1, std::min<size_t>((size_t)pool.size(), n / 256));
+        pool.run(parts, [&](int part) {
+            for (size_t id = from + n * part / parts; id < from + n * (part + 1) / parts; id++) {
+                ThetaTask &t = tt[id];
+                const double ba2 = (t.tmax - t.tmin) * 0.5, apb2 = (t.tmax + t.tmin) * 0.5;
+                for (int k = 0; k < 15; k++) {
+                    const double th = apb2 + ba2 * kX[k];
+                    t.qs[k] = q[t.q] * std::sin(th);
+                    t.qc[k] = q[t.q] * std::cos(th);
+                }
+            }
+        });
+    };
     for (int i = 0; i < nq; i++) {
         if (q[i] == 0.0) {
             zero_q.push_back(i);
@@ -246,6 +257,7 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         const int root = thetaTrees[i].add(-1);
         open_theta(i, 0.0, kPi / 2.0, max_depth - 1, -1, root);
     }
+    fill_nodes(0);
     // q = 0: |A(0)|^2 (Amplitude.cpp:1479-1484)
     if (!zero_q.empty()) {
         std::vector<double> pts(3 * zero_q.size(), 0.0), vals(zero_q.size());
@@ -273,7 +285,6 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
     bnodes.upload(&nodes, sizeof nodes, s);
     Pinned<double4> htask;
     Pinned<double> hval;
-    Pool pool(std::max(1, std::min(16, (int)std::thread::hardware_concurrency())));
     std::vector<unsigned char> split;
     // [begin, end) ranges of `open` for the host threads, cut at theta-task boundaries
     auto ranges = [&]() {
@@ -368,17 +379,33 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
             }
         });
         std::swap(open, next);
-        // theta intervals whose 15 phi integrals are complete, in interval order
-        for (int part = 0; part < nt; part++)
-            for (int id : fin_part[part]) {
-                const ThetaTask T = tt[id];
-                const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
-                double kv = 0.0, gv = 0.0;
-                for (int i = 0; i < 15; i++) {
-                    const double sT = std::sin(apb2 + ba2 * kX[i]);
-                    kv += T.phival[i] * kW[i] * ba2 * sT;
-                    if (i % 2 == 1) gv += T.phival[i] * gW[i / 2] * ba2 * sT;
+        // theta intervals whose 15 phi integrals are complete, in interval order:
+        // their Kronrod and Gauss sums in parallel, then the decisions in order
+        std::vector<int> fin;
+        for (int part = 0; part < nt; part++) fin.insert(fin.end(), fin_part[part].begin(), fin_part[part].end());
+        std::vector<double> fkv(fin.size()), fgv(fin.size());
+        {
+            const int parts = (int)std::max<size_t>(1, std::min<size_t>((size_t)pool.size(), fin.size() / 64));
+            pool.run(parts, [&](int part) {
+                for (size_t f = fin.size() * part / parts; f < fin.size() * (part + 1) / parts; f++) {
+                    const ThetaTask &T = tt[fin[f]];
+                    const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
+                    double kv = 0.0, gv = 0.0;
+                    for (int i = 0; i < 15; i++) {
+                        const double sT = std::sin(apb2 + ba2 * kX[i]);
+                        kv += T.phival[i] * kW[i] * ba2 * sT;
+                        if (i % 2 == 1) gv += T.phival[i] * gW[i / 2] * ba2 * sT;
+                    }
+                    fkv[f] = kv;
+                    fgv[f] = gv;
                 }
+            });
+        }
+        const size_t tt_before = tt.size();
+        for (size_t f = 0; f < fin.size(); f++) {
+            {
+                const ThetaTask T = tt[fin[f]];
+                const double kv = fkv[f], gv = fgv[f];
                 Tree &tr = thetaTrees[T.q];
                 if (T.mind < 0 && (T.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv)) {
                     const int r = tr.resolve(T.node, kv);
@@ -393,6 +420,8 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
                     open_theta(T.q, mid, T.tmax, T.maxd - 1, T.mind - 1, rgt);
                 }
             }
+        }
+        if (tt.size() > tt_before) fill_nodes(tt_before);
     }
     if (st) st->evaluations = evals;
 }
