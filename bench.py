@@ -100,7 +100,7 @@ def cpu_baseline(W, g64_data, g64_interp, budget_nodes=1500):
                       f"(scaled x{W['iters'] * W['ppb']}); full curve estimate {total:.0f} s"}
 
 
-def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, streams=8, fp64=True,
+def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, streams=16, fp64=True,
                    batch_builds=False):
     """Batched serving stream: n_requests with Poisson arrivals at `rate`/s; each
     request is one of `pool_size` seeded random-walk structures (atoms

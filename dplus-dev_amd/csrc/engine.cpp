@@ -645,7 +645,7 @@ int dplus_engine_create(const dplus_engine_config *cfg, dplus_engine **out) {
     ENGINE_GUARD({
         if (!out) throw Error(kErrInvalidArgs, "null output");
         const long long budget = cfg ? cfg->hbm_budget_bytes : 0;
-        const int streams = cfg && cfg->streams > 0 ? cfg->streams : 8;
+        const int streams = cfg && cfg->streams > 0 ? cfg->streams : 16;
         *out = (dplus_engine *)new Engine(budget, streams, cfg && cfg->batch_builds != 0, cfg ? cfg->batch_terms : 0,
                                           cfg && cfg->autotune != 0);
     })

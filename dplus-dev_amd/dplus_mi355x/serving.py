@@ -16,7 +16,7 @@ class Engine:
     launches; autotune tunes each new (grid, tile class) build shape on first
     sight."""
 
-    def __init__(self, hbm_budget_bytes=0, streams=8, batch_builds=False, batch_terms=0, autotune=False):
+    def __init__(self, hbm_budget_bytes=0, streams=16, batch_builds=False, batch_terms=0, autotune=False):
         h = ctypes.c_void_p()
         cfg = L.EngineConfig(int(hbm_budget_bytes), int(streams), 1 if batch_builds else 0, int(batch_terms),
                              1 if autotune else 0)
