@@ -300,7 +300,7 @@ std::string amp_header(const ModelNode &n, const Grid &g) {
 
 // ------------------------------------------------------------------ backend object
 struct Backend {
-    std::mutex mu;
+    std::mutex mu, start_mu;
     std::thread worker;
     std::atomic<bool> stop_flag{false};
     bool running = false;
@@ -571,6 +571,10 @@ struct Backend {
         const json::Value doc = json::parse(state);
         Device::get().require();
         Spec s = parse_state(doc);
+        // one start at a time: the running check, the join of the finished
+        // worker and the assignment of the new one happen under start_mu, so a
+        // concurrent start sees running == true and fails with ERROR_JOBRUNNING
+        std::lock_guard<std::mutex> start_lk(start_mu);
         {
             std::lock_guard<std::mutex> lk(mu);
             if (running) throw Error(kErrJobRunning, "a job is already running");  // JobManager.cpp:150-153

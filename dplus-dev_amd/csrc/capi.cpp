@@ -1072,6 +1072,8 @@ void launch_sphr_batch(bool fp64, const void *g, int N, double step, int n, cons
                        const double *ph, double2 *out, hipStream_t s);
 void launch_seeded_samples(bool fp64, const void *g, int N, double step, double q, long long n,
                            const double *phi, const double *v2, double *out, hipStream_t s);
+void launch_plane_batch(bool fp64, const void *g, int n, const int *ri, const double *th, const double *ph,
+                        double2 *out, hipStream_t s);
 }  // namespace dplus
 extern "C" {
 
@@ -1095,6 +1097,30 @@ int dplus_grid_get_sphr(dplus_grid *gh, int n, const double *q, const double *th
         check_hip(hipGetLastError(), "sphr launch");
         check_hip(hipMemcpyAsync(out, bo.p, n * sizeof(double2), hipMemcpyDeviceToHost, s), "D2H");
         check_hip(hipStreamSynchronize(s), "sphr");
+    })
+}
+
+int dplus_grid_interpolate_theta_phi_plane(dplus_grid *gh, int n, const int *ri, const double *th, const double *ph,
+                                           double *out) {
+    API_GUARD({
+        Grid *g = (Grid *)gh;
+        if (!g || n < 0 || (n > 0 && (!ri || !th || !ph || !out))) throw Error(kErrInvalidArgs, "bad arguments");
+        if (n == 0) return kOk;
+        for (int i = 0; i < n; i++)
+            if (!(ri[i] >= 1 && ri[i] <= g->N && th[i] >= 0 && th[i] <= kPi && ph[i] >= 0 && ph[i] <= k2Pi))
+                throw Error(kErrInvalidArgs,
+                            "plane lookup outside the grid (shell in [1, actual size], theta in [0,pi], phi in [0,2pi])");
+        hipStream_t s = Device::get().stream;
+        DevBuf br, bt, bp, bo;
+        br.upload(ri, n * sizeof(int), s);
+        bt.upload(th, n * sizeof(double), s);
+        bp.upload(ph, n * sizeof(double), s);
+        bo.ensure(n * sizeof(double2));
+        launch_plane_batch(g->fp64(), g->dev.p, n, (const int *)br.p, (const double *)bt.p, (const double *)bp.p,
+                           (double2 *)bo.p, s);
+        check_hip(hipGetLastError(), "plane launch");
+        check_hip(hipMemcpyAsync(out, bo.p, n * sizeof(double2), hipMemcpyDeviceToHost, s), "D2H");
+        check_hip(hipStreamSynchronize(s), "plane");
     })
 }
 

@@ -266,6 +266,11 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                          dplus_oa_stats *st, hipStream_t s, Chi2Request *chi = nullptr);
 void gk_average(const DockSet &D, int nq, const double *q, double eps, long long max_depth, double *out,
                 dplus_oa_stats *st, hipStream_t s);
+// Gauss-Kronrod levels are evaluated on the device in batches of at most this
+// many phi intervals (32 B task + 16 B sums each)
+constexpr size_t kGkBatch = (size_t)1 << 20;
+// upper bound of the device memory of one Gauss-Kronrod average over nq q values
+inline size_t gk_device_bytes(size_t nq) { return kGkBatch * 48 + nq * 32 + 4096; }
 // space-filling lattice of a docked grid: basis vectors v[3k..3k+2] (nm) and
 // repetitions n[k]; on = 0 for none
 struct LatticeDev {

@@ -154,7 +154,7 @@ size_t request_bytes(const Request &r) {
     } else if (p.method == DPLUS_OA_VEGAS) {
         b += ng * mstride * 1024 * 8 + ng * 1024 * 8 + (size_t)std::max(1, p.vegas_iterations) * r.q.size() * 8;
     } else {
-        b += 64u << 20;
+        b += gk_device_bytes(r.q.size());
     }
     b += (r.rot.size() / 4 + 1) * sizeof(RotDev) + (r.trans.size() / 3 + 1) * sizeof(double4);
     return b;
@@ -832,6 +832,14 @@ int dplus_session_create(int nleaf, dplus_grid *const *leaves, int nq, const dou
         }
         for (int i = 1; i < nq; i++)
             if (q[i] < q[i - 1]) throw Error(kErrInvalidArgs, "q must be ascending");
+        // the orientation average's range checks, before anything is allocated or
+        // captured: a q past shell grid_size/2 + 1 would make the four-plane lookup
+        // read shells beyond the grid
+        if (!(q[0] >= 0)) throw Error(kErrInvalidArgs, "q must be non-negative");
+        {
+            const Grid *g0 = S->leaves[0];
+            if (!(q[nq - 1] <= (g0->grid_size / 2 + 1) * g0->step)) throw Error(kErrInvalidArgs, "q beyond the grid");
+        }
         S->nq = nq;
         S->seed = seed;
         S->groups = make_groups(q, nq, S->step);

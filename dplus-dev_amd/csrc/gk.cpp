@@ -316,15 +316,22 @@ void gk_average(const DockSet &D, int nq, const double *q, double eps, long long
         // the device returns each interval's Kronrod and Gauss sums
         double *vals = hval.get(2 * n);
         {
-            btask.ensure(n * sizeof(double4));
-            bo.ensure(2 * n * sizeof(double));
-            check_hip(hipMemcpyAsync(btask.p, tasks, n * sizeof(double4), hipMemcpyHostToDevice, s), "H2D");
+            // device batches of at most kGkBatch intervals, so a level's device
+            // memory is bounded (gk_device_bytes) however many intervals it holds
+            const size_t nb = std::min<size_t>(n, kGkBatch);
+            btask.ensure(nb * sizeof(double4));
+            bo.ensure(2 * nb * sizeof(double));
             KTimer kt("gk", s);
-            launch_docked_gk(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
-                             (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (long long)n,
-                             (const double4 *)btask.p, (const GkNodes *)bnodes.p, (double *)bo.p, s);
-            check_hip(hipGetLastError(), "gk launch");
-            check_hip(hipMemcpyAsync(vals, bo.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            for (size_t b0 = 0; b0 < n; b0 += nb) {
+                const size_t m = std::min(nb, n - b0);
+                check_hip(hipMemcpyAsync(btask.p, tasks + b0, m * sizeof(double4), hipMemcpyHostToDevice, s), "H2D");
+                launch_docked_gk(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
+                                 (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, (long long)m,
+                                 (const double4 *)btask.p, (const GkNodes *)bnodes.p, (double *)bo.p, s);
+                check_hip(hipGetLastError(), "gk launch");
+                check_hip(hipMemcpyAsync(vals + 2 * b0, bo.p, 2 * m * sizeof(double), hipMemcpyDeviceToHost, s),
+                          "D2H");
+            }
             check_hip(hipStreamSynchronize(s), "gk batch");
             evals += 15 * (long long)n;
         }

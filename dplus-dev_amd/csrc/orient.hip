@@ -702,6 +702,19 @@ __global__ __launch_bounds__(256) void sphr_batch(const T4 *__restrict__ g, int 
     out[i] = make_double2((double)a.x, (double)a.y);
 }
 
+// batch InterpolateThetaPhiPlane (Grid.cpp:61-134): shell ri[i] >= 1 at
+// (theta, phi) with the CPU variant's snapping of tiny fractions
+template <typename T, typename T4>
+__global__ __launch_bounds__(256) void plane_batch(const T4 *__restrict__ g, int n, const int *__restrict__ ri,
+                                                   const double *__restrict__ th, const double *__restrict__ ph,
+                                                   double2 *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int r = ri[i];
+    cplx<T> a = plane_lookup<T, T4, true>(g + shell_base(r), r, th[i] * kInvPi, ph[i] * kInv2Pi);
+    out[i] = make_double2((double)a.x, (double)a.y);
+}
+
 // |A|^2 of JacobianSphereGrid::CalculateIntensity's samples (Grid.cpp:421-466):
 // phi[i] and v2[i] in [0,2), theta = acos(v2-1), lookup through GetCart
 template <typename T, typename T4>
@@ -867,6 +880,15 @@ void launch_sphr_batch(bool fp64, const void *g, int N, double step, int n, cons
         sphr_batch<double, double4><<<nb, 256, 0, s>>>((const double4 *)g, N, step, n, q, th, ph, out);
     else
         sphr_batch<float, float4><<<nb, 256, 0, s>>>((const float4 *)g, N, step, n, q, th, ph, out);
+}
+
+void launch_plane_batch(bool fp64, const void *g, int n, const int *ri, const double *th, const double *ph,
+                        double2 *out, hipStream_t s) {
+    unsigned nb = (n + 255) / 256;
+    if (fp64)
+        plane_batch<double, double4><<<nb, 256, 0, s>>>((const double4 *)g, n, ri, th, ph, out);
+    else
+        plane_batch<float, float4><<<nb, 256, 0, s>>>((const float4 *)g, n, ri, th, ph, out);
 }
 
 void launch_seeded_samples(bool fp64, const void *g, int N, double step, double q, long long n,

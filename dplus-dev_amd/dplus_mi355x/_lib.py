@@ -96,6 +96,7 @@ SIGNATURES = {
     "dplus_grid_calculate_splines": (ctypes.c_int, [ctypes.c_void_p]),
     "dplus_grid_download": (ctypes.c_int, [ctypes.c_void_p]),
     "dplus_grid_get_sphr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D]),
+    "dplus_grid_interpolate_theta_phi_plane": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I, _D, _D, _D]),
     "dplus_grid_calculate_intensity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                                       ctypes.c_double, ctypes.c_uint, ctypes.c_ulonglong,
                                                       ctypes.c_double, ctypes.c_double, _D]),

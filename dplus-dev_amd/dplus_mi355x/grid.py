@@ -56,6 +56,17 @@ class Grid:
         L.check(L.lib().dplus_grid_get_sphr(self._h, n, L.dp(q), L.dp(theta), L.dp(phi), L.dp(out)))
         return out[0::2] + 1j * out[1::2]
 
+    def interpolate_theta_phi_plane(self, ri, theta, phi):
+        """InterpolateThetaPhiPlane (Grid.cpp:61-134) inside shell ri at (theta, phi), many points."""
+        ri = np.atleast_1d(L.i32(ri))
+        theta, phi = (np.atleast_1d(L.f64(x)) for x in (theta, phi))
+        ri, theta, phi = (np.ascontiguousarray(x) for x in np.broadcast_arrays(ri, theta, phi))
+        n = len(ri)
+        out = np.zeros(2 * n)
+        L.check(L.lib().dplus_grid_interpolate_theta_phi_plane(self._h, n, L.ip(ri), L.dp(theta), L.dp(phi),
+                                                                 L.dp(out)))
+        return out[0::2] + 1j * out[1::2]
+
     def calculate_intensity(self, q, theta=None, epsilon=1e-3, seed=0, iterations=1000000, phi_min=0.0,
                             phi_max=2 * np.pi):
         r = ctypes.c_double()

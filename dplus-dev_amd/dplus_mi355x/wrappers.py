@@ -122,7 +122,6 @@ class CJacobianSphereGrid:
 
     theta_divisions = 3  # Grid.h: thetaDivisions
     phi_divisions = 6    # Grid.h: phiDivisions
-    extras = 3
 
     def __init__(self, gridSize=0, qMax=0, precision=L.PRECISION_FP64):
         from .grid import Grid
@@ -154,6 +153,11 @@ class CJacobianSphereGrid:
     @property
     def actual_size(self):
         return self._g.actual_size
+
+    @property
+    def extras(self):
+        """GetExtras (Grid.h:203): shells past gridSize/2 (actualGridSize = gridSize/2 + Extras, Grid.cpp:178)."""
+        return self.actual_size - self.grid_size // 2
 
     @property
     def real_size(self):
@@ -234,6 +238,10 @@ class CJacobianSphereGrid:
     def get_sphr(self, q, theta, phi):
         """GetSphr (Grid.cpp:487-548) at one point."""
         return complex(self._g.get_sphr(q, theta, phi)[0])
+
+    def interpolate_theta_phi_plane(self, ri, theta, phi):
+        """InterpolateThetaPhiPlane (Grid.pyx:139-140, Grid.cpp:61-134): the value inside shell ri."""
+        return complex(self._g.interpolate_theta_phi_plane(ri, theta, phi)[0])
 
     def get_sphr_array(self, q, theta, phi):
         """GetSphr at many points in one GPU launch."""

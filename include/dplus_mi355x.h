@@ -68,6 +68,12 @@ int dplus_grid_download(dplus_grid *g);
 /* GetSphr(q, theta, phi) (Grid.cpp:487-548) for n points; out = 2n doubles */
 int dplus_grid_get_sphr(dplus_grid *g, int n, const double *q, const double *theta,
                         const double *phi, double *out);
+/* InterpolateThetaPhiPlane(ri, theta, phi) (Grid.cpp:61-134, bound by
+ * Grid.pyx:139-140) for n points: the phi-Hermite / theta four-point
+ * interpolation inside shell ri[i] (1 <= ri <= actual size), theta in [0, pi],
+ * phi in [0, 2 pi]; out = 2n doubles */
+int dplus_grid_interpolate_theta_phi_plane(dplus_grid *g, int n, const int *ri, const double *theta,
+                                           const double *phi, double *out);
 /* CalculateIntensity(q, [theta,] eps, seed, iterations, phi_min, phi_max)
  * (Grid.cpp:393-484) with the reference's std::mt19937 stream; theta < 0 draws
  * theta at random and, like the reference's overload, ignores the phi range. */

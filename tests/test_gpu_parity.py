@@ -110,6 +110,32 @@ def test_get_sphr_matches_oracle(dplus, oracle, small):
     assert relmax(a, b) < 1e-10
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_interpolate_theta_phi_plane_matches_oracle(dplus, oracle, small, prec):
+    """InterpolateThetaPhiPlane (Grid.cpp:61-134) through the C ABI against the oracle:
+    FP64 grid 1e-10, mixed (FP32 storage) 1e-5 of the shell's largest value; the
+    CJacobianSphereGrid mirror gives the same; out-of-range shells are rejected."""
+    say(f"[case] interpolate_theta_phi_plane precision={prec}")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    N = s["gs"] // 2 + 3
+    rng = np.random.default_rng(11)
+    n = 600
+    ri = np.concatenate([rng.integers(1, N + 1, n - 8), [1, 1, N, N, 2, 5, N, 1]])
+    th = np.concatenate([rng.uniform(0, np.pi, n - 8), [0.0, np.pi, 0.0, np.pi, np.pi / 6, 1.0, 3.1, 0.0]])
+    ph = np.concatenate([rng.uniform(0, 2 * np.pi, n - 8), [0.0, 2 * np.pi, 6.0, 0.0, np.pi, 2 * np.pi, 0.1, 3.0]])
+    a = g.interpolate_theta_phi_plane(ri, th, ph)
+    b = np.array([oracle.interp_plane(s["gs"], s["data"], s["co"], int(r), t, p) for r, t, p in zip(ri, th, ph)])
+    assert relmax(a, b) < (1e-10 if prec == 0 else 1e-5)
+    import dplus_mi355x.wrappers as W
+    cg = W.CJacobianSphereGrid._wrap(g)
+    assert cg.interpolate_theta_phi_plane(int(ri[0]), th[0], ph[0]) == a[0]
+    assert cg.extras == 3
+    for bad in ((0, 1.0, 1.0), (N + 1, 1.0, 1.0), (1, -0.1, 1.0), (1, 1.0, 7.0)):
+        with pytest.raises(dplus.DplusError):
+            g.interpolate_theta_phi_plane(*bad)
+
+
 def test_seeded_intensity_matches_oracle(dplus, oracle, small):
     say("[case] seeded CalculateIntensity")
     s = small

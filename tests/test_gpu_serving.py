@@ -157,6 +157,25 @@ def test_fit_session_graph_replays(dplus):
     sess.close()
 
 
+def test_fit_session_rejects_q_outside_the_grid(dplus):
+    """A session checks q like the orientation average does (q >= 0, q up to shell
+    grid_size/2 + 1) before it allocates or captures anything."""
+    from dplus_mi355x.serving import FitSession
+    say("[case] fit session: q outside the grid is rejected at creation")
+    pdb = make_pdb(200, 22, 1.0)
+    g = dplus.Grid(24, 4.0, dplus.PRECISION_MIXED)
+    dplus.Structure(pdb["xyz"], pdb["ion"], pdb["coef9"], pdb["rad"], sol_ed=333.0, flags=3).build(g)
+    step = 4.0 / 12
+    for q in (np.linspace(0.05, 13 * step * 1.001, 40), np.linspace(-0.1, 3.0, 40), np.linspace(0.05, 4.3 * 1.5, 40)):
+        with pytest.raises(dplus.DplusError) as ei:
+            FitSession([g], q, np.ones(40), np.ones(40), seed=1, directions=1024)
+        assert ei.value.error_code == 21
+    sess = FitSession([g], np.linspace(0.0, 13 * step, 40), np.ones(40), np.ones(40), seed=1, directions=1024)
+    I, _ = sess.evaluate([(np.array([[0.0, 0.0, 0.0, 1.0]]), np.array([1]), np.zeros((1, 3)))])
+    assert np.all(np.isfinite(I)) and np.all(I > 0)
+    sess.close()
+
+
 def test_concurrent_sessions_capture_while_others_allocate(dplus):
     """Eight fit sessions in threads, each re-capturing its graph as the copy
     count grows, while the main thread keeps allocating grids: every capture

@@ -130,6 +130,25 @@ def test_get_sphr_matches_reference():
 
 
 @needs_ref
+def test_interp_plane_matches_reference():
+    """The oracle's InterpolateThetaPhiPlane against the reference's own: GetSphr at
+    q = ri * stepSize returns the plane of shell ri (Grid.cpp:498-511)."""
+    gs, qmax = 20, 5.0
+    rng = np.random.default_rng(8)
+    data = rng.normal(size=2 * O.total_points(gs))
+    co = O.grid_splines(gs, data)
+    R = O.RefGrid(gs, qmax, data)
+    step = qmax / (gs // 2)
+    N = gs // 2 + 3
+    pts = [(int(rng.integers(1, N - 1)), rng.uniform(0, np.pi), rng.uniform(0, 2 * np.pi)) for _ in range(2000)]
+    pts += [(ri, th, ph) for ri in (1, 2, N - 2) for th in (0.0, np.pi, np.pi / 3) for ph in (0.0, 2 * np.pi, np.pi)]
+    for ri, th, ph in pts:
+        a = O.interp_plane(gs, data, co, ri, th, ph)
+        b = R.get_sphr(ri * step, th, ph)
+        assert abs(a - b) <= 1e-12 * max(1.0, abs(b)), (ri, th, ph)
+
+
+@needs_ref
 def test_amplitudes_at_points_match_reference():
     gs, qmax = 20, 5.0
     rng = np.random.default_rng(4)
