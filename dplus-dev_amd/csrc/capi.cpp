@@ -161,6 +161,11 @@ std::shared_mutex &capture_mutex() {
     return m;
 }
 
+static thread_local Arena *t_arena = nullptr;
+void set_arena(Arena *a) { t_arena = a; }
+static std::atomic<long long> g_device_allocs{0};
+long long device_allocations() { return g_device_allocs.load(); }
+
 DevBuf::~DevBuf() { release(); }
 static void free_locked(void *&p, size_t &n, hipStream_t s) {
     if (p) (void)hipFreeAsync(p, s);
@@ -168,14 +173,38 @@ static void free_locked(void *&p, size_t &n, hipStream_t s) {
     n = 0;
 }
 void DevBuf::release() {
+    if (borrowed) {
+        p = nullptr;
+        n = 0;
+        borrowed = false;
+        return;
+    }
     std::shared_lock<std::shared_mutex> lk(capture_mutex());
     free_locked(p, n, s);
 }
 void DevBuf::ensure(size_t bytes) {
     if (bytes <= n) return;
+    if (t_arena) {
+        release();
+        const size_t off = (t_arena->used + 255) & ~(size_t)255;
+        if (off > t_arena->size || bytes > t_arena->size - off)
+            throw Error(kErrInvalidArgs, "scratch exhausted: " + std::to_string(off + bytes) + " bytes needed, " +
+                                             std::to_string(t_arena->size) + " available");
+        p = t_arena->base + off;
+        n = bytes;
+        borrowed = true;
+        t_arena->used = off + bytes;
+        return;
+    }
     std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    if (borrowed) {
+        p = nullptr;
+        n = 0;
+        borrowed = false;
+    }
     free_locked(p, n, s);
     if (bytes == 0) return;
+    g_device_allocs++;
     s = alloc_stream();
     const hipError_t e = hipMallocAsync(&p, bytes, s);
     if (e != hipSuccess) {
@@ -896,7 +925,8 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
         bsum.ensure((size_t)ng * mstride * 1024 * sizeof(double));
         bvar.ensure((size_t)ng * 1024 * sizeof(double));
         bh.ensure((size_t)iters * nq * sizeof(double));
-        check_hip(hipMemsetAsync(bh.p, 0, (size_t)iters * nq * sizeof(double), s), "memset");
+        launch_fill_zero((double *)bh.p, (size_t)iters * nq, s);
+        check_hip(hipGetLastError(), "fill launch");
         std::vector<double> hist((size_t)iters * nq, 0.0);
         std::vector<int> active = act, done_k(ng, iters - 1);
         int used = 0;
@@ -972,7 +1002,8 @@ int dplus_device_info(char *buf, int len) {
         Device &d = Device::get();
         std::string s = "{\"ok\":" + std::string(d.ok ? "true" : "false") + ",\"device\":" + std::to_string(d.id) +
                         ",\"arch\":\"" + json_escape(d.name) + "\",\"cus\":" + std::to_string(d.cus) +
-                        ",\"memory\":" + std::to_string(d.mem) + "}";
+                        ",\"memory\":" + std::to_string(d.mem) +
+                        ",\"device_allocations\":" + std::to_string(device_allocations()) + "}";
         if (buf && len > 0) {
             strncpy(buf, s.c_str(), len - 1);
             buf[len - 1] = 0;
@@ -1207,6 +1238,77 @@ int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const
         Device::get().require();
         auto t0 = std::chrono::steady_clock::now();
         hipStream_t s = Device::get().stream;
+        DockSet D(nleaf, leaves, s);
+        if (st) memset(st, 0, sizeof *st);
+        orientation_average(D, nq, q, p, intensity, st, s);
+        timing_collect();
+        if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    })
+}
+
+// Device bytes the Monte Carlo / VEGAS orientation average takes from a
+// scratch arena: every DevBuf of DockSet and of orientation_average's MC and
+// VEGAS branches, in their sizes, each 256-byte aligned.
+static size_t oa_arena_bytes(int nleaf, long long nrot, long long ntr, int nq, const double *q, double step,
+                             const dplus_oa_params *p) {
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t b = al((size_t)nleaf * sizeof(void *)) + al((size_t)nleaf * sizeof(LeafGeo)) +
+               al((size_t)std::max<long long>(1, nrot) * sizeof(RotDev)) +
+               al((size_t)std::max<long long>(1, ntr) * sizeof(double4));
+    if (nq <= 0) return b;
+    const std::vector<GroupDev> groups = make_groups(q, nq, step);
+    const size_t ng = groups.size();
+    size_t mstride = 1;
+    for (auto &g : groups) mstride = std::max<size_t>(mstride, (size_t)g.m);
+    b += al((size_t)nq * 8) + al(ng * sizeof(GroupDev)) + al(ng * sizeof(int));
+    if (p->method == DPLUS_OA_MC) {
+        const long long chunk = p->chunk > 0 ? p->chunk : 8192;
+        const long long nchunks = (std::max<long long>(1, p->max_dirs) + chunk - 1) / chunk;
+        const size_t nblk = (size_t)(p->eps <= 0.0 ? nchunks * chunk / 256 : chunk / 256);
+        b += al(ng * nblk * mstride * 8) + al(ng * mstride * 8) + (p->eps <= 0.0 ? al((size_t)nq * 8) : 0);
+    } else {
+        const size_t iters = (size_t)std::max(1, p->vegas_iterations);
+        b += al(ng * 66 * 8) + al(ng * mstride * 1024 * 8) + al(ng * 1024 * 8) + al(iters * nq * 8);
+    }
+    return b;
+}
+
+int dplus_orientation_average_noalloc(int nleaf, const dplus_leaf *leaves, int nq, const double *q,
+                                      const dplus_oa_params *p, double *intensity, dplus_oa_stats *st) {
+    API_GUARD({
+        if (!leaves || !q || !p || !intensity || nq < 0 || nleaf <= 0) throw Error(kErrInvalidArgs, "null argument");
+        if (p->method != DPLUS_OA_MC && p->method != DPLUS_OA_VEGAS)
+            throw Error(kErrInvalidArgs, "the allocation-free orientation average runs Monte Carlo or VEGAS only");
+        if (p->method == DPLUS_OA_MC) {
+            const int chunk = p->chunk > 0 ? p->chunk : 8192;
+            if (chunk % 256) throw Error(kErrInvalidArgs, "MC chunk must be a multiple of 256");
+        }
+        for (int i = 1; i < nq; i++)
+            if (q[i] < q[i - 1]) throw Error(kErrInvalidArgs, "q must be ascending");
+        if (nq > 0 && !(q[0] >= 0)) throw Error(kErrInvalidArgs, "q must be non-negative");
+        long long nrot = 0, ntr = 0;
+        for (int l = 0; l < nleaf; l++) {
+            if (!leaves[l].grid) throw Error(kErrInvalidArgs, "leaf without grid");
+            nrot += leaves[l].nrot;
+            for (int r = 0; r < leaves[l].nrot; r++) ntr += leaves[l].ntrans[r];
+        }
+        const size_t need = oa_arena_bytes(nleaf, nrot, ntr, nq, q, ((Grid *)leaves[0].grid)->step, p);
+        if (need > scratch_capacity())
+            throw Error(kErrInvalidArgs, "the allocation-free orientation average needs " + std::to_string(need) +
+                                             " bytes of scratch, above its maximum of " +
+                                             std::to_string(scratch_capacity()) + " bytes");
+        Device &dev = Device::get();
+        dev.require();
+        std::lock_guard<std::mutex> use(scratch_lock(dev.id));
+        Arena arena;
+        arena.base = (char *)scratch_address(dev.id);
+        arena.size = scratch_capacity();
+        struct Install {
+            explicit Install(Arena *a) { set_arena(a); }
+            ~Install() { set_arena(nullptr); }
+        } install(&arena);
+        auto t0 = std::chrono::steady_clock::now();
+        hipStream_t s = dev.stream;
         DockSet D(nleaf, leaves, s);
         if (st) memset(st, 0, sizeof *st);
         orientation_average(D, nq, q, p, intensity, st, s);

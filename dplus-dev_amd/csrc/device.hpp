@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include <mutex>
 #include <shared_mutex>
 
 #include "../../include/dplus_mi355x.h"
@@ -55,11 +56,13 @@ void set_alloc_stream(hipStream_t s);
 // capture (HIP rejects them there).
 std::shared_mutex &capture_mutex();
 
-// owning device allocation that only grows
+// owning device allocation that only grows (or, while the calling thread has
+// a scratch arena installed, a region of that arena)
 struct DevBuf {
     void *p = nullptr;
     size_t n = 0;
     hipStream_t s = nullptr;  // stream the allocation is ordered on
+    bool borrowed = false;    // p lies in a scratch arena: nothing to free
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -68,6 +71,26 @@ struct DevBuf {
     void ensure(size_t bytes);
     void upload(const void *src, size_t bytes, hipStream_t s);
 };
+
+// Scratch arena of the allocation-free entry point: while a thread has one
+// installed, DevBuf::ensure carves its buffers from [base, base + size) in
+// call order (256-byte aligned) instead of allocating, failing with
+// ERROR_INVALID_ARGS when the arena is exhausted, and release() frees nothing.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0, used = 0;
+};
+void set_arena(Arena *a);  // nullptr: allocate again
+// the allocation-free orientation average's device array (scratch.hip): its
+// size, the address of this device's copy (looked up on first use, with the
+// device current), and the lock a call holds while it uses it
+size_t scratch_capacity();
+void *scratch_address(int device);
+std::mutex &scratch_lock(int device);
+// zero n doubles on the stream with a fill kernel
+void launch_fill_zero(double *p, size_t n, hipStream_t s);
+// device buffers allocated so far by DevBuf (pool allocations, not arena regions)
+long long device_allocations();
 
 // Rotation record of a docked leaf: m = R^T row-major (R = Rx(a) Ry(b) Rz(c) for
 // Euler angles, Amplitude.h:557-573, or a composition of such), the amplitude

@@ -103,6 +103,8 @@ SIGNATURES = {
     "dplus_pdb_build_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(PdbDesc), ctypes.c_int]),
     "dplus_orientation_average": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_int, _D,
                                                  ctypes.POINTER(OAParams), _D, ctypes.POINTER(OAStats)]),
+    "dplus_orientation_average_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_int, _D,
+                                                 ctypes.POINTER(OAParams), _D, ctypes.POINTER(OAStats)]),
     "dplus_dock_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_double]),
     "dplus_dock_grid_space_filling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Leaf),
                                                      ctypes.c_double, _D, _D, _D]),

@@ -191,6 +191,19 @@ def orientation_average(dock, q, method=L.OA_MC, seed=1, max_dirs=1 << 20, chunk
     return out, st
 
 
+def orientation_average_noalloc(dock, q, method=L.OA_MC, seed=1, max_dirs=1 << 20, chunk=8192, eps=0.0,
+                                vegas_iterations=10, vegas_points_per_bin=4):
+    """orientation_average (Monte Carlo or VEGAS) with every device buffer taken
+    from the library's fixed device scratch array: no device allocation."""
+    q = L.f64(q)
+    out = np.zeros(len(q))
+    p = L.OAParams(method, seed, max_dirs, chunk, eps, vegas_iterations, vegas_points_per_bin, 0)
+    st = L.OAStats()
+    L.check(L.lib().dplus_orientation_average_noalloc(dock.n, dock.arr, len(q), L.dp(q), ctypes.byref(p),
+                                                      L.dp(out), ctypes.byref(st)))
+    return out, st
+
+
 def dock_grid(out, dock, scale=1.0):
     L.check(L.lib().dplus_dock_grid(out.handle, dock.n, dock.arr, float(scale)))
 

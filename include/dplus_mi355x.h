@@ -199,6 +199,19 @@ typedef struct {
 int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const double *q,
                               const dplus_oa_params *p, double *intensity, dplus_oa_stats *st);
 
+/* Allocation-free variant of dplus_orientation_average, with no counterpart in
+ * the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Monte
+ * Carlo or VEGAS only (ERROR_INVALID_ARGS for Gauss-Kronrod); the same kernels
+ * and the same intensities as dplus_orientation_average.  Every device buffer
+ * (leaf tables, q and shell groups, MC block partials, VEGAS bin sums and
+ * history) comes from one static 64 MiB device array per GPU, held until the
+ * process ends; a call that would need more is rejected with
+ * ERROR_INVALID_ARGS before any work.  Calls on one GPU share the array, so
+ * overlapping calls would race on it: the library serialises them. */
+int dplus_orientation_average_noalloc(int nleaf, const dplus_leaf *leaves, int nq, const double *q,
+                                      const dplus_oa_params *p, double *intensity, dplus_oa_stats *st);
+
 /* ------------------------------------------------------------------ manual symmetry */
 /* ManualSymmetry::calculateGrid (DefaultModels/Symmetries.cpp:879-962, GPU
  * ConstructManualSymmetryJacobianKernel Backend/GPU/ManSymGenJacob.cu:35-282):
