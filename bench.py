@@ -182,34 +182,9 @@ def fit_sessions(D, S, n_sessions=16, evals=4, directions=8192, nq=600, qmax=5.0
     from dplus_mi355x.serving import FitSession
     rng = np.random.default_rng(seed)
     q = np.linspace(0.01, qmax, nq)
-    plans = []
     t0 = time.time()
-    for s in range(n_sessions):
-        gs = int(rng.choice([150, 200, 250]))
-        nleaf = int(rng.integers(1, 5))
-        copies = int(rng.integers(1, 34)) * 60
-        kind = "helix" if rng.random() < 0.5 else "icosahedral"
-        leaves = []
-        for l in range(nleaf):
-            n = int(rng.integers(2000, 20001))
-            R = (3.0 * n / (4.0 * np.pi * 50.0)) ** (1.0 / 3.0)
-            xyz, el = S.walk_structure(7000 + 10 * s + l, n, (R, R, R))
-            coef9, rad, ion, _ = S.structure_tables(el)
-            leaves.append((xyz, ion, coef9, rad))
-        plans.append(dict(gs=gs, leaves=leaves, copies=copies, kind=kind, r0=float(rng.uniform(6.0, 12.0)),
-                          tw0=float(rng.uniform(20.0, 40.0)), R0=float(rng.uniform(10.0, 20.0)), seed=s))
-
-    def placements(p, scale_r, dtw):
-        per = max(1, p["copies"] // len(p["leaves"]))
-        out = []
-        for l in range(len(p["leaves"])):
-            if p["kind"] == "helix":
-                rows = S.helix_rows(per, p["r0"] * scale_r, 0.9, p["tw0"] + dtw)
-                rows[:, 2] += 0.45 * l
-            else:
-                rows = S.icosahedral_rows(p["R0"] * scale_r, max(60, (per // 60) * 60), seed=p["seed"] * 7 + l)
-            out.append(S.rows_to_docking(rows))
-        return out
+    plans = S.session_plans(n_sessions, seed)
+    placements = S.session_placements
 
     def build(p, prec):
         gl = []

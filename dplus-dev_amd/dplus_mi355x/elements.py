@@ -56,12 +56,14 @@ def ion_tables(elements):
 
     Rows follow the species present, in table order (atoms by atomic number,
     then the implicit-hydrogen groups), matching the reference's ion-sorted atoms."""
-    order = [e for e in SPECIES if e in set(elements)]
-    unknown = set(elements) - set(order)
+    names, inverse = np.unique(np.asarray(elements, dtype=str), return_inverse=True)
+    present = set(names.tolist())
+    order = [e for e in SPECIES if e in present]
+    unknown = present - set(order)
     if unknown:
         raise ValueError(f"no X-ray form factor for {sorted(unknown)}")
     row = {e: i for i, e in enumerate(order)}
     coef9 = np.array([XRAY[e] for e in order], dtype=np.float64)
     rad = np.array([SVG_RADIUS[e] for e in order], dtype=np.float64)
-    ion = np.array([row[e] for e in elements], dtype=np.int32)
+    ion = np.array([row[e] for e in names.tolist()], dtype=np.int32)[inverse.ravel()]
     return coef9, rad, ion, order
