@@ -70,40 +70,100 @@ def sync():
         pass
 
 
-def cpu_baseline(W, g64_data, g64_interp, budget_nodes=1500):
-    """The FP64 oracle timed on a bounded sample of the same curve: the
-    amplitude of `budget_nodes` random grid nodes (all atoms and boxes) and one
-    VEGAS iteration of 1024 points per shell group, scaled to one full curve."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(W, g64_data, g64_interp, shells=(8, 16, 24)):
+    """The FP64 oracle timed on a bounded sample of the same curve: the amplitude
+    of every node of whole shells (all atoms and boxes), the splines, and one full
+    VEGAS iteration (1024 bins x the workload's points per bin, every shell
+    group), scaled to one curve: node time x P / sampled nodes + iteration time x
+    iterations."""
     import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     P = O.total_points(W["gs"])
-    rng = np.random.default_rng(0)
-    idx = rng.integers(1, P, budget_nodes)
-    qv = np.array([O.qvector_from_index(W["gs"], W["qmax"], int(i)) for i in idx])
+    idx = np.concatenate([O.index_from_indices(i, 0, 0) + np.arange(6 * i * (3 * i + 1)) for i in shells])
+    qv = np.array([O.qvector_from_index(W["gs"], W["qmax"], int(k)) for k in idx])
     counts = np.bincount(W["ion"], minlength=len(W["coef9"]))
     ions = O.Ions(W["coef9"], W["rad"], counts, W["sol"], 1.0, O.CALC_ATOMIC_FORMFACTORS | O.CALC_DUMMY_SOLVENT)
     t0 = time.time()
     O.amp_atoms(W["xyz"], W["ion"], ions, qv)
     O.amp_boxes(W["dims"], W["com"], W["vox"], W["contrast"], qv)
     t_nodes = time.time() - t0
-    t_grid = t_nodes * P / budget_nodes
+    t_grid = t_nodes * P / len(idx)
+    t0 = time.time()
     co = O.grid_splines(W["gs"], g64_data)
+    t_spl = time.time() - t0
     t0 = time.time()
     dock = O.Docked.single(W["gs"], W["qmax"], g64_data, co)
-    O.vegas_oa(dock, W["q"], 1, 1, 1)
-    t_oa = (time.time() - t0) * W["iters"] * W["ppb"]
-    total = t_grid + t_oa
-    return {"value": 1.0 / total, "unit": "curves/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle: amplitude of {budget_nodes} of {P} grid nodes over all "
-                      f"{len(W['xyz'])} atoms + {len(W['dims'])} boxes ({t_nodes:.1f} s, scaled x{P / budget_nodes:.0f}), "
-                      f"plus splines and one VEGAS iteration of 1024 points per shell group "
-                      f"(scaled x{W['iters'] * W['ppb']}); full curve estimate {total:.0f} s"}
+    O.vegas_oa(dock, W["q"], 1, 1, W["ppb"])
+    t_iter = time.time() - t0
+    total = t_grid + t_spl + t_iter * W["iters"]
+    return {"value": 1.0 / total, "unit": "curves/s", "cores": threads, "nproc": os.cpu_count(),
+            "cpu_model": cpu_model(), "kind": "port",
+            "sample": f"FP64 oracle on {threads} threads: every node of shells {list(shells)} ({len(idx)} of {P} "
+                      f"nodes) over all {len(W['xyz'])} atoms + {len(W['dims'])} boxes in {t_nodes:.1f} s (scaled "
+                      f"x{P / len(idx):.0f}), splines {t_spl:.1f} s, one full VEGAS iteration (1024 x {W['ppb']} "
+                      f"points per shell group) in {t_iter:.1f} s (scaled x{W['iters']}); full curve estimate "
+                      f"{total:.0f} s"}
 
 
-def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, streams=16, fp64=True,
+def docking_curve(D, S, directions=2_000_000):
+    """The docking workload: a 5,000-atom subunit (2.5 nm sphere) on a grid of 200
+    (qmax 6) docked as a 1,000-copy helix (radius 12 nm, rise 0.9 nm, twist 27.7
+    deg), 800 q on [0, 6], MC over `directions` directions (chunks of 8,192), one
+    curve in mixed precision and one in FP64, plus the gather ceiling of the
+    docked lookups on the same grid and rotations."""
+    xyz, el = S.walk_structure(3, 5000, (2.5, 2.5, 2.5))
+    coef9, rad, ion, _ = S.structure_tables(el)
+    rot, nt, tr = S.rows_to_docking(S.helix_rows(1000, 12.0, 0.9, 27.7))
+    q = np.linspace(0.0, 6.0, 800)
+    st = D.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=3)
+    out = {}
+    for name, prec in (("mixed", D.PRECISION_MIXED), ("fp64", D.PRECISION_FP64)):
+        g = D.Grid(200, 6.0, prec)
+        st.build(g)
+        D.timing_reset()
+        D.timing_enable(True)
+        sync()
+        t0 = time.perf_counter()
+        I, stt = D.orientation_average(D.Docking([(g, rot, nt, tr)]), q, seed=5, max_dirs=directions, chunk=8192)
+        sync()
+        dt = time.perf_counter() - t0
+        D.timing_enable(False)
+        out[name] = (I, dt, D.timing_get("mc")[1], stt.dirs_used, g)
+        log(f"[docking] {name}: {stt.dirs_used} directions in {dt:.2f} s (MC kernels {out[name][2]:.0f} ms)")
+    ngroups = len(np.unique(np.maximum(np.ceil(q / out["mixed"][4].step_size) - 1, 0)))
+    lookups = float(out["mixed"][3]) * ngroups * len(rot)
+    rate = lookups * 512 / (out["mixed"][2] * 1e-3)
+    pms, plk = D.gather_probe(out["mixed"][4], 32768, 0, ngroups, rot)
+    ceiling = plk * 512 / (pms * 1e-3)
+    err = float(np.max(np.abs(out["mixed"][0] - out["fp64"][0]) / np.abs(out["fp64"][0])))
+    log(f"[docking] mixed vs FP64 {err:.2e}; docked lookups {rate / 1e12:.2f} TB/s of nodes, gather ceiling "
+        f"{ceiling / 1e12:.2f} TB/s")
+    return {"workload": "5,000-atom subunit on grid 200 (qmax 6) docked as a 1,000-copy helix (R 12 nm, rise 0.9 nm, "
+                        "twist 27.7 deg), 800 q, MC 2e6 directions",
+            "value": 1.0 / out["mixed"][1], "unit": "curves/s", "fp64_value": 1.0 / out["fp64"][1],
+            "speedup_vs_fp64": out["fp64"][1] / out["mixed"][1], "max_rel_err_vs_fp64": err,
+            "directions": int(out["mixed"][3]),
+            "roofline": {"bound": "gather", "kernel": "mc_kernel<float, float4, false>",
+                         "achieved": rate / 1e9, "peak": ceiling / 1e9, "unit": "GB/s of nodes",
+                         "frac": rate / ceiling, "lookups": lookups, "bytes_per_lookup": 512,
+                         "ceiling": "dplus_gather_probe: the same lookups' node loads alone, 1,000 rotations"}}
+
+
+def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, streams=16, fp64=True,
                    batch_builds=False):
     """Batched serving stream: n_requests with Poisson arrivals at `rate`/s; each
-    request is one of `pool_size` seeded random-walk structures (atoms
+    request is its own seeded random-walk structure (or one of `pool_size`; atoms
     log-uniform in 500..200,000 at 50 atoms/nm^3, dummy-atom solvent), qmax
     U(2, 8) nm^-1, grid size round-up-10(2 qmax L / pi + 3) with L the structure's
     diameter, 200..1,000 q points, MC (65,536 directions) or VEGAS (5 x 4,096
@@ -117,7 +177,8 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, stre
     from dplus_mi355x.serving import Engine
     t0 = time.time()
     pool, reqs, arrivals = S.serving_workload(n_requests, pool_size, seed, rate)
-    log(f"[serving] {n_requests} requests over {pool_size} structures prepared in {time.time() - t0:.1f} s")
+    nstruct = n_requests if pool_size is None else pool_size
+    log(f"[serving] {n_requests} requests over {nstruct} structures prepared in {time.time() - t0:.1f} s")
 
     def run_mixed(arr):
         eng = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=streams, batch_builds=batch_builds)
@@ -160,7 +221,7 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=48, seed=4, stre
     eng64.close()
     log(f"[serving] plain FP64, one request at a time: {n_requests / span64:.2f} curves/s; "
         f"max rel err mixed vs FP64 {max(errs):.2e}")
-    return {"workload": f"{n_requests} Poisson requests at {rate:g}/s over {pool_size} seeded structures "
+    return {"workload": f"{n_requests} Poisson requests at {rate:g}/s over {nstruct} seeded structures "
                         "(500..200,000 atoms), qmax U(2,8), 200..1,000 q, MC 65,536 dirs or VEGAS 5x4,096",
             "value": n_requests / span, "unit": "curves/s", "p50_ms": float(np.percentile(lat, 50)),
             "p99_ms": float(np.percentile(lat, 99)), "fp64_value": n_requests / span64,
@@ -261,9 +322,11 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream and fit-session measurements")
     ap.add_argument("--serving-requests", type=int, default=512)
     ap.add_argument("--session-evals", type=int, default=4)
-    ap.add_argument("--pmc-probe", action="store_true",
-                    help="one mixed build, a 256 MiB device copy (counter calibration), one VEGAS curve and one "
-                         "MC average of 262,144 directions on the built grid, for rocprofv3 --pmc")
+    ap.add_argument("--pmc-probe", "--probe", dest="pmc_probe", action="store_true",
+                    help="one mixed build of the 50k-atom grid, a 256 MiB device copy (counter calibration), one "
+                         "VEGAS curve, one MC average of 262,144 directions and the gather probe on that grid, then "
+                         "the 1,000-copy helix docked MC at 32,768 directions and its gather probe: for rocprofv3")
+    ap.add_argument("--no-docking", action="store_true", help="skip the 1,000-copy helix docking curve")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (the path shards by shell "
@@ -294,11 +357,30 @@ def main():
         D.timing_enable(False)
         # shell groups: q in (l step, (l + 1) step] share layer l, q = 0 joins layer 0
         ngroups = len(np.unique(np.maximum(np.ceil(W["q"] / g32.step_size) - 1, 0)))
-        print(json.dumps({"probe": "build_atoms_pairs + copy_probe_kernel + vegas_sample + mc_kernel",
+        vegas_ms, mc_ms = D.timing_get("vegas")[1], D.timing_get("mc")[1]
+        g_ms, g_lk = D.gather_probe(g32, 1 << 18, 0, ngroups)
+        # the docking workload's MC at 32,768 directions and its gather probe
+        xyz, el = S.walk_structure(3, 5000, (2.5, 2.5, 2.5))
+        coef9, rad, ion, _ = S.structure_tables(el)
+        rot, nt, tr = S.rows_to_docking(S.helix_rows(1000, 12.0, 0.9, 27.7))
+        q3 = np.linspace(0.0, 6.0, 800)
+        g3 = D.Grid(200, 6.0, D.PRECISION_MIXED)
+        D.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=3).build(g3)
+        D.timing_reset()
+        D.timing_enable(True)
+        D.orientation_average(D.Docking([(g3, rot, nt, tr)]), q3, seed=5, max_dirs=32768, chunk=8192)
+        D.timing_enable(False)
+        ng3 = len(np.unique(np.maximum(np.ceil(q3 / g3.step_size) - 1, 0)))
+        d_ms, d_lk = D.gather_probe(g3, 32768, 0, ng3, rot)
+        print(json.dumps({"probe": "build_atoms_pairs + copy_probe_kernel + vegas_sample + mc_kernel + gather_probe "
+                                   "(50k-atom grid 250), mc_kernel docked + gather_probe<true> (1,000-copy helix)",
                           "copy_bytes": nbytes, "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9,
                           "shell_groups": ngroups,
-                          "vegas_ms": D.timing_get("vegas")[1], "vegas_units": W["iters"] * 1024 * W["ppb"] * ngroups,
-                          "mc_ms": D.timing_get("mc")[1], "mc_units": (1 << 18) * ngroups,
+                          "vegas_ms": vegas_ms, "vegas_units": W["iters"] * 1024 * W["ppb"] * ngroups,
+                          "mc_ms": mc_ms, "mc_units": (1 << 18) * ngroups,
+                          "gather_probe_ms": g_ms, "gather_probe_units": g_lk,
+                          "docked_mc_ms": D.timing_get("mc")[1], "docked_mc_units": 32768 * ng3 * len(rot),
+                          "docked_gather_probe_ms": d_ms, "docked_gather_probe_units": d_lk,
                           "node_bytes_per_unit": 512}))
         return
     g64 = D.Grid(W["gs"], W["qmax"], D.PRECISION_FP64)
@@ -354,6 +436,19 @@ def main():
     log(f"[bench] GK (tol 1e-4): mixed {gk['mixed'][1]:.3f} s, fp64 {gk['fp64'][1]:.3f} s per average "
         f"({gk['mixed'][2]} integrand evaluations); max rel I(q) mixed vs fp64 = {gk_err:.2e}")
 
+    # ---- the VEGAS kernels against the gather ceiling of the same lookups on the same grid
+    ngroups = len(np.unique(np.maximum(np.ceil(W["q"] / g32.step_size) - 1, 0)))
+    v_lookups = float(W["iters"]) * 1024 * W["ppb"] * ngroups
+    v_rate = v_lookups * 512 / (ms_vegas / max(args.steps, 1) * 1e-3)
+    p_ms, p_lk = D.gather_probe(g32, 1 << 18, 0, ngroups)
+    ceiling = p_lk * 512 / (p_ms * 1e-3)
+    oa_roof = {"bound": "gather", "kernel": "vegas_sample<float, float4, true> + vegas_reduce",
+               "achieved": v_rate / 1e9, "peak": ceiling / 1e9, "unit": "GB/s of nodes", "frac": v_rate / ceiling,
+               "lookups_per_curve": v_lookups, "bytes_per_lookup": 512,
+               "ceiling": "dplus_gather_probe: the same lookups' node loads alone at full occupancy"}
+    log(f"[bench] VEGAS lookups {v_rate / 1e12:.2f} TB/s of nodes, gather ceiling {ceiling / 1e12:.2f} TB/s "
+        f"({v_rate / ceiling:.2f})")
+
     # ---- roofline of the dominant kernel (the amplitude builder, VALU-bound)
     N = W["gs"] // 2 + 3
     nhalf = 3 * (N + 1) ** 2 * N
@@ -377,6 +472,8 @@ def main():
         g64.download()
         cpu = cpu_baseline(W, g64.data.copy(), g64.interp.copy())
         log(f"[bench] cpu baseline: {cpu}")
+
+    docking = None if args.no_docking else docking_curve(D, S)
 
     serving = sessions = None
     if not args.no_serving:
@@ -417,7 +514,9 @@ def main():
                      "hbm": None if traffic is None else {
                          "achieved": traffic / t_launch / 1e9, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
                          "frac": traffic / t_launch / 1e9 / PEAK_HBM_GBPS}},
+        "oa_roofline": oa_roof,
         "cpu_baseline": cpu,
+        "docking": docking,
         "serving": serving,
         "fit_sessions": sessions,
     }

@@ -1105,6 +1105,8 @@ void launch_seeded_samples(bool fp64, const void *g, int N, double step, double 
                            const double *phi, const double *v2, double *out, hipStream_t s);
 void launch_plane_batch(bool fp64, const void *g, int n, const int *ri, const double *th, const double *ph,
                         double2 *out, hipStream_t s);
+void launch_gather_probe(const void *grid, int N, const RotDev *rots, int nrot, int layer0, int nlayers, int nblk,
+                         uint64_t seed, int chunk, float *sink, hipStream_t s);
 }  // namespace dplus
 extern "C" {
 
@@ -1528,6 +1530,50 @@ int dplus_copy_probe(long long bytes, double *ms) {
         if (bytes <= 0 || bytes % 16) throw Error(kErrInvalidArgs, "bytes must be a positive multiple of 16");
         Device::get().require();
         copy_probe(Device::get().stream, bytes, ms);
+    })
+}
+
+int dplus_gather_probe(dplus_grid *gh, int nrot, const double *rot, long long directions, int layer0, int nlayers,
+                       double *ms, double *lookups) {
+    API_GUARD({
+        Grid *g = (Grid *)gh;
+        if (!g || g->fp64()) throw Error(kErrInvalidArgs, "the gather probe reads a mixed-precision grid");
+        if (nrot < 0 || (nrot > 0 && !rot) || directions < 256 || directions % 256 || layer0 < 0 || nlayers < 1 ||
+            layer0 + nlayers > g->N - 1)
+            throw Error(kErrInvalidArgs, "bad gather-probe arguments");
+        if (directions / 256 > 2147483647LL) throw Error(kErrInvalidArgs, "too many directions");
+        Device::get().require();
+        hipStream_t s = Device::get().stream;
+        const int nblk = (int)(directions / 256);
+        const int chunk = directions % 8192 == 0 ? 8192 : 256;
+        DevBuf brot, bsink;
+        std::vector<RotDev> rots((size_t)std::max(1, nrot));
+        for (int r = 0; r < nrot; r++) {
+            euler_rt(rot[4 * r], rot[4 * r + 1], rot[4 * r + 2], rots[r].m);
+            rots[r].scale = rot[4 * r + 3];
+            rots[r].leaf = rots[r].t0 = rots[r].pad = 0;
+            rots[r].nt = 1;
+        }
+        brot.upload(rots.data(), rots.size() * sizeof(RotDev), s);
+        bsink.ensure((size_t)nblk * nlayers * 4 * sizeof(float));
+        hipEvent_t e0, e1;
+        check_hip(hipEventCreate(&e0), "event");
+        check_hip(hipEventCreate(&e1), "event");
+        // one untimed launch brings the grid into the caches, as in an average
+        launch_gather_probe(g->dev.p, g->N, nrot ? (const RotDev *)brot.p : nullptr, std::max(1, nrot), layer0,
+                            nlayers, nblk, 12345, chunk, (float *)bsink.p, s);
+        check_hip(hipEventRecord(e0, s), "record");
+        launch_gather_probe(g->dev.p, g->N, nrot ? (const RotDev *)brot.p : nullptr, std::max(1, nrot), layer0,
+                            nlayers, nblk, 777, chunk, (float *)bsink.p, s);
+        check_hip(hipGetLastError(), "gather probe launch");
+        check_hip(hipEventRecord(e1, s), "record");
+        check_hip(hipEventSynchronize(e1), "sync");
+        float t = 0.f;
+        check_hip(hipEventElapsedTime(&t, e0, e1), "elapsed");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms) *ms = t;
+        if (lookups) *lookups = (double)directions * nlayers * std::max(1, nrot);
     })
 }
 

@@ -307,6 +307,89 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T
     }
 }
 
+// ------------------------------------------------------------------ gather ceiling
+// The node reads of the mixed MC average without its arithmetic: for every
+// (direction, layer) of an MC launch (the same stratified Philox directions) and
+// every rotation, the 4 planes x 4 theta rows x 2 phi nodes of the lookup
+// (plane_lookup's cells; the antipode / origin planes of layers 0 and 1 read
+// shell 1) are loaded and summed, and each wave writes one float.  The rotated
+// directions' angles are formed in FP32.  With few registers it runs at full
+// occupancy, so its rate is the ceiling of these gathers on the device.
+__device__ __forceinline__ void plane_gather(const float4 *__restrict__ plane, int shell, float ut, float up,
+                                             float4 &acc) {
+    const int phiPoints = 6 * shell, thePoints = 3 * shell;
+    const int tI = (int)(ut * (float)thePoints);
+    int pI = (int)(up * (float)phiPoints);
+    pI = (pI >= phiPoints) ? 0 : pI;
+    const int pI1 = (pI + 1 == phiPoints) ? 0 : pI + 1;
+    const int half = phiPoints / 2;
+#pragma unroll
+    for (int rl = -1; rl < 3; rl++) {
+        const int v = tI + rl;
+        int pos1, pos2;
+        if (v < 0 || v > thePoints) {
+            const int theInd = v < 0 ? v + thePoints + 1 : v - (thePoints + 1);
+            pos1 = phiPoints * theInd + pI + ((2 * pI >= phiPoints) ? -half : half);
+            pos2 = phiPoints * theInd + pI1 + ((2 * pI1 >= phiPoints) ? -half : half);
+        } else {
+            pos1 = phiPoints * v + pI;
+            pos2 = phiPoints * v + pI1;
+        }
+        const float4 a = plane[pos1], b = plane[pos2];
+        acc.x += a.x + b.x;
+        acc.y += a.y + b.y;
+        acc.z += a.z + b.z;
+        acc.w += a.w + b.w;
+    }
+}
+
+template <bool kRot>
+__global__ __launch_bounds__(256) void gather_probe(const float4 *__restrict__ g, int N,
+                                                    const RotDev *__restrict__ rots, int nrot, int layer0,
+                                                    uint64_t seed, int chunk, int nu, int nv,
+                                                    float *__restrict__ sink) {
+    const int layer = layer0 + blockIdx.y;
+    const long long d = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double u, v;
+    uniform_pair(seed, (uint32_t)d, (uint32_t)(d >> 32), (uint32_t)layer, kStreamMC, u, v);
+    mc_stratify(d, chunk, nu, nv, u, v);
+    const float th = acosf((float)(2.0 * u - 1.0)), ph = (float)(k2Pi * v);
+    float st, ct, sp, cp;
+    sincosf(th, &st, &ct);
+    sincosf(ph, &sp, &cp);
+    const float qv[3] = {cp * st, sp * st, ct};
+    float4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int nr = kRot ? nrot : 1;
+    for (int r = 0; r < nr; r++) {
+        float ut, up;
+        if (kRot) {
+            const double *m = rots[r].m;
+            float rv[3];
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                rv[i] = (float)m[3 * i] * qv[0] + (float)m[3 * i + 1] * qv[1] + (float)m[3 * i + 2] * qv[2];
+            rv[2] = fminf(fmaxf(rv[2], -1.f), 1.f);
+            float a = atan2f(rv[1], rv[0]);
+            if (a < 0.f) a += (float)k2Pi;
+            ut = acosf(rv[2]) * (float)kInvPi;
+            up = fminf(a * (float)kInv2Pi, 1.f);
+        } else {
+            ut = fminf(th * (float)kInvPi, 1.f);
+            up = (float)v;
+        }
+#pragma unroll
+        for (int i = -1; i <= 2; i++) {
+            const int lq = min(max(layer + i, 1), N);
+            plane_gather(g + shell_base(lq), lq, ut, up, acc);
+        }
+    }
+    float t = acc.x + acc.y + acc.z + acc.w;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+    if ((threadIdx.x & 63) == 0)
+        sink[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6)] = t;
+}
+
 // sum the block partials of every active group in a fixed order:
 // out[gi * mstride + j]
 __global__ void mc_reduce(const GroupDev *__restrict__ groups, const int *__restrict__ active,
@@ -833,6 +916,21 @@ void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *r
         DPLUS_MC(float, float4, false);
 #undef DPLUS_MC
     mc_reduce<<<nactive, 64, 0, s>>>(groups, active, partial, nblk, mstride, out);
+}
+
+// the gather-ceiling probe over layers [layer0, layer0 + nlayers) of a mixed grid
+// with N shells: nblk blocks of 256 directions per layer; sink holds
+// nblk * nlayers * 4 floats
+void launch_gather_probe(const void *grid, int N, const RotDev *rots, int nrot, int layer0, int nlayers, int nblk,
+                         uint64_t seed, int chunk, float *sink, hipStream_t s) {
+    int nu, nv;
+    mc_strata(chunk, nu, nv);
+    dim3 gr(nblk, nlayers);
+    if (rots)
+        gather_probe<true><<<gr, 256, 0, s>>>((const float4 *)grid, N, rots, nrot, layer0, seed, chunk, nu, nv, sink);
+    else
+        gather_probe<false><<<gr, 256, 0, s>>>((const float4 *)grid, N, nullptr, 1, layer0, seed, chunk, nu, nv,
+                                               sink);
 }
 
 void launch_vegas_iter(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot,

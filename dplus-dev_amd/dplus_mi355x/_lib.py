@@ -96,6 +96,8 @@ SIGNATURES = {
     "dplus_grid_calculate_splines": (ctypes.c_int, [ctypes.c_void_p]),
     "dplus_grid_download": (ctypes.c_int, [ctypes.c_void_p]),
     "dplus_grid_get_sphr": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, _D, _D, _D]),
+    "dplus_gather_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _D, ctypes.c_longlong, ctypes.c_int,
+                                          ctypes.c_int, _D, _D]),
     "dplus_grid_interpolate_theta_phi_plane": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I, _D, _D, _D]),
     "dplus_grid_calculate_intensity": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                                       ctypes.c_double, ctypes.c_uint, ctypes.c_ulonglong,
@@ -249,3 +251,12 @@ def copy_probe(nbytes):
     ms = ctypes.c_double()
     check(lib().dplus_copy_probe(int(nbytes), ctypes.byref(ms)))
     return ms.value
+
+
+def gather_probe(grid, directions, layer0, nlayers, rot=None):
+    """dplus_gather_probe: (ms, lookups) of the MC lookups' node loads alone."""
+    ms, n = ctypes.c_double(), ctypes.c_double()
+    r = None if rot is None else f64(rot).reshape(-1, 4)
+    check(lib().dplus_gather_probe(grid.handle, 0 if r is None else len(r), None if r is None else dp(r),
+                                   int(directions), int(layer0), int(nlayers), ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value

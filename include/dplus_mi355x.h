@@ -162,6 +162,17 @@ int dplus_microbench_valu(double *fma32, double *sin32, double *cos32, double *f
  * known-size kernel hardware byte counters are calibrated against. */
 int dplus_copy_probe(long long bytes, double *ms);
 
+/* Gather ceiling of the orientation averages (no counterpart in the reference):
+ * the node loads of a mixed MC average without its arithmetic -- for each of
+ * `directions` stratified directions (a multiple of 256) of every layer in
+ * [layer0, layer0 + nlayers) and each of nrot rotations (alpha, beta, gamma,
+ * scale per rotation; nrot = 0: the drawn direction itself), 4 planes x 4 theta
+ * rows x 2 phi nodes of 16 B -- at full occupancy on a mixed grid.  Writes the
+ * launch time (ms, after one untimed launch) and the lookups it made (each
+ * 512 B of nodes). */
+int dplus_gather_probe(dplus_grid *g, int nrot, const double *rot, long long directions, int layer0, int nlayers,
+                       double *ms, double *lookups);
+
 /* ------------------------------------------------------------------ orientation average */
 /* A leaf of the hybrid regime (Amplitude.cpp:2028-2303): a grid seen under nrot
  * rotations rot[4r..4r+3] = (alpha, beta, gamma [rad], scale), rotation r placed
