@@ -110,7 +110,8 @@ __device__ __forceinline__ float edge_weight(float q, int n, float vox) {
 
 // Tiles: (first, count, index, kind).  kind 0: atoms of ion `index`, weighted by
 // the ion's form factor at the node's shell; kind 1: solvent boxes of shape
-// bdims[index], weighted by contrast * box_weight at the node.
+// bdims[index], weighted by that shape's contrast bcon[index] * box_weight at
+// the node (hydration-layer and excluded-volume boxes carry their own contrast).
 //
 // build_atoms_phase64: FP32 grid and FP32 tile sums, but every phase q.r formed
 // and reduced in FP64 -- a diagnostic form of the mixed builder (environment
@@ -118,7 +119,7 @@ __device__ __forceinline__ float edge_weight(float q, int n, float vox) {
 template <int K>
 __global__ __launch_bounds__(256) void build_atoms_phase64(
     const double4 *__restrict__ atoms64, const int4 *__restrict__ tiles, int ntiles, const double *__restrict__ ff, int nion,
-    const int4 *__restrict__ bdims, double vox, double contrast,
+    const int4 *__restrict__ bdims, const double *__restrict__ bcon, double vox,
     double step, long long nhalf, float4 *__restrict__ grid, double scale) {
     const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * K;
     float qx[K], qy[K], qz[K];
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(256) void build_atoms_phase64(
             const float k2 = (float)k2Pi;
 #pragma unroll
             for (int p = 0; p < K; p++) {
-                double f = contrast * (double)box_weight32(qx[p] * k2, qy[p] * k2, qz[p] * k2, d, (float)vox);
+                double f = bcon[tl.z] * (double)box_weight32(qx[p] * k2, qy[p] * k2, qz[p] * k2, d, (float)vox);
                 accR[p] = fma(f, (double)sr[p], accR[p]);
                 accI[p] = fma(f, (double)si[p], accI[p]);
             }
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs(BuildBatch batch) {
 #pragma unroll
         for (int p = 0; p < kPair; p++) {
             const double f = tl.w == 0 ? fion
-                                       : J.contrast * (double)(wz * edge_weight(qx[p] * k2, d.x, vox) *
+                                       : J.bcon[tl.z] * (double)(wz * edge_weight(qx[p] * k2, d.x, vox) *
                                                                edge_weight(qy[p] * k2, d.y, vox));
             aR[p] = fma(f, (double)(cc[p] - ss[p]), aR[p]);
             aI[p] = fma(f, (double)(sc[p] + cs[p]), aI[p]);
@@ -425,8 +426,8 @@ __global__ __launch_bounds__(256) void pairs_finish(BuildBatch batch) {
 template <int K>
 __global__ __launch_bounds__(256) void build_atoms_fp64(
     const double4 *__restrict__ atoms, const int4 *__restrict__ tiles, int ntiles,
-    const double *__restrict__ ff, int nion, const int4 *__restrict__ bdims, double vox,
-    double contrast, double step, long long nhalf, double4 *__restrict__ grid, double scale) {
+    const double *__restrict__ ff, int nion, const int4 *__restrict__ bdims, const double *__restrict__ bcon,
+    double vox, double step, long long nhalf, double4 *__restrict__ grid, double scale) {
     const long long c0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * K;
     double qx[K], qy[K], qz[K], accR[K], accI[K];
     NodeInfo node[K];
@@ -457,7 +458,7 @@ __global__ __launch_bounds__(256) void build_atoms_fp64(
 #pragma unroll
         for (int p = 0; p < K; p++) {
             double f = (tl.w == 0) ? ff[(size_t)node[p].shell * nion + tl.z]
-                                   : contrast * box_weight64(qx[p], qy[p], qz[p], bdims[tl.z], vox);
+                                   : bcon[tl.z] * box_weight64(qx[p], qy[p], qz[p], bdims[tl.z], vox);
             accR[p] = fma(f, sr[p], accR[p]);
             accI[p] = fma(f, si[p], accI[p]);
         }
@@ -516,19 +517,19 @@ constexpr int kNodesPerThread = 4;
 
 void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms, const void *atoms64,
                         const int4 *tiles, int ntiles, const double *ff, int nion, const int4 *bdims,
-                        double vox, double contrast, double step, long long nhalf, void *grid,
+                        const double *bcon, double vox, double step, long long nhalf, void *grid,
                         double scale, hipStream_t s) {
     const int K = kNodesPerThread;
     unsigned nb = blocks_for(nhalf, 256 * K);
     if (fp64) {
         build_atoms_fp64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, bdims,
-                                               vox, contrast, step, nhalf, (double4 *)grid, scale);
+                                               bcon, vox, step, nhalf, (double4 *)grid, scale);
     } else {
         // mixed grids reach this launcher only for FP64 phases (the paired builder
         // takes every other mixed build)
         (void)phase64;
         build_atoms_phase64<K><<<nb, 256, 0, s>>>((const double4 *)atoms64, tiles, ntiles, ff, nion, bdims,
-                                                  vox, contrast, step, nhalf, (float4 *)grid, scale);
+                                                  bcon, vox, step, nhalf, (float4 *)grid, scale);
     }
 }
 

@@ -36,6 +36,9 @@ namespace dplus {
 
 void hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness, double vox,
                      bool fill_holes, std::vector<int> &dims, std::vector<double> &com);
+void solvent_boxes(int n, const double *xyz, const double *rad, double probe, double thickness, double vox,
+                   bool fill_holes, bool want_layer, bool want_excl, std::vector<int> &ldims, std::vector<double> &lcom,
+                   std::vector<int> &edims, std::vector<double> &ecom);
 void euler_rt(double a, double b, double c, double m[9]);
 std::string amp_legacy_bytes(Grid &g, const std::string &header, hipStream_t s);
 void amp_write(Grid &g, const std::string &path, const std::string &title, hipStream_t s);
@@ -360,21 +363,30 @@ struct Backend {
                         rad.push_back(kElements[e].svg_radius);
                         break;
                     }
-            std::vector<int> ion(natoms);
-            std::vector<double> arad(natoms);
-            for (int a = 0; a < natoms; a++) {
-                ion[a] = row[A.species[a]];
-                arad[a] = kElements[A.species[a]].svg_radius;
-            }
             // ExtraParameters (PDBAmplitude.cpp:1205-1214)
             const double scale = p[0], sol_ed = p[1], c1 = p[2], vox = p[3], probe = p[4], thick = p[5], out_ed = p[6];
             const bool solvent_only = std::fabs(p[8] - 1.0) < 1e-6;
             const int rad_type = (int)(p[9] + 0.1);
-            if ((sol_ed != 0.0 || out_ed != 0.0) && (rad_type == 1 || rad_type == 2 || rad_type == 3 || rad_type == 5))
-                throw Error(kErrUnimplemented, "voxelised excluded volume and van der Waals / empirical / calculated "
-                                               "radii (radius types 1, 2, 3, 5) are not on this path; use radius type "
-                                               "4 (dummy atoms)");
             const bool fill_holes = std::fabs(p[7] - 1.0) < 1e-6;
+            // radii of the voxel marking by radius type (PDBReaderOb::SetRadiusType,
+            // PDBReaderLib.cpp:1207-1236): 1 van der Waals (also the default), 2
+            // empirical, 3 calculated, 4 and 5 the dummy-atom radii
+            std::vector<int> ion(natoms);
+            std::vector<double> arad(natoms);
+            for (int a = 0; a < natoms; a++) {
+                const ElementInfo &E = kElements[A.species[a]];
+                ion[a] = row[A.species[a]];
+                arad[a] = rad_type == 4 || rad_type == 5 ? E.svg_radius
+                          : rad_type == 2                ? E.emp_radius
+                          : rad_type == 3                ? E.calc_radius
+                                                         : E.vdw_radius;
+            }
+            // the voxelised solvent of PDBAmplitude::PreCalculate (PDBAmplitude.cpp:1250-1263):
+            // the excluded volume for radius types 1, 2, 3, 5 with a solvent density,
+            // the outer-solvent layer for types 1..5 with an outer-solvent density
+            const bool voxel_types = rad_type == 1 || rad_type == 2 || rad_type == 3 || rad_type == 5;
+            const bool want_excl = sol_ed != 0.0 && voxel_types && vox > 0.0;
+            const bool want_layer = out_ed != 0.0 && (voxel_types || rad_type == 4) && vox > 0.0;
             dplus_pdb_desc d{};
             d.natoms = natoms;
             d.xyz = xyz.data();
@@ -387,15 +399,20 @@ struct Backend {
             d.flags = (solvent_only ? 0 : DPLUS_CALC_ATOMIC_FORMFACTORS) |
                       ((rad_type == 4 && sol_ed != 0.0) ? DPLUS_CALC_DUMMY_SOLVENT : 0);
             d.scale = scale;
-            std::vector<int> bdims;
-            std::vector<double> bcom;
-            if (out_ed != 0.0 && rad_type >= 1 && rad_type <= 5) {
-                hydration_boxes(natoms, xyz.data(), arad.data(), probe, thick, vox, fill_holes, bdims, bcom);
+            std::vector<int> bdims, edims;
+            std::vector<double> bcom, ecom;
+            if (want_excl || want_layer) {
+                solvent_boxes(natoms, xyz.data(), arad.data(), probe, thick, vox, fill_holes, want_layer, want_excl,
+                              bdims, bcom, edims, ecom);
+                d.vox_step = vox;
                 d.nbox = (int)(bdims.size() / 3);
                 d.box_dims = bdims.data();
                 d.box_com = bcom.data();
-                d.vox_step = vox;
                 d.box_contrast = out_ed - sol_ed;
+                d.nexcl = (int)(edims.size() / 3);
+                d.excl_dims = edims.data();
+                d.excl_com = ecom.data();
+                d.excl_contrast = -sol_ed;
             }
             sit = structures.emplace(skey, std::unique_ptr<Structure>(new Structure(&d, st))).first;
         }

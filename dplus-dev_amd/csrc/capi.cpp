@@ -298,8 +298,9 @@ void Grid::upload(hipStream_t s) {
 
 // ------------------------------------------------------------------ PDB builder
 void launch_build_atoms(bool fp64, bool phase64, bool reduce, const void *atoms, const void *atoms64,
-                        const int4 *tiles, int ntiles, const double *ff, int nion, const int4 *bdims, double vox,
-                        double contrast, double step, long long nhalf, void *grid, double scale, hipStream_t s);
+                        const int4 *tiles, int ntiles, const double *ff, int nion, const int4 *bdims,
+                        const double *bcon, double vox, double step, long long nhalf, void *grid, double scale,
+                        hipStream_t s);
 void launch_set_origin(bool fp64, void *grid, double re, hipStream_t s);
 
 // Atomic Form Factor.cpp:11-27 (cube-mean radius), 91-115 (dummy-atom
@@ -344,28 +345,45 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
     if (d->natoms < 0 || d->nion <= 0 || d->nion > 255) throw Error(kErrInvalidArgs, "bad atom/ion counts");
     if (d->nbox < 0 || (d->nbox > 0 && (!d->box_dims || !d->box_com || !(d->vox_step > 0))))
         throw Error(kErrInvalidArgs, "bad solvent boxes");
+    if (d->nexcl < 0 || (d->nexcl > 0 && (!d->excl_dims || !d->excl_com || !(d->vox_step > 0))))
+        throw Error(kErrInvalidArgs, "bad excluded-volume boxes");
     for (int a = 0; a < d->natoms; a++)
         if (d->ion[a] < 0 || d->ion[a] >= d->nion) throw Error(kErrInvalidArgs, "ion index out of range");
     const bool atoms_on = (d->flags & (DPLUS_CALC_ATOMIC_FORMFACTORS | DPLUS_CALC_DUMMY_SOLVENT)) && d->natoms > 0;
-    const bool boxes_on = d->nbox > 0 && d->box_contrast != 0.0;
+    // box sets: the outer-solvent (hydration) layer added with box_contrast and the
+    // excluded volume added with excl_contrast (PDBAmplitude.cpp:277-314)
+    struct BoxSet {
+        int n;
+        const int *dims;
+        const double *com;
+        double contrast;
+    };
+    const BoxSet sets[2] = {{d->box_contrast != 0.0 ? d->nbox : 0, d->box_dims, d->box_com, d->box_contrast},
+                            {d->excl_contrast != 0.0 ? d->nexcl : 0, d->excl_dims, d->excl_com, d->excl_contrast}};
+    const bool boxes_on = sets[0].n > 0 || sets[1].n > 0;
     // atoms sorted by ion (ExtractRelevantCoeffs, PDBReaderLib.cpp:1443-1509), then
-    // boxes sorted by shape (SortSolventBoxEntry, PDBAmplitude.cpp:161-188)
+    // each box set sorted by shape (SortSolventBoxEntry, PDBAmplitude.cpp:161-188)
     std::vector<int> order(d->natoms);
     for (int a = 0; a < d->natoms; a++) order[a] = a;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d->ion[x] < d->ion[y]; });
-    std::vector<int> border(boxes_on ? d->nbox : 0);
-    for (size_t b = 0; b < border.size(); b++) border[b] = (int)b;
-    auto bkey = [&](int b) {
-        return std::make_tuple(d->box_dims[3 * b], d->box_dims[3 * b + 1], d->box_dims[3 * b + 2], d->box_com[3 * b],
-                               d->box_com[3 * b + 1], d->box_com[3 * b + 2]);
-    };
-    std::sort(border.begin(), border.end(), [&](int x, int y) { return bkey(x) < bkey(y); });
-    const size_t nsrc = (atoms_on ? d->natoms : 0) + border.size();
+    std::vector<int> border[2];
+    for (int si = 0; si < 2; si++) {
+        const BoxSet &B = sets[si];
+        border[si].resize(B.n);
+        for (int b = 0; b < B.n; b++) border[si][b] = b;
+        auto bkey = [&](int b) {
+            return std::make_tuple(B.dims[3 * b], B.dims[3 * b + 1], B.dims[3 * b + 2], B.com[3 * b], B.com[3 * b + 1],
+                                   B.com[3 * b + 2]);
+        };
+        std::sort(border[si].begin(), border[si].end(), [&](int x, int y) { return bkey(x) < bkey(y); });
+    }
+    const size_t nsrc = (atoms_on ? d->natoms : 0) + border[0].size() + border[1].size();
     std::vector<float4> at32(std::max<size_t>(1, nsrc));
     std::vector<double4> at64(std::max<size_t>(1, nsrc));
     std::vector<int> counts(d->nion, 0);
     for (int a = 0; a < d->natoms; a++) counts[d->ion[a]]++;
     std::vector<int4> tiles, bdims;
+    std::vector<double> bcon;
     size_t k = 0;
     if (atoms_on) {
         for (int i = 0; i < d->natoms; i++) {
@@ -383,24 +401,29 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
             i = e;
         }
     }
-    double box_volume = 0.0;
-    if (boxes_on) {
+    double box_origin = 0.0;
+    for (int si = 0; si < 2; si++) {
+        const BoxSet &B = sets[si];
+        const std::vector<int> &bo = border[si];
+        if (bo.empty()) continue;
         const size_t base = k;
-        for (int b : border) {
-            const double *p = d->box_com + 3 * b;
+        double volume = 0.0;
+        for (int b : bo) {
+            const double *p = B.com + 3 * b;
             at32[k] = make_float4((float)p[0], (float)p[1], (float)p[2], 0.f);
             at64[k] = make_double4(p[0], p[1], p[2], 0.0);
-            box_volume += (double)d->box_dims[3 * b] * d->box_dims[3 * b + 1] * d->box_dims[3 * b + 2];
+            volume += (double)B.dims[3 * b] * B.dims[3 * b + 1] * B.dims[3 * b + 2];
             k++;
         }
-        box_volume *= d->vox_step * d->vox_step * d->vox_step;
+        box_origin += B.contrast * volume * d->vox_step * d->vox_step * d->vox_step;
         size_t i = 0;
-        while (i < border.size()) {
-            const int *di = d->box_dims + 3 * border[i];
-            bdims.push_back(make_int4(di[0], di[1], di[2], 0));
+        while (i < bo.size()) {
+            const int *di = B.dims + 3 * bo[i];
+            bdims.push_back(make_int4(di[0], di[1], di[2], si));
+            bcon.push_back(B.contrast);
             size_t e = i;
-            while (e < border.size() && e - i < 256) {
-                const int *de = d->box_dims + 3 * border[e];
+            while (e < bo.size() && e - i < 256) {
+                const int *de = B.dims + 3 * bo[e];
                 if (de[0] != di[0] || de[1] != di[1] || de[2] != di[2]) break;
                 e++;
             }
@@ -408,18 +431,23 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
             i = e;
         }
     }
-    if (bdims.empty()) bdims.push_back(make_int4(0, 0, 0, 0));
+    if (bdims.empty()) {
+        bdims.push_back(make_int4(0, 0, 0, 0));
+        bcon.push_back(0.0);
+    }
     desc = *d;
     desc.xyz = nullptr;
     desc.ion = nullptr;
     desc.box_dims = nullptr;
     desc.box_com = nullptr;
+    desc.excl_dims = nullptr;
+    desc.excl_com = nullptr;
     coef9.assign(d->coef9, d->coef9 + 9 * d->nion);
     rad.assign(d->rad, d->rad + d->nion);
     desc.coef9 = coef9.data();
     desc.rad = rad.data();
     ion_counts = counts;
-    this->box_volume = box_volume;
+    this->box_origin = box_origin;
     this->atoms_on = atoms_on;
     this->boxes_on = boxes_on;
     ntiles = (int)tiles.size();
@@ -433,6 +461,7 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
     b64.upload(at64.data(), at64.size() * sizeof(double4), s);
     bt.upload(tiles.data(), std::max<size_t>(1, tiles.size()) * sizeof(int4), s);
     bd.upload(bdims.data(), bdims.size() * sizeof(int4), s);
+    bc.upload(bcon.data(), bcon.size() * sizeof(double), s);
     check_hip(hipStreamSynchronize(s), "structure upload");
 }
 
@@ -472,11 +501,11 @@ BuildJob Structure::prepare(Grid &g, int slices, Pending &P, hipStream_t s) cons
     if (g.fp64()) throw Error(kErrInvalidArgs, "the paired builder serves mixed-precision grids");
     std::vector<double> ff;
     form_factor_table(&desc, ion_counts, g.N, g.step, ff);
-    // origin: sum of f~(0) over atoms + contrast * total box volume
+    // origin: sum of f~(0) over atoms + each box set's contrast * total volume
     P.origin = 0.0;
     if (atoms_on)
         for (int i = 0; i < desc.nion; i++) P.origin += ion_counts[i] * ff[i];
-    if (boxes_on) P.origin += desc.box_contrast * box_volume;
+    if (boxes_on) P.origin += box_origin;
     // unreduced hardware sin/cos while every phase stays below 128 revolutions
     const double qtop = (double)(g.N + 1) * g.step;
     P.reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
@@ -489,11 +518,11 @@ BuildJob Structure::prepare(Grid &g, int slices, Pending &P, hipStream_t s) cons
     J.tiles = (const int4 *)bt.p;
     J.ff = (const double *)P.ff.p;
     J.bdims = (const int4 *)bd.p;
+    J.bcon = (const double *)bc.p;
     J.partial = (double *)P.part.p;
     J.grid = (float4 *)g.dev.p;
     J.nunits = nunits;
     J.vox = desc.vox_step;
-    J.contrast = desc.box_contrast;
     J.step = g.step;
     J.scale = desc.scale;
     J.nion = desc.nion;
@@ -543,7 +572,7 @@ void Structure::build(Grid &g, bool host_copy, hipStream_t s, bool use_table) {
     double origin = 0.0;
     if (atoms_on)
         for (int i = 0; i < desc.nion; i++) origin += ion_counts[i] * ff[i];
-    if (boxes_on) origin += desc.box_contrast * box_volume;
+    if (boxes_on) origin += box_origin;
     DevBuf bff;
     bff.upload(ff.data(), ff.size() * sizeof(double), s);
     const long long nhalf = half_base(g.N + 1);
@@ -552,7 +581,7 @@ void Structure::build(Grid &g, bool host_copy, hipStream_t s, bool use_table) {
         const double qtop = (double)(g.N + 1) * g.step;
         const bool reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
         launch_build_atoms(g.fp64(), phase64, reduce, b32.p, b64.p, (const int4 *)bt.p, ntiles,
-                           (const double *)bff.p, desc.nion, (const int4 *)bd.p, desc.vox_step, desc.box_contrast,
+                           (const double *)bff.p, desc.nion, (const int4 *)bd.p, (const double *)bc.p, desc.vox_step,
                            g.step, nhalf, g.dev.p, desc.scale, s);
         check_hip(hipGetLastError(), "build_atoms launch");
     }

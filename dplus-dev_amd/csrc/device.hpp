@@ -207,10 +207,11 @@ struct BuildJob {
     const int4 *tiles;
     const double *ff;
     const int4 *bdims;
+    const double *bcon;  // contrast of each box shape
     double *partial;
     float4 *grid;
     long long nunits;
-    double vox, contrast, step, scale;
+    double vox, step, scale;
     int nion, nslices;
     int xblocks;    // workgroups per slice (set by the launcher)
     int wg_begin;   // first builder workgroup of the job (set by the launcher)
@@ -232,13 +233,13 @@ struct Structure {
     dplus_pdb_desc desc{};
     std::vector<double> coef9, rad;
     std::vector<int> ion_counts;
-    double box_volume = 0.0;
+    double box_origin = 0.0;  // sum over box sets of contrast * total volume (the boxes' A(0))
     double rmax = 0.0;  // largest |position| of an atom or box centre (nm)
     bool atoms_on = false, boxes_on = false;
     int ntiles = 0;
     std::vector<int> tile_len;  // terms of each tile
     long long nsources = 0;
-    DevBuf b32, b64, bt, bd;  // immutable after construction
+    DevBuf b32, b64, bt, bd, bc;  // immutable after construction
     Structure(const dplus_pdb_desc *d, hipStream_t s);
     // use_table: the launch shape from the autotuner's table where it has the grid's
     // class (shapes timed on a device to themselves), else the occupancy model's

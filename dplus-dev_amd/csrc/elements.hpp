@@ -3,10 +3,12 @@
 //
 // X-ray form factors: Cromer-Mann 4 Gaussians + c (International Tables Vol IV
 // Table 2.2B) for atoms and the fitted group coefficients, as tabulated in
-// PDBReaderLib/PDBReaderLib.cpp:2270-2489; dummy-atom radii (nm) of radius type 4
-// (PDBReaderLib.cpp:204-264); atomic weights for centring on the centre of mass
-// (PDBReaderLib.cpp:77-264; a group weighs what its heavy atom weighs, since the
-// reference adds atmWt[0] = 0 per hydrogen).
+// PDBReaderLib/PDBReaderLib.cpp:2270-2489; dummy-atom radii (nm) of radius types
+// 4 and 5 (PDBReaderLib.cpp:204-264); van der Waals, empirical and calculated
+// radii (nm) of radius types 1, 2 and 3 (PDBReaderLib.cpp:77-111; a group takes
+// its heavy atom's radius, since the reference adds the radius of entry 0, which
+// is zero, per hydrogen, :213-267); atomic weights for centring on the centre of
+// mass (PDBReaderLib.cpp:77-264; likewise a group weighs what its heavy atom weighs).
 #pragma once
 
 #include <cstring>
@@ -21,33 +23,34 @@ struct ElementInfo {
     double coef9[9];    // a1 b1 a2 b2 a3 b3 a4 b4 c
     double svg_radius;
     double weight;
+    double vdw_radius, emp_radius, calc_radius;
 };
 
 static const ElementInfo kElements[] = {
-    {" h  ", "H", {0.49300, 10.51090, 0.32290, 26.1257, 0.14020, 3.14240, 0.04080, 57.79980, 0.0030}, 0.107, 1.008},
-    {" c  ", "C", {2.31000, 20.8439, 1.02000, 10.2075, 1.58860, 0.56870, 0.86500, 51.65120, 0.2156}, 0.1577, 12.011},
-    {" n  ", "N", {12.2126, 0.00570, 3.13220, 9.89330, 2.01250, 28.9975, 1.16630, 0.58260, -11.5290}, 0.08414, 14.007},
-    {" o  ", "O", {3.04850, 13.2771, 2.28680, 5.70110, 1.54630, 0.32390, 0.86700, 32.90890, 0.2508}, 0.130, 15.999},
-    {" p  ", "P", {6.43450, 1.90670, 4.17910, 27.1570, 1.78000, 0.52600, 1.49080, 68.16450, 1.1149}, 0.111, 30.97376},
-    {" s  ", "S", {6.90530, 1.46790, 5.20340, 22.2151, 1.43790, 0.25360, 1.58630, 56.1720, 0.86690}, 0.168, 32.06},
-    {"na  ", "Na", {4.76260, 3.28500, 3.17360, 8.84220, 1.26740, 0.31360, 1.11280, 129.42400, 0.676}, 0.180, 22.98976},
-    {"mg  ", "Mg", {5.42040, 2.82750, 2.17350, 79.2611, 1.22690, 0.38080, 2.30730, 7.19370, 0.8584}, 0.160, 24.3050},
-    {"cl  ", "Cl", {11.46040, 0.01040, 7.19640, 1.16620, 6.25560, 18.5194, 1.64550, 47.77840, -9.5574}, 0.100, 35.45},
-    {" k  ", "K", {8.21860, 12.79490, 7.43980, 0.77480, 1.05190, 213.187, 0.86590, 41.68410, 1.4228}, 0.220, 39.0983},
-    {"ca  ", "Ca", {8.62660, 10.44210, 7.38730, 0.65990, 1.58990, 85.7484, 1.02110, 178.43700, 1.3751}, 0.197, 40.078},
-    {"mn  ", "Mn", {11.28190, 5.34090, 7.35730, 0.34320, 3.01930, 17.86740, 2.24410, 83.75430, 1.0896}, 0.130, 54.93804},
-    {"fe  ", "Fe", {11.76950, 4.76110, 7.35730, 0.30720, 3.52220, 15.35350, 2.30450, 76.88050, 1.0369}, 0.124, 55.845},
-    {"cu  ", "Cu", {13.33800, 3.58280, 7.16760, 0.24700, 5.61580, 11.39660, 1.67350, 64.81260, 1.191}, 0.128, 63.546},
-    {"zn  ", "Zn", {14.07430, 3.26550, 7.03180, 0.23330, 5.16520, 10.31630, 2.41000, 58.70970, 1.3041}, 0.133, 65.38},
+    {" h  ", "H", {0.49300, 10.51090, 0.32290, 26.1257, 0.14020, 3.14240, 0.04080, 57.79980, 0.0030}, 0.107, 1.008, 0.12, 0.025, 0.053},
+    {" c  ", "C", {2.31000, 20.8439, 1.02000, 10.2075, 1.58860, 0.56870, 0.86500, 51.65120, 0.2156}, 0.1577, 12.011, 0.17, 0.07, 0.067},
+    {" n  ", "N", {12.2126, 0.00570, 3.13220, 9.89330, 2.01250, 28.9975, 1.16630, 0.58260, -11.5290}, 0.08414, 14.007, 0.155, 0.065, 0.056},
+    {" o  ", "O", {3.04850, 13.2771, 2.28680, 5.70110, 1.54630, 0.32390, 0.86700, 32.90890, 0.2508}, 0.130, 15.999, 0.152, 0.06, 0.048},
+    {" p  ", "P", {6.43450, 1.90670, 4.17910, 27.1570, 1.78000, 0.52600, 1.49080, 68.16450, 1.1149}, 0.111, 30.97376, 0.18, 0.1, 0.098},
+    {" s  ", "S", {6.90530, 1.46790, 5.20340, 22.2151, 1.43790, 0.25360, 1.58630, 56.1720, 0.86690}, 0.168, 32.06, 0.18, 0.1, 0.088},
+    {"na  ", "Na", {4.76260, 3.28500, 3.17360, 8.84220, 1.26740, 0.31360, 1.11280, 129.42400, 0.676}, 0.180, 22.98976, 0.227, 0.18, 0.19},
+    {"mg  ", "Mg", {5.42040, 2.82750, 2.17350, 79.2611, 1.22690, 0.38080, 2.30730, 7.19370, 0.8584}, 0.160, 24.3050, 0.173, 0.15, 0.145},
+    {"cl  ", "Cl", {11.46040, 0.01040, 7.19640, 1.16620, 6.25560, 18.5194, 1.64550, 47.77840, -9.5574}, 0.100, 35.45, 0.175, 0.1, 0.079},
+    {" k  ", "K", {8.21860, 12.79490, 7.43980, 0.77480, 1.05190, 213.187, 0.86590, 41.68410, 1.4228}, 0.220, 39.0983, 0.275, 0.22, 0.243},
+    {"ca  ", "Ca", {8.62660, 10.44210, 7.38730, 0.65990, 1.58990, 85.7484, 1.02110, 178.43700, 1.3751}, 0.197, 40.078, 0.231, 0.18, 0.194},
+    {"mn  ", "Mn", {11.28190, 5.34090, 7.35730, 0.34320, 3.01930, 17.86740, 2.24410, 83.75430, 1.0896}, 0.130, 54.93804, 0.186, 0.14, 0.161},
+    {"fe  ", "Fe", {11.76950, 4.76110, 7.35730, 0.30720, 3.52220, 15.35350, 2.30450, 76.88050, 1.0369}, 0.124, 55.845, 0.181, 0.14, 0.156},
+    {"cu  ", "Cu", {13.33800, 3.58280, 7.16760, 0.24700, 5.61580, 11.39660, 1.67350, 64.81260, 1.191}, 0.128, 63.546, 0.14, 0.135, 0.145},
+    {"zn  ", "Zn", {14.07430, 3.26550, 7.03180, 0.23330, 5.16520, 10.31630, 2.41000, 58.70970, 1.3041}, 0.133, 65.38, 0.139, 0.135, 0.142},
     // implicit-hydrogen groups
-    {"ch", "CH", {0.894937, 55.7145, 0.894429, 4.03158, 3.78824, 24.8323, 3.14683e-6, 956.628, 1.42149}, 0.173, 12.011},
-    {"ch2", "CH2", {1.61908, 52.1451, 2.27205, 24.6589, 2.1815, 24.6587, 0.0019254, 152.165, 1.92445}, 0.185, 12.011},
-    {"ch3", "CH3", {12.5735, 38.7341, -0.456658, -6.28167, 5.71547, 54.955, -11.711, 47.898, 2.87762}, 0.197, 12.011},
-    {"nh", "NH", {0.00506991, 108.256, 2.03147, 14.6199, 1.82122, 14.628, 2.06506, 35.4102, 2.07168}, 0.122, 14.007},
-    {"nh2", "NH2", {3.00872, 28.3717, 0.288137, 63.9637, 3.39248, 3.51866, 2.03511, 28.3675, 0.269952}, 0.145, 14.007},
-    {"nh3", "NH3", {0.294613, 67.4408, 6.48379, 29.1576, 5.67182, 0.54735, 6.57164, 0.547493, -9.02757}, 0.162, 14.007},
-    {"oh", "OH", {-2.73406, 22.1288, 0.00966263, 94.3428, 6.64439, 13.9044, 2.67949, 32.7607, 2.39981}, 0.150, 15.999},
-    {"sh", "SH", {-127.811, 7.19935, 62.5514, 12.1591, 160.747, 1.88979, 2.34822, 55.952, -80.836}, 0.181, 32.06},
+    {"ch", "CH", {0.894937, 55.7145, 0.894429, 4.03158, 3.78824, 24.8323, 3.14683e-6, 956.628, 1.42149}, 0.173, 12.011, 0.17, 0.07, 0.067},
+    {"ch2", "CH2", {1.61908, 52.1451, 2.27205, 24.6589, 2.1815, 24.6587, 0.0019254, 152.165, 1.92445}, 0.185, 12.011, 0.17, 0.07, 0.067},
+    {"ch3", "CH3", {12.5735, 38.7341, -0.456658, -6.28167, 5.71547, 54.955, -11.711, 47.898, 2.87762}, 0.197, 12.011, 0.17, 0.07, 0.067},
+    {"nh", "NH", {0.00506991, 108.256, 2.03147, 14.6199, 1.82122, 14.628, 2.06506, 35.4102, 2.07168}, 0.122, 14.007, 0.155, 0.065, 0.056},
+    {"nh2", "NH2", {3.00872, 28.3717, 0.288137, 63.9637, 3.39248, 3.51866, 2.03511, 28.3675, 0.269952}, 0.145, 14.007, 0.155, 0.065, 0.056},
+    {"nh3", "NH3", {0.294613, 67.4408, 6.48379, 29.1576, 5.67182, 0.54735, 6.57164, 0.547493, -9.02757}, 0.162, 14.007, 0.155, 0.065, 0.056},
+    {"oh", "OH", {-2.73406, 22.1288, 0.00966263, 94.3428, 6.64439, 13.9044, 2.67949, 32.7607, 2.39981}, 0.150, 15.999, 0.152, 0.06, 0.048},
+    {"sh", "SH", {-127.811, 7.19935, 62.5514, 12.1591, 160.747, 1.88979, 2.34822, 55.952, -80.836}, 0.181, 32.06, 0.18, 0.1, 0.088},
 };
 constexpr int kNumElements = sizeof(kElements) / sizeof(kElements[0]);
 

@@ -93,8 +93,8 @@ struct Request {
     long long id = 0;
     bool has_pdb = false;
     uint64_t key[2] = {0, 0};  // content hash of the PDB description (structure cache)
-    std::vector<double> xyz, coef9, rad, box_com;
-    std::vector<int> ion, box_dims;
+    std::vector<double> xyz, coef9, rad, box_com, excl_com;
+    std::vector<int> ion, box_dims, excl_dims;
     dplus_pdb_desc desc{};
     Grid *grid = nullptr;
     int grid_size = 0, precision = DPLUS_PRECISION_MIXED;
@@ -136,7 +136,7 @@ size_t request_bytes(const Request &r) {
             const long long units = pair_units(half_base(gs / 2 + 4));
             b += (size_t)kMaxSlices * 12 * units * sizeof(double);
         }
-        b += (size_t)(r.desc.natoms + r.desc.nbox) * 48;
+        b += (size_t)(r.desc.natoms + r.desc.nbox + r.desc.nexcl) * 48;
         step = r.qmax / (double)(gs / 2);
     } else {
         fp64 = r.grid->fp64();
@@ -424,8 +424,8 @@ struct Engine {
     static void structure_key(Request &r) {
         const dplus_pdb_desc &d = r.desc;
         uint64_t key[2] = {0x13198A2E03707344ULL, 0x243F6A8885A308D3ULL};
-        const double sc[6] = {d.sol_ed, d.c1, d.scale, d.vox_step, d.box_contrast, (double)d.flags};
-        const long long n[3] = {d.natoms, d.nion, d.nbox};
+        const double sc[7] = {d.sol_ed, d.c1, d.scale, d.vox_step, d.box_contrast, (double)d.flags, d.excl_contrast};
+        const long long n[4] = {d.natoms, d.nion, d.nbox, d.nexcl};
         hash_bytes(key, n, sizeof n);
         hash_bytes(key, sc, sizeof sc);
         hash_bytes(key, r.xyz.data(), r.xyz.size() * sizeof(double));
@@ -434,6 +434,8 @@ struct Engine {
         hash_bytes(key, r.rad.data(), r.rad.size() * sizeof(double));
         hash_bytes(key, r.box_dims.data(), r.box_dims.size() * sizeof(int));
         hash_bytes(key, r.box_com.data(), r.box_com.size() * sizeof(double));
+        hash_bytes(key, r.excl_dims.data(), r.excl_dims.size() * sizeof(int));
+        hash_bytes(key, r.excl_com.data(), r.excl_com.size() * sizeof(double));
         r.key[0] = key[0];
         r.key[1] = key[1];
     }
@@ -454,7 +456,7 @@ struct Engine {
                 }
         }
         std::shared_ptr<Structure> S(new Structure(&r.desc, s));
-        const size_t bytes = (size_t)(r.desc.natoms + r.desc.nbox) * 48 + (size_t)(S->ntiles + 1) * 16 + 4096;
+        const size_t bytes = (size_t)(r.desc.natoms + r.desc.nbox + r.desc.nexcl) * 48 + (size_t)(S->ntiles + 1) * 16 + 4096;
         std::lock_guard<std::mutex> lk(cache_mu);
         cache_misses++;
         for (auto &c : cache)
@@ -661,7 +663,8 @@ int dplus_engine_submit(dplus_engine *eh, const dplus_request *rq, long long *ti
         std::unique_ptr<Request> r(new Request());
         if (rq->pdb) {
             const dplus_pdb_desc &d = *rq->pdb;
-            if (d.natoms < 0 || d.nion <= 0 || d.nbox < 0) throw Error(kErrInvalidArgs, "bad PDB description");
+            if (d.natoms < 0 || d.nion <= 0 || d.nbox < 0 || d.nexcl < 0)
+                throw Error(kErrInvalidArgs, "bad PDB description");
             r->has_pdb = true;
             r->xyz.assign(d.xyz, d.xyz + 3 * (size_t)d.natoms);
             r->ion.assign(d.ion, d.ion + d.natoms);
@@ -671,6 +674,10 @@ int dplus_engine_submit(dplus_engine *eh, const dplus_request *rq, long long *ti
                 r->box_dims.assign(d.box_dims, d.box_dims + 3 * (size_t)d.nbox);
                 r->box_com.assign(d.box_com, d.box_com + 3 * (size_t)d.nbox);
             }
+            if (d.nexcl > 0) {
+                r->excl_dims.assign(d.excl_dims, d.excl_dims + 3 * (size_t)d.nexcl);
+                r->excl_com.assign(d.excl_com, d.excl_com + 3 * (size_t)d.nexcl);
+            }
             r->desc = d;
             r->desc.xyz = r->xyz.data();
             r->desc.ion = r->ion.data();
@@ -678,6 +685,8 @@ int dplus_engine_submit(dplus_engine *eh, const dplus_request *rq, long long *ti
             r->desc.rad = r->rad.data();
             r->desc.box_dims = d.nbox > 0 ? r->box_dims.data() : nullptr;
             r->desc.box_com = d.nbox > 0 ? r->box_com.data() : nullptr;
+            r->desc.excl_dims = d.nexcl > 0 ? r->excl_dims.data() : nullptr;
+            r->desc.excl_com = d.nexcl > 0 ? r->excl_com.data() : nullptr;
             // the structure cache's key is hashed by the worker that runs the request
             r->grid_size = rq->grid_size;
             r->qmax = rq->qmax;

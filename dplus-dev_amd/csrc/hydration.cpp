@@ -20,7 +20,10 @@
 //   corner    the reference's slice marking of y = 1..1/voxel, z = 1..4 for
 //             every x (:104-115) is kept, so a layer voxel there is dropped
 //             exactly as the reference drops it;
-//   boxes     LAYER voxels reduced to boxes.  Voxel (h,k,l) is the cube of
+//   boxes     LAYER voxels (the hydration layer) and ATOM voxels (the excluded
+//             volume of radius types 1, 2, 3 and 5: atoms, filled cavities and
+//             the probe shell the layer leaves) reduced to boxes, each set on
+//             request.  Voxel (h,k,l) is the cube of
 //             side `vox` centred at (xMin + h vox, yMin + k vox, zMin + l vox),
 //             the reference's box-centre convention
 //             (ReduceSolventSpaceToIrregularBoxes, :263-362).  The partition is
@@ -115,11 +118,52 @@ void mark_layer(Space &S, double radius, double vox, char type, char neighbor, c
 
 }  // namespace
 
-void hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness, double vox,
-                     bool fill_holes, std::vector<int> &dims, std::vector<double> &com) {
-    dims.clear();
-    com.clear();
-    if (n <= 0 || probe <= 0.0) return;  // the layer is marked only with a probe radius (:56, :75)
+// greedy 3-D meshing of the voxels of one colour into boxes (grow z, then y,
+// then x); the meshed voxels are cleared
+static void mesh_colour(Space &S, char colour, const Frame &F, std::vector<int> &dims, std::vector<double> &com) {
+    auto todo = [&](long long i, long long j, long long k) { return *S.at(i, j, k) == colour; };
+    for (long long i = 0; i < S.X; i++)
+        for (long long j = 0; j < S.Y; j++)
+            for (long long k = 0; k < S.Z; k++) {
+                if (!todo(i, j, k)) continue;
+                long long nk = 1;
+                while (k + nk < S.Z && todo(i, j, k + nk)) nk++;
+                long long nj = 1;
+                while (j + nj < S.Y) {
+                    bool ok = true;
+                    for (long long c = 0; c < nk && ok; c++) ok = todo(i, j + nj, k + c);
+                    if (!ok) break;
+                    nj++;
+                }
+                long long ni = 1;
+                while (i + ni < S.X) {
+                    bool ok = true;
+                    for (long long b2 = 0; b2 < nj && ok; b2++)
+                        for (long long c = 0; c < nk && ok; c++) ok = todo(i + ni, j + b2, k + c);
+                    if (!ok) break;
+                    ni++;
+                }
+                for (long long a2 = 0; a2 < ni; a2++)
+                    for (long long b2 = 0; b2 < nj; b2++)
+                        for (long long c = 0; c < nk; c++) *S.at(i + a2, j + b2, k + c) = kEmpty;
+                dims.push_back((int)ni);
+                dims.push_back((int)nj);
+                dims.push_back((int)nk);
+                com.push_back(F.xmin + ((double)i + 0.5 * (double)(ni - 1)) * F.vox);
+                com.push_back(F.ymin + ((double)j + 0.5 * (double)(nj - 1)) * F.vox);
+                com.push_back(F.zmin + ((double)k + 0.5 * (double)(nk - 1)) * F.vox);
+            }
+}
+
+void solvent_boxes(int n, const double *xyz, const double *rad, double probe, double thickness, double vox,
+                   bool fill_holes, bool want_layer, bool want_excl, std::vector<int> &ldims, std::vector<double> &lcom,
+                   std::vector<int> &edims, std::vector<double> &ecom) {
+    ldims.clear();
+    lcom.clear();
+    edims.clear();
+    ecom.clear();
+    want_layer = want_layer && probe > 0.0;  // the layer is marked only with a probe radius (:56, :75)
+    if (n <= 0 || (!want_layer && !want_excl)) return;
     // the reference keeps PDB coordinates and radii in float
     std::vector<float> x(n), y(n), z(n), r(n);
     for (int a = 0; a < n; a++) {
@@ -151,12 +195,12 @@ void hydration_boxes(int n, const double *xyz, const double *rad, double probe, 
     S.X = xs;
     S.Y = ys;
     S.Z = zs + (15 - (zs + 15) % 16);
-    if ((double)S.X * S.Y * S.Z > 4e9) throw Error(kErrInvalidArgs, "hydration voxel grid too large; use a coarser voxel step");
+    if ((double)S.X * S.Y * S.Z > 4e9) throw Error(kErrInvalidArgs, "solvent voxel grid too large; use a coarser voxel step");
     S.v.assign((size_t)(S.X * S.Y * S.Z), kEmpty);
     const Frame F{xmin, ymin, zmin, vox};
 
     mark_atoms(S, F, n, x.data(), y.data(), z.data(), r.data(), 0.0, kEmpty, kAtom);
-    mark_atoms(S, F, n, x.data(), y.data(), z.data(), r.data(), probe, kEmpty, kProbe);
+    if (probe > 0.0) mark_atoms(S, F, n, x.data(), y.data(), z.data(), r.data(), probe, kEmpty, kProbe);
     if (fill_holes) {
         std::deque<long long> q;
         if (*S.at(0, 0, 0) == kEmpty) {
@@ -182,66 +226,57 @@ void hydration_boxes(int n, const double *xyz, const double *rad, double probe, 
         for (auto &c : S.v)
             if (c == kEmpty) c = kBulk;
     }
-    mark_layer(S, probe, vox, kBulk, kProbe, kLayer, kProbe);
-    if (probe < thickness)
-        mark_layer(S, std::fabs(thickness - probe), vox, kLayer, kBulk, kLayer, kBulk);
-    else
-        mark_layer(S, std::fabs(thickness - probe), vox, kLayer, kBulk, kBulk, kLayer);
-    for (auto &c : S.v)
-        if (c == kProbe) c = kAtom;
-    const int per_nm = (int)(1.0 / vox);
-    for (long long i = 0; i < S.X; i++)
-        for (int t = 0; t < per_nm && 1 + t < S.Y; t++)
-            for (int zk = 1; zk <= 4 && zk < S.Z; zk++) *S.at(i, 1 + t, zk) = kCorner;
-
-    // greedy 3-D meshing of the layer voxels
-    struct Box {
-        long long i0, ni, j0, nj, k0, nk;
-    };
-    std::vector<Box> boxes;
-    auto todo = [&](long long i, long long j, long long k) { return *S.at(i, j, k) == kLayer; };
-    for (long long i = 0; i < S.X; i++)
-        for (long long j = 0; j < S.Y; j++)
-            for (long long k = 0; k < S.Z; k++) {
-                if (!todo(i, j, k)) continue;
-                long long nk = 1;
-                while (k + nk < S.Z && todo(i, j, k + nk)) nk++;
-                long long nj = 1;
-                while (j + nj < S.Y) {
-                    bool ok = true;
-                    for (long long c = 0; c < nk && ok; c++) ok = todo(i, j + nj, k + c);
-                    if (!ok) break;
-                    nj++;
-                }
-                long long ni = 1;
-                while (i + ni < S.X) {
-                    bool ok = true;
-                    for (long long b2 = 0; b2 < nj && ok; b2++)
-                        for (long long c = 0; c < nk && ok; c++) ok = todo(i + ni, j + b2, k + c);
-                    if (!ok) break;
-                    ni++;
-                }
-                for (long long a2 = 0; a2 < ni; a2++)
-                    for (long long b2 = 0; b2 < nj; b2++)
-                        for (long long c = 0; c < nk; c++) *S.at(i + a2, j + b2, k + c) = kEmpty;
-                boxes.push_back({i, ni, j, nj, k, nk});
-            }
-    dims.resize(3 * boxes.size());
-    com.resize(3 * boxes.size());
-    for (size_t b = 0; b < boxes.size(); b++) {
-        const Box &B = boxes[b];
-        dims[3 * b] = (int)B.ni;
-        dims[3 * b + 1] = (int)B.nj;
-        dims[3 * b + 2] = (int)B.nk;
-        com[3 * b] = xmin + ((double)B.i0 + 0.5 * (double)(B.ni - 1)) * vox;
-        com[3 * b + 1] = ymin + ((double)B.j0 + 0.5 * (double)(B.nj - 1)) * vox;
-        com[3 * b + 2] = zmin + ((double)B.k0 + 0.5 * (double)(B.nk - 1)) * vox;
+    if (probe > 0.0) {
+        mark_layer(S, probe, vox, kBulk, kProbe, kLayer, kProbe);
+        if (probe < thickness)
+            mark_layer(S, std::fabs(thickness - probe), vox, kLayer, kBulk, kLayer, kBulk);
+        else
+            mark_layer(S, std::fabs(thickness - probe), vox, kLayer, kBulk, kBulk, kLayer);
+        // leftover probe voxels are excluded volume (:100-101)
+        for (auto &c : S.v)
+            if (c == kProbe) c = kAtom;
+        const int per_nm = (int)(1.0 / vox);
+        for (long long i = 0; i < S.X; i++)
+            for (int t = 0; t < per_nm && 1 + t < S.Y; t++)
+                for (int zk = 1; zk <= 4 && zk < S.Z; zk++) *S.at(i, 1 + t, zk) = kCorner;
     }
+    if (want_excl) mesh_colour(S, kAtom, F, edims, ecom);
+    if (want_layer) mesh_colour(S, kLayer, F, ldims, lcom);
+}
+
+void hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness, double vox,
+                     bool fill_holes, std::vector<int> &dims, std::vector<double> &com) {
+    std::vector<int> ed;
+    std::vector<double> ec;
+    solvent_boxes(n, xyz, rad, probe, thickness, vox, fill_holes, true, false, dims, com, ed, ec);
 }
 
 }  // namespace dplus
 
 using namespace dplus;
+
+extern "C" int dplus_excluded_volume_boxes(int n, const double *xyz, const double *rad, double probe,
+                                           double thickness, double vox, int fill_holes, int capacity, int *nbox,
+                                           int *dims, double *com) {
+    try {
+        if (n < 0 || (n > 0 && (!xyz || !rad)) || !nbox || vox <= 0 || probe < 0 || thickness < 0)
+            throw Error(kErrInvalidArgs, "bad excluded-volume arguments");
+        std::vector<int> ld, d;
+        std::vector<double> lc, c;
+        solvent_boxes(n, xyz, rad, probe, thickness, vox, fill_holes != 0, false, true, ld, lc, d, c);
+        *nbox = (int)(d.size() / 3);
+        if (*nbox > capacity) return kOk;  // caller retries with capacity >= *nbox
+        if (*nbox) {
+            std::copy(d.begin(), d.end(), dims);
+            std::copy(c.begin(), c.end(), com);
+        }
+        return kOk;
+    } catch (const Error &e) {
+        return set_error(e.code, e.what());
+    } catch (const std::exception &e) {
+        return set_error(kErrGeneral, e.what());
+    }
+}
 
 extern "C" int dplus_hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness,
                                      double vox, int fill_holes, int capacity, int *nbox, int *dims, double *com) {

@@ -37,7 +37,8 @@ class PdbDesc(ctypes.Structure):
         ("natoms", ctypes.c_int), ("xyz", _D), ("ion", _I), ("nion", ctypes.c_int), ("coef9", _D),
         ("rad", _D), ("sol_ed", ctypes.c_double), ("c1", ctypes.c_double), ("flags", ctypes.c_int),
         ("scale", ctypes.c_double), ("nbox", ctypes.c_int), ("box_dims", _I), ("box_com", _D),
-        ("vox_step", ctypes.c_double), ("box_contrast", ctypes.c_double),
+        ("vox_step", ctypes.c_double), ("box_contrast", ctypes.c_double), ("nexcl", ctypes.c_int),
+        ("excl_dims", _I), ("excl_com", _D), ("excl_contrast", ctypes.c_double),
     ]
 
 
@@ -114,6 +115,8 @@ SIGNATURES = {
                                         ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
     "dplus_hydration_boxes": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, _I, _I, _D]),
+    "dplus_excluded_volume_boxes": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.c_int, ctypes.c_int, _I, _I, _D]),
     "dplus_structure_create": (ctypes.c_int, [ctypes.POINTER(PdbDesc), ctypes.POINTER(ctypes.c_void_p)]),
     "dplus_structure_destroy": (None, [ctypes.c_void_p]),
     "dplus_structure_build_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]),

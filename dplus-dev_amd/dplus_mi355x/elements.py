@@ -4,8 +4,9 @@
   Tables Vol IV Table 2.2B, as tabulated in PDBReaderLib/PDBReaderLib.cpp:2270-2489
   (row order a1, b1, a2, b2, a3, b3, a4, b4, c; q in nm^-1 with
   s^2 = q^2 / (100 * 16 pi^2), Backend/GPU/Atomic Form Factor.cpp:151).
-* Dummy-atom ("svg") radii in nm used with radius type 4
-  (PDBReaderLib.cpp:204-264, selected by SetRadiusType :1216-1219).
+* Dummy-atom ("svg") radii in nm used with radius types 4 and 5
+  (PDBReaderLib.cpp:204-264, selected by SetRadiusType :1216-1219), and the van der
+  Waals, empirical and calculated radii of types 1, 2 and 3.
 * Atomic weights for centring on the centre of mass (PDBReaderLib.cpp:77-264, 271-292).
 """
 import numpy as np
@@ -41,6 +42,19 @@ SVG_RADIUS = {"H": 0.107, "C": 0.1577, "N": 0.08414, "O": 0.130, "P": 0.111, "S"
               "Mg": 0.160, "Cl": 0.100, "K": 0.220, "Ca": 0.197, "Mn": 0.130, "Fe": 0.124, "Cu": 0.128,
               "Zn": 0.133, "CH": 0.173, "CH2": 0.185, "CH3": 0.197, "NH": 0.122, "NH2": 0.145, "NH3": 0.162,
               "OH": 0.150, "SH": 0.181}
+
+# van der Waals, empirical and calculated radii (nm) of radius types 1, 2 and 3
+# (PDBReaderLib.cpp:77-111, pm / 1000); a group takes its heavy atom's radius (the
+# reference adds entry 0's radius, which is zero, per hydrogen, :213-267)
+_ATOM_RADII = {"H": (0.120, 0.025, 0.053), "C": (0.170, 0.070, 0.067), "N": (0.155, 0.065, 0.056),
+               "O": (0.152, 0.060, 0.048), "P": (0.180, 0.100, 0.098), "S": (0.180, 0.100, 0.088),
+               "Na": (0.227, 0.180, 0.190), "Mg": (0.173, 0.150, 0.145), "Cl": (0.175, 0.100, 0.079),
+               "K": (0.275, 0.220, 0.243), "Ca": (0.231, 0.180, 0.194), "Mn": (0.186, 0.140, 0.161),
+               "Fe": (0.181, 0.140, 0.156), "Cu": (0.140, 0.135, 0.145), "Zn": (0.139, 0.135, 0.142)}
+_HEAVY = {"CH": "C", "CH2": "C", "CH3": "C", "NH": "N", "NH2": "N", "NH3": "N", "OH": "O", "SH": "S"}
+VDW_RADIUS = {e: _ATOM_RADII[_HEAVY.get(e, e)][0] for e in SVG_RADIUS}
+EMP_RADIUS = {e: _ATOM_RADII[_HEAVY.get(e, e)][1] for e in SVG_RADIUS}
+CALC_RADIUS = {e: _ATOM_RADII[_HEAVY.get(e, e)][2] for e in SVG_RADIUS}
 
 # a group weighs what its heavy atom weighs (the reference adds atmWt[0] = 0 per hydrogen)
 WEIGHT = {"H": 1.008, "C": 12.011, "N": 14.007, "O": 15.999, "P": 30.97376, "S": 32.06, "Na": 22.98976,

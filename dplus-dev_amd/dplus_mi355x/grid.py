@@ -77,16 +77,12 @@ class Grid:
         return r.value
 
     def build_pdb(self, xyz, ion, coef9, rad, sol_ed=0.0, c1=1.0, flags=L.CALC_ATOMIC_FORMFACTORS, scale=1.0,
-                  box_dims=None, box_com=None, vox_step=0.0, box_contrast=0.0, host_copy=True):
+                  box_dims=None, box_com=None, vox_step=0.0, box_contrast=0.0, host_copy=True, excl_dims=None,
+                  excl_com=None, excl_contrast=0.0):
         """PDBAmplitude::calculateGrid for atoms (nm) of ion types ion[i]."""
-        xyz, ion = L.f64(xyz).reshape(-1, 3), L.i32(ion)
-        coef9, rad = L.f64(coef9).reshape(-1, 9), L.f64(rad)
-        if box_dims is None:
-            bd, bc, nb = np.zeros(3, np.int32), np.zeros(3), 0
-        else:
-            bd, bc, nb = L.i32(box_dims).reshape(-1, 3), L.f64(box_com).reshape(-1, 3), len(box_dims)
-        d = L.PdbDesc(len(ion), L.dp(xyz), L.ip(ion), len(coef9), L.dp(coef9), L.dp(rad), sol_ed, c1, flags, scale,
-                      nb, L.ip(bd), L.dp(bc), vox_step, box_contrast)
+        keep = []
+        d = _pdb_desc(xyz, ion, coef9, rad, sol_ed, c1, flags, scale, box_dims, box_com, vox_step, box_contrast, keep,
+                      excl_dims, excl_com, excl_contrast)
         L.check(L.lib().dplus_pdb_build_grid(self._h, ctypes.byref(d), 1 if host_copy else 0))
 
     def __del__(self):
@@ -98,25 +94,32 @@ class Grid:
             pass
 
 
-def _pdb_desc(xyz, ion, coef9, rad, sol_ed, c1, flags, scale, box_dims, box_com, vox_step, box_contrast, keep):
+def _boxes(dims, com):
+    if dims is None or len(dims) == 0:
+        return np.zeros((1, 3), np.int32), np.zeros((1, 3)), 0
+    return L.i32(dims).reshape(-1, 3), L.f64(com).reshape(-1, 3), len(dims)
+
+
+def _pdb_desc(xyz, ion, coef9, rad, sol_ed, c1, flags, scale, box_dims, box_com, vox_step, box_contrast, keep,
+              excl_dims=None, excl_com=None, excl_contrast=0.0):
     xyz, ion = L.f64(xyz).reshape(-1, 3), L.i32(ion)
     coef9, rad = L.f64(coef9).reshape(-1, 9), L.f64(rad)
-    if box_dims is None or len(box_dims) == 0:
-        bd, bc, nb = np.zeros((1, 3), np.int32), np.zeros((1, 3)), 0
-    else:
-        bd, bc, nb = L.i32(box_dims).reshape(-1, 3), L.f64(box_com).reshape(-1, 3), len(box_dims)
-    keep += [xyz, ion, coef9, rad, bd, bc]
+    bd, bc, nb = _boxes(box_dims, box_com)
+    ed, ec, ne = _boxes(excl_dims, excl_com)
+    keep += [xyz, ion, coef9, rad, bd, bc, ed, ec]
     return L.PdbDesc(len(ion), L.dp(xyz), L.ip(ion), len(coef9), L.dp(coef9), L.dp(rad), sol_ed, c1, flags, scale,
-                     nb, L.ip(bd), L.dp(bc), vox_step, box_contrast)
+                     nb, L.ip(bd), L.dp(bc), vox_step, box_contrast, ne, L.ip(ed), L.dp(ec), excl_contrast)
 
 
 class Structure:
     """Atoms and solvent boxes of a PDB leaf resident in HBM (sorted and tiled once)."""
 
     def __init__(self, xyz, ion, coef9, rad, sol_ed=0.0, c1=1.0, flags=L.CALC_ATOMIC_FORMFACTORS, scale=1.0,
-                 box_dims=None, box_com=None, vox_step=0.0, box_contrast=0.0):
+                 box_dims=None, box_com=None, vox_step=0.0, box_contrast=0.0, excl_dims=None, excl_com=None,
+                 excl_contrast=0.0):
         keep = []
-        d = _pdb_desc(xyz, ion, coef9, rad, sol_ed, c1, flags, scale, box_dims, box_com, vox_step, box_contrast, keep)
+        d = _pdb_desc(xyz, ion, coef9, rad, sol_ed, c1, flags, scale, box_dims, box_com, vox_step, box_contrast, keep,
+                      excl_dims, excl_com, excl_contrast)
         h = ctypes.c_void_p()
         L.check(L.lib().dplus_structure_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h
@@ -216,18 +219,27 @@ def dock_grid_space_filling(out, dock, distances, angles_deg, reps, scale=1.0):
                                                   L.dp(r)))
 
 
-def hydration_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05, fill_holes=True):
-    """Hydration-layer boxes (dims [n,3] voxels, centres [n,3] nm) around atoms of radius rad,
-    with the reference's voxel marking (cavities count as atoms when fill_holes)."""
+def _voxel_boxes(fn, xyz, rad, probe, thickness, vox, fill_holes):
     xyz, rad = L.f64(xyz).reshape(-1, 3), L.f64(rad)
     n = ctypes.c_int()
     fh = 1 if fill_holes else 0
-    L.check(L.lib().dplus_hydration_boxes(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, fh, 0,
-                                          ctypes.byref(n), None, None))
+    L.check(fn(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, fh, 0, ctypes.byref(n), None, None))
     dims, com = np.zeros((max(1, n.value), 3), np.int32), np.zeros((max(1, n.value), 3))
-    L.check(L.lib().dplus_hydration_boxes(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, fh, n.value,
-                                          ctypes.byref(n), L.ip(dims), L.dp(com)))
+    L.check(fn(len(rad), L.dp(xyz), L.dp(rad), probe, thickness, vox, fh, n.value, ctypes.byref(n), L.ip(dims),
+               L.dp(com)))
     return dims[:n.value], com[:n.value]
+
+
+def hydration_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05, fill_holes=True):
+    """Hydration-layer boxes (dims [n,3] voxels, centres [n,3] nm) around atoms of radius rad,
+    with the reference's voxel marking (cavities count as atoms when fill_holes)."""
+    return _voxel_boxes(L.lib().dplus_hydration_boxes, xyz, rad, probe, thickness, vox, fill_holes)
+
+
+def excluded_volume_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05, fill_holes=True):
+    """Excluded-volume boxes of the same voxel marking (atoms, filled cavities and the
+    probe shell the hydration layer leaves), subtracted with rho_0 by radius types 1, 2, 3, 5."""
+    return _voxel_boxes(L.lib().dplus_excluded_volume_boxes, xyz, rad, probe, thickness, vox, fill_holes)
 
 
 def synth_walk(seed, n, step=0.38, min_sep=0.12, axes=(3.0, 3.0, 3.0)):

@@ -29,7 +29,7 @@ class Engine:
                vegas_points_per_bin=4, gk_max_depth=1 << 40, data=None, sigma=None, fit_scale=1.0,
                fit_constant=0.0):
         """pdb: dict of Structure/build_pdb arguments (xyz, ion, coef9, rad, sol_ed, c1, flags, scale,
-        box_dims, box_com, vox_step, box_contrast); or grid: a resident Grid.  placements:
+        box_dims, box_com, vox_step, box_contrast, excl_dims, excl_com, excl_contrast); or grid: a resident Grid.  placements:
         (rot [n,4], ntrans [n], trans [m,3]) of the leaf."""
         keep = []
         q = L.f64(q)
@@ -37,10 +37,11 @@ class Engine:
         r = L.Request()
         if pdb is not None:
             p = dict(sol_ed=0.0, c1=1.0, flags=L.CALC_ATOMIC_FORMFACTORS, scale=1.0, box_dims=None, box_com=None,
-                     vox_step=0.0, box_contrast=0.0)
+                     vox_step=0.0, box_contrast=0.0, excl_dims=None, excl_com=None, excl_contrast=0.0)
             p.update(pdb)
             desc = _pdb_desc(p["xyz"], p["ion"], p["coef9"], p["rad"], p["sol_ed"], p["c1"], p["flags"], p["scale"],
-                             p["box_dims"], p["box_com"], p["vox_step"], p["box_contrast"], keep)
+                             p["box_dims"], p["box_com"], p["vox_step"], p["box_contrast"], keep, p["excl_dims"],
+                             p["excl_com"], p["excl_contrast"])
             keep.append(desc)
             r.pdb = ctypes.pointer(desc)
             r.grid_size, r.qmax, r.precision = int(grid_size), float(qmax), int(precision)

@@ -84,9 +84,11 @@ int dplus_grid_calculate_intensity(dplus_grid *g, double q, double theta, double
 /* ------------------------------------------------------------------ PDB leaf */
 /* Atoms and solvent description of PDBAmplitude (Backend/Backend/PDBAmplitude.cpp:1199-1336):
  * ion[i] indexes the nion rows of coef9 (a1,b1,a2,b2,a3,b3,a4,b4,c; X-ray table,
- * PDBReaderLib/PDBReaderLib.cpp:2260-2293) and rad (nm).  Boxes are solvent
- * boxes (dims in voxels, centre in nm) added with box_contrast
- * (outer-solvent layer: rho_out - rho_0, PDBAmplitude.cpp:298-314). */
+ * PDBReaderLib/PDBReaderLib.cpp:2260-2293) and rad (nm).  Boxes (dims in voxels
+ * of side vox_step, centre in nm) come in two sets: the outer-solvent layer
+ * added with box_contrast (rho_out - rho_0, PDBAmplitude.cpp:298-314) and the
+ * excluded volume of radius types 1, 2, 3 and 5 added with excl_contrast
+ * (-rho_0: the solvent it displaces, PDBAmplitude.cpp:277-296). */
 typedef struct {
     int natoms;
     const double *xyz; /* 3*natoms, nm */
@@ -103,6 +105,10 @@ typedef struct {
     const double *box_com; /* 3*nbox */
     double vox_step;
     double box_contrast;
+    int nexcl;
+    const int *excl_dims;  /* 3*nexcl */
+    const double *excl_com; /* 3*nexcl */
+    double excl_contrast;
 } dplus_pdb_desc;
 
 /* PDBAmplitude::calculateGrid (PDBAmplitude.cpp:319-435, GPU path
@@ -251,6 +257,15 @@ int dplus_dock_grid_space_filling(dplus_grid *out, int nleaf, const dplus_leaf *
  * found; call again with capacity >= *nbox when it is larger. */
 int dplus_hydration_boxes(int n, const double *xyz, const double *rad, double probe, double thickness,
                           double vox, int fill_holes, int capacity, int *nbox, int *dims, double *com);
+/* The excluded volume of the same voxel colouring (the "atoms" voxels that
+ * ReduceSolventSpaceToIrregularBoxes turns into solventBoxDims/COM,
+ * VoxelizedSolvent.cpp:4-194,263-362): voxels whose corner lies within an atom
+ * radius, cavities when fill_holes != 0, and -- with probe > 0 -- the part of the
+ * probe shell that the hydration layer does not take.  Radius types 1, 2, 3 and
+ * 5 subtract it with the solvent density (PDBAmplitude.cpp:277-296).  Same
+ * capacity protocol as dplus_hydration_boxes. */
+int dplus_excluded_volume_boxes(int n, const double *xyz, const double *rad, double probe, double thickness,
+                                double vox, int fill_holes, int capacity, int *nbox, int *dims, double *com);
 
 /* ------------------------------------------------------------------ amplitude files */
 /* AmpGridAmplitude::ReadAmplitudeFromFile (Backend/Backend/Amplitude.cpp:3724-3796):

@@ -108,6 +108,77 @@ def symmetry_cases():
         print(name, len(q))
 
 
+LONG_REF = "/root/reference/PythonInterface/tests/reviewer_tests/files_for_tests/generate/cpu/long"
+UNIT_REF = "/root/reference/PythonInterface/tests/unit_tests/files_for_tests"
+
+
+def read_curve(path):
+    """(q, I, header lines) of a two-column .out signal file."""
+    q, I, head = [], [], []
+    for line in open(path):
+        if line.startswith("#"):
+            head.append(line.rstrip("\n"))
+            continue
+        parts = line.split()
+        if len(parts) >= 2:
+            try:
+                a, b = float(parts[0]), float(parts[1])
+            except ValueError:
+                continue
+            q.append(a)
+            I.append(b)
+    return q, I, head
+
+
+def voxel_cases():
+    """Voxelised-solvent fixtures under tests/golden/voxel/: the hollow-sphere
+    hydration layer with and without filled holes (radius type 5, solvent only),
+    and 1jff with its excluded volume subtracted at 334 e/nm^3 (radius type 5,
+    probe 0.14 nm, voxel 0.2 nm), each with its state, curve and gzip'ed PDB."""
+    import gzip
+    import hashlib
+    out = os.path.join(OUT, "voxel")
+    os.makedirs(out, exist_ok=True)
+
+    def store_pdb(src, name):
+        raw = open(src, "rb").read()
+        with gzip.GzipFile(os.path.join(out, name), "wb", mtime=0) as f:
+            f.write(raw)
+        return hashlib.sha256(raw).hexdigest()
+
+    for name in ("Hollow Sphere PDB Fill Holes", "Hollow Sphere PDB No Fill Holes"):
+        d = os.path.join(LONG_REF, name)
+        state = json.load(open(os.path.join(d, name + ".state")))
+        digest = store_pdb(os.path.join(d, "Sphere.pdb"), "hollow_sphere.pdb.gz")
+        q, I, s, meta = read_standard(os.path.join(d, name + "TestStandard.out"))
+        rec = {"name": name, "source": f"reviewer_tests/files_for_tests/generate/cpu/long/{name}",
+               "pdb": "hollow_sphere.pdb.gz", "pdb_sha256": digest, "state": state, "q": q, "intensity": I,
+               "sigma": s, "meta": meta}
+        with open(os.path.join(out, name.replace(" ", "_") + ".json"), "w") as f:
+            json.dump(rec, f)
+        print(name, len(q))
+    # 1jff: the fit test's state (test_fit.py:98-115) at the parameters its target
+    # curve was generated with (the curve's header: solvent 334, probe 0.14)
+    state = json.load(open(os.path.join(UNIT_REF, "1jff_ED_350_probe_0.125_voxel.state")))
+    model = state["Domain"]["Populations"][0]["Models"][0]
+    ep = model["ExtraParameters"]
+    ep[1] = 334.0
+    ep[4] = 0.14
+    digest = store_pdb(os.path.join(UNIT_REF, "1jff.pdb"), "1jff.pdb.gz")
+    q, I, head = read_curve(os.path.join(UNIT_REF, "1jff_ED_334_probe_0.14_voxel.out"))
+    rec = {"name": "1jff_ED_334_probe_0.14_voxel", "source": "unit_tests/files_for_tests/1jff_ED_334_probe_0.14_voxel.out",
+           "pdb": "1jff.pdb.gz", "pdb_sha256": digest, "state": state, "q": q, "intensity": I,
+           "header": [h for h in head if h.startswith("#")]}
+    with open(os.path.join(out, "1jff_ED_334_probe_0.14_voxel.json"), "w") as f:
+        json.dump(rec, f)
+    print("1jff", len(q))
+
+
 if __name__ == "__main__":
-    main()
-    symmetry_cases()
+    import sys
+    if "--voxel" in sys.argv:
+        voxel_cases()
+    else:
+        main()
+        symmetry_cases()
+        voxel_cases()
