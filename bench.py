@@ -70,6 +70,12 @@ def sync():
         pass
 
 
+def O_total_points(gs):
+    """Nodes of a grid: 1 + 6 N (N + 1)^2 with N = gs / 2 + 3 (Grid.cpp:184-186)."""
+    n = gs // 2 + 3
+    return 1 + 6 * n * (n + 1) ** 2
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -459,7 +465,21 @@ def main():
     # workgroup, whose units span a few rows (under 2 % of the pair work, not counted)
     flop_eq = 11.0 / 2.0 + w_t
     t_launch = ms_build / max(nb, 1) * 1e-3
-    achieved = terms * flop_eq / t_launch / 1e12
+    model_achieved = terms * flop_eq / t_launch / 1e12
+    # counted: the builder launch's VALU wave instructions from rocprofv3 --pmc of this
+    # workload (profiles/build_atoms_pairs_c2_valu.json), 64 lanes each: FP32 flops
+    # (FMA 2, MUL and ADD 1) plus w_T per transcendental, over the live launch time;
+    # issue_frac = their issue time at the microbenchmarked rates / the launch time
+    counted = None
+    vjs = os.path.join(ROOT, "profiles", "build_atoms_pairs_c2_valu.json")
+    if os.path.exists(vjs):
+        wi = json.load(open(vjs))["wave_instructions"]
+        flops32 = 64.0 * (2.0 * wi["valu_fma_f32"] + wi["valu_mul_f32"] + wi["valu_add_f32"])
+        trans = 64.0 * wi["valu_trans_f32"]
+        issue_s = 64.0 * (wi["valu"] - wi["valu_trans_f32"]) / mb["fma32"] + trans / mb["sin32"]
+        counted = {"flops_fp32": flops32, "transcendentals": trans, "valu_instructions_per_term": 64.0 * wi["valu"] / terms,
+                   "achieved": (flops32 + w_t * trans) / t_launch / 1e12, "issue_frac": issue_s / t_launch}
+    achieved = counted["achieved"] if counted else model_achieved
     # HBM bytes per builder launch from the two rocprofv3 --pmc passes of this workload
     # (FETCH_SIZE, WRITE_SIZE, each scaled by a known-size copy), kept in profiles/
     traffic = None
@@ -510,6 +530,12 @@ def main():
                      "kernel": f"build_atoms_pairs<false, {shape['wg']}>", "launch_ms": t_launch * 1e3,
                      "terms_per_launch": terms, "flop_eq_per_term": flop_eq,
                      "transcendental_weight": w_t,
+                     "achieved_from": "counted VALU instructions (rocprofv3 --pmc)" if counted else "flop model",
+                     "counted": counted, "model_achieved": model_achieved,
+                     # section 8(d)'s unit: 8 flops + 2 transcendentals per (node, source) term over all
+                     # P nodes (the builder does half the nodes by Friedel symmetry and shares one sin/cos
+                     # pair between the phi and phi + pi nodes, so this exceeds the peak)
+                     "survey_unit_achieved": float(O_total_points(W["gs"])) * sources * (8.0 + 2.0 * w_t) / t_launch / 1e12,
                      # the same launch against the HBM roofline: PMC bytes per launch over its time
                      "hbm": None if traffic is None else {
                          "achieved": traffic / t_launch / 1e9, "peak": PEAK_HBM_GBPS, "unit": "GB/s",
