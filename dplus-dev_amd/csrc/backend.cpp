@@ -94,8 +94,9 @@ struct Population {
 
 struct Spec {
     int grid_size = 100, method = DPLUS_OA_MC, precision = DPLUS_PRECISION_MIXED;
-    double qmax = 1.0, eps = 0.0, scale = 1.0, constant = 0.0;
+    double qmax = 1.0, qmin = 0.0, eps = 0.0, scale = 1.0, constant = 0.0;
     long long iters = 1000000;
+    bool two_d = false;  // StartGenerate2D: the (q_z, q_perp) matrix
     bool use_grid = true;
     unsigned long long seed = 1;
     std::vector<Population> pops;
@@ -118,6 +119,7 @@ Spec parse_state(const json::Value &doc) {
     s.qmax = s.q.back();  // BackendWrapper.cpp:335
     s.iters = (long long)dp.at("OrientationIterations").number();
     s.eps = dp.at("Convergence").number();
+    if (const json::Value *v = dp.find("qMin")) s.qmin = v->number(0.0);
     s.use_grid = dp.at("UseGrid").truth(true);
     const std::string method = dp.at("OrientationMethod").text();  // Common/Common.h:396-402
     if (method.find("VEGAS") != std::string::npos)
@@ -310,8 +312,9 @@ struct Backend {
     double progress = 0.0;
     int code = 0;
     std::string message;
-    bool have_results = false;
+    bool have_results = false, have_results_2d = false;
     std::vector<double> graph;
+    std::vector<double> graph2d;  // nq x nq, row = q_z index, column = q_perp index
     std::vector<unsigned> model_ptrs;
     std::vector<std::pair<unsigned, std::string>> headers;
     std::map<unsigned, Grid *> model_grid;  // grids of this job's models (get/save amplitude)
@@ -532,10 +535,63 @@ struct Backend {
         return p;
     }
 
+    // DomainModel::CalculateIntensity2DMatrix through CompositeModel::CalculateMatrix
+    // (Amplitude.cpp:843-993, CompositeModel.cpp:265-317): for every pair (q_z =
+    // x[i], q_perp = x[j]) with q = |(q_z, q_perp)| in [qMin, qMax], the azimuthal
+    // average of |A(q, theta = |atan2(q_perp, q_z)|, phi)|^2 (Grid.cpp:609-657);
+    // the matrix is scale * sum_pop size * I_pop + constant (pairs outside the q
+    // range are 0 before the constant).  Every orientation method takes the
+    // azimuthal average (the reference's CPU Gauss-Kronrod branch ignores theta,
+    // Amplitude.cpp:1484-1486); nphi = OrientationIterations within [64, 16384].
+    std::vector<double> run_2d(const Spec &s, hipStream_t st) {
+        const size_t nq = s.q.size();
+        std::vector<double> q, th;
+        std::vector<size_t> at;
+        for (size_t i = 0; i < nq; i++)
+            for (size_t j = 0; j < nq; j++) {
+                const double qz = s.q[i], qp = s.q[j];
+                const double qq = std::sqrt(qz * qz + qp * qp), t = std::fabs(std::atan2(qp, qz));
+                if (qq > s.qmax || qq < s.qmin) continue;
+                q.push_back(qq);
+                th.push_back(std::min(t, kPi));
+                at.push_back(i * nq + j);
+            }
+        const int nphi = (int)std::min<long long>(16384, std::max<long long>(64, s.iters));
+        std::vector<double> total(nq * nq, 0.0);
+        const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, z[3] = {0, 0, 0};
+        for (size_t pi = 0; pi < s.pops.size(); pi++) {
+            std::vector<Placement> pl;
+            for (auto &m : s.pops[pi].models) collect(m, I3, z, 1.0, s, pl, st);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                progress = (pi + 0.5) / s.pops.size();
+            }
+            if (stop_flag) throw Error(kErrStopped, "stopped");
+            DockSet D(pl, st);
+            std::vector<double> I(q.size());
+            azimuthal_average(D, (int)q.size(), q.data(), th.data(), nphi, s.seed, I.data(), st);
+            timing_collect();
+            for (size_t k = 0; k < at.size(); k++) total[at[k]] += s.pops[pi].size * I[k];
+        }
+        for (auto &v : total) v = s.scale * v + s.constant;
+        return total;
+    }
+
     void run(Spec s) {
         hipStream_t st = Device::get().stream;
         try {
             Device::get().require();
+            if (s.two_d) {
+                std::vector<double> m = run_2d(s, st);
+                std::lock_guard<std::mutex> lk(mu);
+                graph2d.swap(m);
+                have_results_2d = true;
+                code = 0;
+                message = "";
+                progress = 1.0;
+                running = false;
+                return;
+            }
             std::vector<double> total(s.q.size(), 0.0);
             const double I3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, z[3] = {0, 0, 0};
             for (size_t pi = 0; pi < s.pops.size(); pi++) {
@@ -582,12 +638,13 @@ struct Backend {
         running = false;
     }
 
-    void start(const std::string &state) {
+    void start(const std::string &state, bool two_d = false) {
         // PythonBackendWrapper::StartGenerate: ERROR_ILLEGAL_JSON first, then
         // CheckCapabilities (BackendWrapper.cpp:317), then the parameter tree
         const json::Value doc = json::parse(state);
         Device::get().require();
         Spec s = parse_state(doc);
+        s.two_d = two_d;
         // one start at a time: the running check, the join of the finished
         // worker and the assignment of the new one happen under start_mu, so a
         // concurrent start sees running == true and fails with ERROR_JOBRUNNING
@@ -603,6 +660,7 @@ struct Backend {
         code = -1;  // running (JobManager.cpp:164-165)
         message = "";
         have_results = false;
+        have_results_2d = false;
         stop_flag = false;
         model_ptrs = s.ptrs;
         job_spec = s;
@@ -632,6 +690,30 @@ struct Backend {
         }
         o += "],\"Graph\":[";
         for (size_t i = 0; i < graph.size(); i++) o += (i ? "," : "") + json::num(graph[i]);
+        return o + "]}";
+    }
+
+    // GetGenerate2DResults (BackendWrapper.cpp:583-624, Write2DGraph :514-536):
+    // {"Headers": [...], "2DGraph": [[row q_z = x[0]], [row q_z = x[1]], ...]}
+    std::string results_2d() {
+        std::lock_guard<std::mutex> lk(mu);
+        if (running) throw Error(kErrJobRunning, "the job is still running");
+        if (!have_results_2d)
+            throw Error(code > 0 ? code : kErrInvalidArgs, message.empty() ? "no 2-D results" : message);
+        std::string o = "{\"Headers\":[";
+        bool first = true;
+        for (auto &h : headers) {
+            if (!first) o += ",";
+            first = false;
+            o += "{\"ModelPtr\":" + std::to_string(h.first) + ",\"Header\":" + json::quote(h.second) + "}";
+        }
+        o += "],\"2DGraph\":[";
+        const size_t n = (size_t)std::llround(std::sqrt((double)graph2d.size()));
+        for (size_t i = 0; i < n; i++) {
+            o += i ? ",[" : "[";
+            for (size_t j = 0; j < n; j++) o += (j ? "," : "") + json::num(graph2d[i * n + j]);
+            o += "]";
+        }
         return o + "]}";
     }
 
@@ -800,6 +882,21 @@ int dplus_backend_start_generate(dplus_backend *b, const char *state, int use_gp
     BACKEND_GUARD({
         if (!b || !state) throw Error(kErrInvalidArgs, "null argument");
         ((Backend *)b)->start(state);
+    })
+}
+
+int dplus_backend_start_generate_2d(dplus_backend *b, const char *state, int use_gpu) {
+    (void)use_gpu;
+    BACKEND_GUARD({
+        if (!b || !state) throw Error(kErrInvalidArgs, "null argument");
+        ((Backend *)b)->start(state, true);
+    })
+}
+
+int dplus_backend_get_generate_2d_results(dplus_backend *b, char **out) {
+    BACKEND_GUARD({
+        if (!b || !out) throw Error(kErrInvalidArgs, "null argument");
+        *out = dup_string(((Backend *)b)->results_2d());
     })
 }
 

@@ -667,6 +667,33 @@ void launch_vegas_iter(bool fp64, bool single, const void *const *grids, const R
                        double *binvar, double *hist, int nq, hipStream_t s);
 void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
                         const double4 *trans, int n, const double *qxyz, double *out, hipStream_t s);
+void launch_azimuthal(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
+                      const double4 *trans, int npts, const double2 *pts, int nphi, uint64_t seed, double *out,
+                      hipStream_t s);
+
+void azimuthal_average(const DockSet &D, int npts, const double *q, const double *theta, int nphi, uint64_t seed,
+                       double *out, hipStream_t s) {
+    if (npts <= 0) return;
+    if (nphi < 1) throw Error(kErrInvalidArgs, "the azimuthal average needs at least one phi sample");
+    const double qlim = (D.grid_size / 2 + 1) * D.step;
+    std::vector<double2> pts(npts);
+    for (int i = 0; i < npts; i++) {
+        if (!(q[i] >= 0.0 && q[i] <= qlim) || !(theta[i] >= 0.0 && theta[i] <= kPi))
+            throw Error(kErrInvalidArgs, "2-D point outside the grid (q in [0, qmax], theta in [0, pi])");
+        pts[i] = make_double2(q[i] * std::sin(theta[i]), q[i] * std::cos(theta[i]));
+    }
+    DevBuf bp, bo;
+    bp.upload(pts.data(), pts.size() * sizeof(double2), s);
+    bo.ensure((size_t)npts * sizeof(double));
+    {
+        KTimer kt("azimuthal", s);
+        launch_azimuthal(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p, (const RotDev *)D.rotb.p,
+                         D.nrot, (const double4 *)D.trb.p, npts, (const double2 *)bp.p, nphi, seed, (double *)bo.p, s);
+    }
+    check_hip(hipGetLastError(), "azimuthal launch");
+    check_hip(hipMemcpyAsync(out, bo.p, (size_t)npts * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    check_hip(hipStreamSynchronize(s), "azimuthal");
+}
 
 // R^T with R = Rx(a) Ry(b) Rz(c) (EulerD, Amplitude.h:557-573; HybridOA.cu:60-63)
 void euler_rt(double a, double b, double c, double m[9]) {
@@ -1345,6 +1372,19 @@ int dplus_orientation_average_noalloc(int nleaf, const dplus_leaf *leaves, int n
         orientation_average(D, nq, q, p, intensity, st, s);
         timing_collect();
         if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    })
+}
+
+int dplus_azimuthal_average(int nleaf, const dplus_leaf *leaves, int npts, const double *q, const double *theta,
+                            int nphi, unsigned long long seed, double *intensity) {
+    API_GUARD({
+        if (!leaves || npts < 0 || (npts > 0 && (!q || !theta || !intensity)))
+            throw Error(kErrInvalidArgs, "null argument");
+        Device::get().require();
+        hipStream_t s = Device::get().stream;
+        DockSet D(nleaf, leaves, s);
+        azimuthal_average(D, npts, q, theta, nphi, seed, intensity, s);
+        timing_collect();
     })
 }
 

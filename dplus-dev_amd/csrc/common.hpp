@@ -122,6 +122,7 @@ DPLUS_HD void mc_stratify(long long d, int chunk, int nu, int nv, double &u, dou
 
 constexpr uint32_t kStreamMC = 0x4D43u;
 constexpr uint32_t kStreamVegas = 0x5645u;
+constexpr uint32_t kStreamAzimuth = 0x415Au;
 
 // ---------------------------------------------------------------------------
 // Error codes of the C ABI (values follow Common/Common.h:184-228 where a

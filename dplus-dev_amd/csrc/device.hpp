@@ -290,6 +290,10 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
                          dplus_oa_stats *st, hipStream_t s, Chi2Request *chi = nullptr);
 void gk_average(const DockSet &D, int nq, const double *q, double eps, long long max_depth, double *out,
                 dplus_oa_stats *st, hipStream_t s);
+// azimuthal averages of |A|^2 at npts (q, theta) points over nphi stratified phi
+// (the 2-D generate path); q outside the grid (beyond grid_size/2 + 1 shells) is rejected
+void azimuthal_average(const DockSet &D, int npts, const double *q, const double *theta, int nphi, uint64_t seed,
+                       double *out, hipStream_t s);
 // Gauss-Kronrod levels are evaluated on the device in batches of at most this
 // many phi intervals (32 B task + 16 B sums each)
 constexpr size_t kGkBatch = (size_t)1 << 20;

@@ -853,6 +853,40 @@ __global__ __launch_bounds__(256) void docked_cart_batch(const T4 *const *__rest
     out[i] = docked_intensity<T, T4>(grids, geo, rots, nrot, trans, q);
 }
 
+// Azimuthal average of the 2-D generate path (DomainModel::CalculateIntensity2DMatrix,
+// Amplitude.cpp:843-993): point p = (q sin(theta), q cos(theta)) of the (q_z, q_perp)
+// plane; I_p = (1 / nphi) sum_k |A(q, theta, phi_k)|^2 over stratified azimuths
+// phi_k = 2 pi (k + u_k) / nphi, u_k from the Philox stream keyed by (seed, k, p).
+// The reference draws phi uniformly per (q, theta) point with a per-layer
+// mt19937 (AverageIntensitiesBetweenLayers2D, :2825-2897); one stratum per
+// sample gives the same expectation with a fraction of the variance.  One block
+// per point; the block's sum is formed in a fixed order.
+template <typename T, typename T4>
+__global__ __launch_bounds__(256) void azimuthal_kernel(const T4 *const *__restrict__ grids,
+                                                        const LeafGeo *__restrict__ geo,
+                                                        const RotDev *__restrict__ rots, int nrot,
+                                                        const double4 *__restrict__ trans,
+                                                        const double2 *__restrict__ pts, int nphi, uint64_t seed,
+                                                        double *__restrict__ out) {
+    __shared__ double red[4];
+    const int p = blockIdx.x;
+    const double2 pt = pts[p];
+    double sum = 0.0;
+    for (int k = threadIdx.x; k < nphi; k += blockDim.x) {
+        double u, v;
+        uniform_pair(seed, (uint32_t)k, (uint32_t)p, 0u, kStreamAzimuth, u, v);
+        const double ph = k2Pi * ((double)k + u) / (double)nphi;
+        double sp, cp;
+        sincos(ph, &sp, &cp);
+        const double q[3] = {pt.x * cp, pt.x * sp, pt.y};
+        sum += docked_intensity<T, T4>(grids, geo, rots, nrot, trans, q);
+    }
+    sum = wave_sum(sum);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) out[p] = (red[0] + red[1] + red[2] + red[3]) / (double)nphi;
+}
+
 // Gauss-Kronrod phi intervals: task t = (q sin(theta), q cos(theta), a, b) of
 // one interval; its 15 integrand points are phi = a + b x_k.  A workgroup of
 // 240 threads evaluates the points of 16 tasks, then one thread per task forms
@@ -1009,6 +1043,18 @@ void launch_docked_gk(bool fp64, const void *const *grids, const LeafGeo *geo, c
     else
         docked_gk_batch<float, float4><<<nb, bs, 0, s>>>((const float4 *const *)grids, geo, rots, nrot, trans,
                                                           ntasks, tasks, nodes, out);
+}
+
+void launch_azimuthal(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
+                      const double4 *trans, int npts, const double2 *pts, int nphi, uint64_t seed, double *out,
+                      hipStream_t s) {
+    if (npts <= 0) return;
+    if (fp64)
+        azimuthal_kernel<double, double4><<<npts, 256, 0, s>>>((const double4 *const *)grids, geo, rots, nrot, trans,
+                                                               pts, nphi, seed, out);
+    else
+        azimuthal_kernel<float, float4><<<npts, 256, 0, s>>>((const float4 *const *)grids, geo, rots, nrot, trans,
+                                                             pts, nphi, seed, out);
 }
 
 void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots,

@@ -4,7 +4,7 @@ symmetry docking and orientation averages) behind a C ABI
 from ._lib import (CALC_ATOMIC_FORMFACTORS, CALC_DUMMY_SOLVENT, OA_GK, OA_MC, OA_VEGAS, PRECISION_FP64,
                    PRECISION_MIXED, DplusError, device_info, lib)
 from ._lib import copy_probe, gather_probe, microbench_valu, timing_enable, timing_get, timing_reset
-from .grid import (Docking, Grid, Structure, build_grids, build_shapes_clear, build_shapes_load, build_shapes_save,
+from .grid import (Docking, Grid, Structure, azimuthal_average, build_grids, build_shapes_clear, build_shapes_load, build_shapes_save,
                    dock_grid, dock_grid_space_filling, excluded_volume_boxes, hydration_boxes, index_from_indices, indices_from_index,
                    orientation_average, synth_walk)
 

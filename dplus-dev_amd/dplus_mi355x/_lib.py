@@ -141,6 +141,10 @@ SIGNATURES = {
     "dplus_backend_get_all_model_metadata": (ctypes.c_int, [ctypes.c_void_p, _VP]),
     "dplus_backend_initialize_cache": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "dplus_backend_start_generate": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    "dplus_backend_start_generate_2d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    "dplus_backend_get_generate_2d_results": (ctypes.c_int, [ctypes.c_void_p, _VP]),
+    "dplus_azimuthal_average": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_int, _D, _D, ctypes.c_int,
+                                               ctypes.c_ulonglong, _D]),
     "dplus_backend_get_job_status": (ctypes.c_int, [ctypes.c_void_p, _VP]),
     "dplus_backend_get_generate_results": (ctypes.c_int, [ctypes.c_void_p, _VP]),
     "dplus_backend_get_model_ptrs": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint), ctypes.c_int, _I]),

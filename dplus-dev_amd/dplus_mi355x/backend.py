@@ -50,6 +50,14 @@ class Backend:
         self._wrapper.start_generate(data.encode("utf-8") if isinstance(data, str) else data, useGPU)
 
     @_guard
+    def start_generate_2D(self, data, useGPU=True):
+        self._wrapper.start_generate_2D(data.encode("utf-8") if isinstance(data, str) else data, useGPU)
+
+    @_guard
+    def get_generate_2D_results(self):
+        return json.loads(self._wrapper.get_generate_2D_results())
+
+    @_guard
     def get_job_status(self):
         return json.loads(self._wrapper.get_job_status())
 
@@ -106,6 +114,14 @@ class EmbeddedLocalRunner:
         if status["code"] not in (0,):
             raise BackendError(RuntimeError(json.dumps({"code": status["code"], "message": status["message"]})))
         return self.wrapper.get_generate_results()
+
+    def generate2D(self, args, use_gpu=True):
+        """CalculationRunner.py:741-756: the 2-D job; returns {"Headers", "2DGraph"}."""
+        self.wrapper.start_generate_2D(json.dumps(args), useGPU=use_gpu)
+        status = self.wait()
+        if status["code"] not in (0,):
+            raise BackendError(RuntimeError(json.dumps({"code": status["code"], "message": status["message"]})))
+        return self.wrapper.get_generate_2D_results()
 
     def get_job_status(self):
         return self.wrapper.get_job_status()

@@ -194,6 +194,15 @@ def orientation_average(dock, q, method=L.OA_MC, seed=1, max_dirs=1 << 20, chunk
     return out, st
 
 
+def azimuthal_average(dock, q, theta, nphi=4096, seed=1):
+    """dplus_azimuthal_average: mean over nphi stratified phi of |A(q_i, theta_i, phi)|^2."""
+    q, theta = L.f64(q), L.f64(theta)
+    out = np.zeros(len(q))
+    L.check(L.lib().dplus_azimuthal_average(dock.n, dock.arr, len(q), L.dp(q), L.dp(theta), int(nphi), int(seed),
+                                            L.dp(out)))
+    return out
+
+
 def orientation_average_noalloc(dock, q, method=L.OA_MC, seed=1, max_dirs=1 << 20, chunk=8192, eps=0.0,
                                 vegas_iterations=10, vegas_points_per_bin=4):
     """orientation_average (Monte Carlo or VEGAS) with every device buffer taken

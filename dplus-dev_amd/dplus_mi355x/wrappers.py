@@ -62,6 +62,16 @@ class BackendWrapper:
     def start_generate(self, state, useGPU=True):
         _call(L.lib().dplus_backend_start_generate(self._h, _text(state), 1 if useGPU else 0))
 
+    def start_generate_2D(self, state, useGPU=True):
+        """BackendWrapper.pyx:22-23: the (q_z, q_perp) intensity matrix job."""
+        _call(L.lib().dplus_backend_start_generate_2d(self._h, _text(state), 1 if useGPU else 0))
+
+    def get_generate_2D_results(self):
+        """BackendWrapper.pyx:31-32: {"Headers", "2DGraph"} JSON."""
+        p = ctypes.c_void_p()
+        _call(L.lib().dplus_backend_get_generate_2d_results(self._h, ctypes.byref(p)))
+        return L.take_string(p).decode()
+
     def get_job_status(self):
         p = ctypes.c_void_p()
         _call(L.lib().dplus_backend_get_job_status(self._h, ctypes.byref(p)))

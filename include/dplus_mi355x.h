@@ -229,6 +229,15 @@ int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const
 int dplus_orientation_average_noalloc(int nleaf, const dplus_leaf *leaves, int nq, const double *q,
                                       const dplus_oa_params *p, double *intensity, dplus_oa_stats *st);
 
+/* Azimuthal average of the 2-D generate path (DomainModel::CalculateIntensity2DMatrix,
+ * Backend/Backend/Amplitude.cpp:843-993, its per-point MC over phi in
+ * AverageIntensitiesBetweenLayers2D :2825-2897): intensity[i] = mean over nphi
+ * stratified azimuths phi of |A(q[i], theta[i], phi)|^2 for the docked leaves,
+ * q[i] in [0, qmax], theta[i] in [0, pi]; phi from a Philox stream keyed by
+ * (seed, sample, point). */
+int dplus_azimuthal_average(int nleaf, const dplus_leaf *leaves, int npts, const double *q, const double *theta,
+                            int nphi, unsigned long long seed, double *intensity);
+
 /* ------------------------------------------------------------------ manual symmetry */
 /* ManualSymmetry::calculateGrid (DefaultModels/Symmetries.cpp:879-962, GPU
  * ConstructManualSymmetryJacobianKernel Backend/GPU/ManSymGenJacob.cu:35-282):
@@ -302,6 +311,14 @@ int dplus_backend_initialize_cache(dplus_backend *b, const char *dir);
  * malformed tree, ERROR_JOBRUNNING (2) while a job runs; the job itself runs
  * on a worker thread. */
 int dplus_backend_start_generate(dplus_backend *b, const char *state, int use_gpu);
+/* StartGenerate2D(state, useGPU) (PythonBackendWrapper.cpp:108-128, BackendWrapper.cpp:376-420):
+ * the same state and job protocol; the job computes the (q_z, q_perp) intensity
+ * matrix over every pair of x values (DomainModel::CalculateIntensity2DMatrix,
+ * Amplitude.cpp:843-993) as azimuthal averages (dplus_azimuthal_average). */
+int dplus_backend_start_generate_2d(dplus_backend *b, const char *state, int use_gpu);
+/* GetGenerate2DResults (PythonBackendWrapper.cpp:160-174, BackendWrapper.cpp:583-624):
+ * {"Headers": [...], "2DGraph": [[I(q_z = x[i], q_perp = x[j]) for j] for i]} */
+int dplus_backend_get_generate_2d_results(dplus_backend *b, char **out);
 /* GetJobStatus (PythonBackendWrapper.cpp:130-143, Conversions.cpp:49-64):
  * {"isRunning": bool, "progress": double, "code": int (-1 while running), "message": str} */
 int dplus_backend_get_job_status(dplus_backend *b, char **out);

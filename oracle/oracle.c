@@ -17,6 +17,8 @@
  *                       mutual agreement of the three quadratures
  *   dock grid / docked  point-atom subunits docked = Debye sum of the expanded list
  *   boxes               box sums equal the sum over unit voxels (exact decomposition)
+ *   azimuthal           two atoms on the x axis: 2 f^2 (1 + J0(2 q d sin theta)) (Bessel closed
+ *                       form of the phi average); isotropic amplitude (exact)
  */
 #include "oracle.h"
 
@@ -652,6 +654,28 @@ void orc_docked_cart(int nleaf, const orc_leaf *leaves, double qx, double qy, do
             out[1] += (a[1] * sc + a[0] * ss) * rt[3];
             cum += L->ntrans[r];
         }
+    }
+}
+
+/* The 2-D generate path's average over phi at fixed (q, theta)
+ * (DomainModel::CalculateIntensity with thetaIn >= 0, Amplitude.cpp:1474-1592, and
+ * AverageIntensitiesBetweenLayers2D :2825-2897), with one Philox-drawn sample per
+ * phi stratum keyed by (seed, sample, point). */
+#define ORC_STREAM_AZIMUTH 0x415Au
+void orc_azimuthal(int nleaf, const orc_leaf *leaves, int npts, const double *q, const double *theta, int nphi,
+                   uint64_t seed, double *out) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < npts; p++) {
+        double sum = 0.0;
+        const double st = sin(theta[p]), ct = cos(theta[p]);
+        for (int k = 0; k < nphi; k++) {
+            double u, v, a[2];
+            orc_uniform_pair(seed, (uint32_t)k, (uint32_t)p, 0u, ORC_STREAM_AZIMUTH, &u, &v);
+            const double ph = M_2PI_D * ((double)k + u) / (double)nphi;
+            orc_docked_cart(nleaf, leaves, q[p] * st * cos(ph), q[p] * st * sin(ph), q[p] * ct, a);
+            sum += a[0] * a[0] + a[1] * a[1];
+        }
+        out[p] = sum / (double)nphi;
     }
 }
 

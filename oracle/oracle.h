@@ -103,6 +103,10 @@ int orc_vegas_oa(int nleaf, const orc_leaf *leaves, int nq, const double *qs, ui
                  int iterations, int points_per_bin, double eps, double *out, int *iters_used);
 /* amplitude at a cartesian q for docked leaves, CPU getAmplitude semantics (Amplitude.cpp:3035-3056) */
 void orc_docked_cart(int nleaf, const orc_leaf *leaves, double qx, double qy, double qz, double *out);
+/* azimuthal average of the 2-D generate path: out[p] = mean over nphi stratified
+ * phi of |A(q[p], theta[p], phi)|^2 (phi_k = 2 pi (k + u_k) / nphi) */
+void orc_azimuthal(int nleaf, const orc_leaf *leaves, int npts, const double *q, const double *theta, int nphi,
+                   uint64_t seed, double *out);
 /* adaptive Gauss-Kronrod OA (Amplitude.cpp:1416-1467) */
 int orc_gk_oa(int nleaf, const orc_leaf *leaves, int nq, const double *qs, double eps,
               long long max_depth, double *out, long long *evals);

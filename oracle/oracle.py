@@ -68,6 +68,8 @@ def lib():
                                    ctypes.c_int, ctypes.c_int, ctypes.c_double, _D, _I]
         L.orc_gk_oa.argtypes = [ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_int, _D, ctypes.c_double,
                                 ctypes.c_longlong, _D, _LL]
+        L.orc_azimuthal.argtypes = [ctypes.c_int, ctypes.POINTER(Leaf), ctypes.c_int, _D, _D, ctypes.c_int,
+                                    ctypes.c_uint64, _D]
         L.orc_mt19937_pairs.argtypes = [ctypes.c_uint32, ctypes.c_longlong] + [ctypes.c_double] * 4 + [_D]
         _lib = L
     return _lib
@@ -278,6 +280,13 @@ def docked_cart(dock, q3):
     lib().orc_docked_cart(dock.n, dock.arr, ctypes.c_double(q3[0]), ctypes.c_double(q3[1]),
                           ctypes.c_double(q3[2]), _dp(out))
     return complex(out[0], out[1])
+
+
+def azimuthal(dock, q, theta, nphi, seed):
+    q, theta = f64(q), f64(theta)
+    out = np.zeros(len(q))
+    lib().orc_azimuthal(dock.n, dock.arr, len(q), _dp(q), _dp(theta), int(nphi), ctypes.c_uint64(seed), _dp(out))
+    return out
 
 
 def mc_oa(dock, qs, seed, max_dirs, chunk=8192, eps=0.0):

@@ -370,6 +370,43 @@ def test_tubulin_symmetry_fixture(name, tmp_path):
         assert frac[0] < 0.5 and frac[1] < 0.3 and frac[2] < 0.1, frac
 
 
+# ---------------------------------------------------------------- 2-D generate
+def test_generate_2d_matrix(oracle, tmp_path):
+    """StartGenerate2D / GetGenerate2DResults on two carbon atoms at (+-0.6, 0, 0) nm
+    (no solvent): an nq x nq matrix over (q_z = x[i], q_perp = x[j]); pairs with
+    |q| beyond qMax hold only the constant; every other entry is
+    scale * 2 f^2 (1 + J0(2 q d sin(theta))) + constant with theta = |atan2(q_perp, q_z)|,
+    the closed form of the phi average, within the grid's interpolation error."""
+    from scipy.special import j0
+    from dplus_mi355x.backend import EmbeddedLocalRunner
+    say("[case] generate 2-D: two atoms, Bessel closed form")
+    d = 0.6
+    path = str(tmp_path / "two.pdb")
+    write_pdb(path, ["C", "C"], np.array([[d, 0.0, 0.0], [-d, 0.0, 0.0]]))
+    x = list(np.linspace(0.0, 4.0, 21))
+    st = state(domain(60, iters=4096), [pdb_model(path, 4, extra=(1, 0, 1, 0.05, 0.14, 0.28, 0, 0, 0, 4))],
+               scale=2.0, constant=0.5)
+    r = EmbeddedLocalRunner()
+    res = r.generate2D({"state": st, "x": x})
+    M = np.array(res["2DGraph"])
+    assert M.shape == (21, 21) and len(res["Headers"]) >= 1
+    qx = np.array(x)
+    qz, qp = np.meshgrid(qx, qx, indexing="ij")
+    qq = np.sqrt(qz ** 2 + qp ** 2)
+    th = np.abs(np.arctan2(qp, qz))
+    out = qq > 4.0
+    assert np.all(M[out] == 0.5)
+    coef9 = np.array([[2.31, 20.8439, 1.02, 10.2075, 1.5886, 0.5687, 0.865, 51.6512, 0.2156]])
+    ions = oracle.Ions(coef9, np.array([0.1577]), np.array([2]), 0.0, 1.0, oracle.CALC_ATOMIC_FORMFACTORS)
+    f2 = np.array([oracle.form_factors(ions, v)[0] ** 2 for v in qq[~out]])
+    ref = 2.0 * 2 * f2 * (1 + j0(2 * qq[~out] * d * np.sin(th[~out]))) + 0.5
+    err = np.abs(M[~out] - ref) / (2.0 * 4 * f2)
+    say(f"    max err / (scale 4 f^2) {err.max():.2e}")
+    assert err.max() < 3e-3
+    # the 1-D job still runs after a 2-D one on the same backend
+    r.generate({"state": st, "x": x})
+
+
 # ---------------------------------------------------------------- voxelised solvent
 VOXDIR = os.path.join(GOLDEN, "voxel")
 VOX_CASES = sorted(f[:-5] for f in os.listdir(VOXDIR) if f.endswith(".json"))

@@ -420,3 +420,24 @@ def test_2000_atoms_grid100_500q_mc_and_gk_vs_debye(dplus, oracle):
     # MC median 1.1e-3, max 3.3e-3
     assert np.median(egk) < 3e-3 and egk.max() < 1e-2
     assert np.median(err) < 3e-3 and err.max() < 1e-2
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_azimuthal_average_matches_oracle(dplus, oracle, small, prec):
+    """dplus_azimuthal_average (the 2-D path's phi average) against the oracle on the
+    same stratified Philox azimuths, a single grid and a docked helix."""
+    import dplus_mi355x.synth as S
+    say(f"[case] azimuthal average precision={prec}")
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    g.download()
+    rng = np.random.default_rng(4)
+    q = np.concatenate([rng.uniform(0, s["qmax"], 60), [0.0, s["qmax"]]])
+    th = np.concatenate([rng.uniform(0, np.pi, 60), [0.3, np.pi]])
+    rot, nt, tr = S.rows_to_docking(S.helix_rows(6, 1.5, 0.4, 33.0))
+    for dock, odock in ((dplus.Docking.single(g), oracle.Docked.single(s["gs"], s["qmax"], g.data, g.interp)),
+                        (dplus.Docking([(g, rot, nt, tr)]),
+                         oracle.Docked([(s["gs"], s["qmax"], g.data, g.interp, rot, nt, tr)]))):
+        mine = dplus.azimuthal_average(dock, q, th, nphi=1000, seed=3)
+        ref = oracle.azimuthal(odock, q, th, 1000, 3)
+        assert np.max(np.abs(mine - ref) / ref) < (1e-10 if prec == 0 else 1e-4)

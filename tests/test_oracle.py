@@ -416,3 +416,30 @@ def test_rows_to_docking_groups_equal_rotations():
     grouped, _ = O.gk_oa(O.Docked([(gs, qmax, sub, co, rot, nt, tr)]), q, 1e-6)
     single, _ = O.gk_oa(O.Docked([(gs, qmax, sub, co, flat_rot, [1] * 5, rows[:, :3])]), q, 1e-6)
     assert np.allclose(grouped, single, rtol=1e-9)
+
+
+def test_azimuthal_average_two_atoms_bessel():
+    """The 2-D path's phi average: two identical atoms at (+-d, 0, 0) give
+    |A|^2 = 4 f^2 cos^2(q d sin(theta) cos(phi)), whose average over phi is
+    2 f^2 (1 + J0(2 q d sin(theta))); atoms at (0, 0, +-d) give 4 f^2 cos^2(q d cos(theta))
+    for every phi.  Grid 60 (qmax 5): within the grid's interpolation error."""
+    from scipy.special import j0
+    gs, qmax, d = 60, 5.0, 0.6
+    coef9 = np.array([[2.31, 20.8439, 1.02, 10.2075, 1.5886, 0.5687, 0.865, 51.6512, 0.2156]])
+    ions = O.Ions(coef9, np.array([0.1577]), np.array([2]), 0.0, 1.0, O.CALC_ATOMIC_FORMFACTORS)
+    rng = np.random.default_rng(1)
+    q = rng.uniform(0.2, 4.0, 40)
+    th = rng.uniform(0.0, np.pi, 40)
+    for axis in (0, 2):
+        xyz = np.zeros((2, 3))
+        xyz[0, axis], xyz[1, axis] = d, -d
+        data = O.grid_fill_pdb(gs, qmax, xyz, np.zeros(2, np.int32), ions)
+        co = O.grid_splines(gs, data)
+        I = O.azimuthal(O.Docked.single(gs, qmax, data, co), q, th, 2048, 7)
+        f2 = np.array([O.form_factors(ions, x)[0] ** 2 for x in q])
+        if axis == 0:
+            ref = 2 * f2 * (1 + j0(2 * q * d * np.sin(th)))
+        else:
+            ref = 4 * f2 * np.cos(q * d * np.cos(th)) ** 2
+        err = np.abs(I - ref) / (4 * f2)
+        assert err.max() < 2e-3, (axis, err.max())
