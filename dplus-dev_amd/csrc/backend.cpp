@@ -16,6 +16,8 @@
 
 #include <sys/stat.h>
 
+#include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -40,6 +42,8 @@ void solvent_boxes(int n, const double *xyz, const double *rad, double probe, do
                    bool fill_holes, bool want_layer, bool want_excl, std::vector<int> &ldims, std::vector<double> &lcom,
                    std::vector<int> &edims, std::vector<double> &ecom);
 void euler_rt(double a, double b, double c, double m[9]);
+std::vector<std::array<double, 6>> lua_symmetry_rows(const std::string &script,
+                                                     const std::vector<std::vector<double>> &params, int *nlp);
 std::string amp_legacy_bytes(Grid &g, const std::string &header, hipStream_t s);
 void amp_write(Grid &g, const std::string &path, const std::string &title, hipStream_t s);
 Grid *amp_read(const std::string &path, int precision, std::string &header, hipStream_t s);
@@ -53,6 +57,7 @@ struct ModelNode {
     bool centered = true, use_grid = true;
     std::vector<double> extra;
     std::vector<std::vector<double>> rows;
+    std::vector<std::vector<double>> script_params;  // a scripted symmetry's layer parameters
     double loc[6] = {0, 0, 0, 0, 0, 0};
     std::vector<ModelNode> children;
 };
@@ -84,6 +89,25 @@ ModelNode parse_model(const json::Value &m, std::vector<unsigned> &ptrs) {
     ptrs.push_back(n.ptr);
     if (const json::Value *c = m.find("Children"))
         for (auto &ch : c->arr) n.children.push_back(parse_model(ch, ptrs));
+    std::string lower = n.type;
+    std::transform(lower.begin(), lower.end(), lower.begin(), ::tolower);
+    if (lower == "scripted symmetry") {
+        // LuaSymmetry (Symmetry.cpp:404-521): the script is the text of the file
+        // the model names (ParameterTreeConverter::GetScript,
+        // ParamTreeConversions.cpp:340-349); its Populate(p, nlayers) turns the
+        // layer parameters into rows {x, y, z, alpha, beta, gamma} (degrees),
+        // which then act as a manual symmetry's rows
+        if (n.filename.empty()) throw Error(kErrInvalidArgs, "invalid scripted symmetry, no script (model.Filename) defined");
+        std::ifstream f(n.filename, std::ios::binary);
+        if (!f) throw Error(kErrFileNotFound, "cannot open symmetry script " + n.filename);
+        std::stringstream text;
+        text << f.rdbuf();
+        n.type = "Scripted Symmetry";
+        n.script_params = n.rows;
+        n.rows.clear();
+        for (const auto &r : lua_symmetry_rows(text.str(), n.script_params, nullptr))
+            n.rows.emplace_back(r.begin(), r.end());
+    }
     return n;
 }
 
@@ -106,7 +130,12 @@ struct Spec {
 
 bool is_pdb(const ModelNode &n) { return n.type == "PDB"; }
 bool is_amp(const ModelNode &n) { return n.type == "AMP"; }
-bool is_manual_symmetry(const ModelNode &n) { return n.type == ",26" || n.type == "Manual Symmetry"; }
+bool is_scripted_symmetry(const ModelNode &n) { return n.type == "Scripted Symmetry"; }
+// a scripted symmetry holds its evaluated rows and is docked as a manual symmetry
+bool is_manual_symmetry(const ModelNode &n) {
+    return n.type == ",26" || n.type == "Manual Symmetry" || is_scripted_symmetry(n);
+}
+const char *symmetry_kind(const ModelNode &n);
 
 Spec parse_state(const json::Value &doc) {
     Spec s;
@@ -169,6 +198,9 @@ void matvec(const double A[9], const double v[3], double o[3]) {
 
 bool is_space_filling(const ModelNode &n) { return n.type == ",25" || n.type == "Space-filling Symmetry"; }
 bool is_symmetry(const ModelNode &n) { return is_manual_symmetry(n) || is_space_filling(n); }
+const char *symmetry_kind(const ModelNode &n) {
+    return is_space_filling(n) ? "Space-filling Symmetry" : is_scripted_symmetry(n) ? "Scripted Symmetry" : "Manual Symmetry";
+}
 
 struct SubLoc {
     double R[9];
@@ -293,7 +325,7 @@ std::string pdb_record(const PdbAtoms &A, size_t i, const PdbPoint &x) {
 
 std::string amp_header(const ModelNode &n, const Grid &g) {
     std::ostringstream o;
-    o << "# Model: " << (is_pdb(n) ? "PDB" : n.type == "AMP" ? "AMP" : is_space_filling(n) ? "Space-filling Symmetry" : "Manual Symmetry")
+    o << "# Model: " << (is_pdb(n) ? "PDB" : n.type == "AMP" ? "AMP" : symmetry_kind(n))
       << (n.name.empty() ? "" : " \"" + n.name + "\"") << "\n";
     if (!n.filename.empty()) o << "# File: " << n.filename << "\n";
     o << "# Grid size: " << g.grid_size << ", qmax: " << json::num(g.qmax) << " nm^-1, step " << json::num(g.step) << "\n";
@@ -490,8 +522,7 @@ struct Backend {
             return;
         }
         // hybrid: the symmetry is expanded into its sub-locations (Amplitude.cpp:594-631)
-        add_header(n.ptr, "# Model: " + std::string(is_space_filling(n) ? "Space-filling" : "Manual") +
-                              " Symmetry (docked at orientation-average time)\n");
+        add_header(n.ptr, "# Model: " + std::string(symmetry_kind(n)) + " (docked at orientation-average time)\n");
         const double sym_scale = n.extra.empty() ? 1.0 : n.extra[0];
         for (const SubLoc &L : sub_locations(n)) {
             double Rc[9], tc[3];
@@ -864,7 +895,9 @@ int dplus_backend_get_all_model_metadata(dplus_backend *b, char **out) {
             "{\"Type\":\",26\",\"Name\":\"Manual Symmetry\",\"LayerParameters\":[\"X\",\"Y\",\"Z\",\"Alpha\",\"Beta\","
             "\"Gamma\"],\"ExtraParameters\":[\"Scale\"]},"
             "{\"Type\":\",25\",\"Name\":\"Space-filling Symmetry\",\"LayerParameters\":[\"Distance\",\"Angle\","
-            "\"Repetitions\"],\"ExtraParameters\":[\"Scale\"]}],"
+            "\"Repetitions\"],\"ExtraParameters\":[\"Scale\"]},"
+            "{\"Type\":\"Scripted Symmetry\",\"Name\":\"Scripted Symmetry\",\"LayerParameters\":[\"Parameter\"],"
+            "\"ExtraParameters\":[\"Scale\"]}],"
             "\"OrientationMethods\":[\"Monte Carlo (Mersenne Twister)\",\"Adaptive (VEGAS) Monte Carlo\","
             "\"Adaptive Gauss Kronrod\"]}");
     })

@@ -27,6 +27,20 @@ XRAY = {
     "Fe": (11.76950, 4.76110, 7.35730, 0.30720, 3.52220, 15.35350, 2.30450, 76.88050, 1.0369),
     "Cu": (13.33800, 3.58280, 7.16760, 0.24700, 5.61580, 11.39660, 1.67350, 64.81260, 1.191),
     "Zn": (14.07430, 3.26550, 7.03180, 0.23330, 5.16520, 10.31630, 2.41000, 58.70970, 1.3041),
+    # ions (PDBReaderLib.cpp:2270-2489, the rows of ion indices 11, 16, 18, 26, 29, 31,
+    # 46, 50, 51, 59, 60, 62; Cu2+'s b2 as tabulated there, International Tables 0.244078)
+    "O1-": (4.19160, 12.8573, 1.63969, 4.17236, 1.52673, 47.0179, -20.307, -0.01404, 21.9412),
+    "Na1+": (3.25650, 2.66710, 3.93620, 6.11530, 1.39980, 0.20010, 1.00320, 14.03900, 0.404),
+    "Mg2+": (3.49880, 2.16760, 3.83780, 4.75420, 1.32840, 0.18500, 0.84970, 10.14110, 0.4853),
+    "Cl1-": (18.29150, 0.00660, 7.40840, 1.17170, 6.53370, 19.5424, 2.33860, 60.44860, -16.378),
+    "K1+": (7.95780, 12.63310, 7.49170, 0.76740, 6.35900, -0.0020, 1.19150, 31.91280, -4.9978),
+    "Ca2+": (15.63480, -0.00740, 7.95180, 0.60890, 8.43720, 10.3116, 0.85370, 25.99050, -14.875),
+    "Mn2+": (10.80610, 5.27960, 7.36200, 0.34350, 3.52680, 14.34300, 0.21840, 41.32350, 1.0874),
+    "Fe2+": (11.04240, 4.65380, 7.37400, 0.30530, 4.13460, 12.05460, 0.43990, 31.28090, 1.0097),
+    "Fe3+": (11.17640, 4.61470, 7.38630, 0.30050, 3.39480, 11.67290, 0.07240, 38.55660, 0.9707),
+    "Cu1+": (11.94750, 3.36690, 7.35730, 0.22740, 6.24550, 8.66250, 1.55780, 25.84870, 0.89),
+    "Cu2+": (11.81680, 3.37484, 7.11181, 244078, 5.78135, 7.98760, 1.14523, 19.89700, 1.14431),
+    "Zn2+": (11.97190, 2.99460, 7.38620, 0.20310, 6.46680, 7.08260, 1.39400, 18.09950, 0.7807),
     # implicit-hydrogen groups (fitted coefficients, PDBReaderLib.cpp:2479-2486)
     "CH": (0.894937, 55.7145, 0.894429, 4.03158, 3.78824, 24.8323, 3.14683e-6, 956.628, 1.42149),
     "CH2": (1.61908, 52.1451, 2.27205, 24.6589, 2.1815, 24.6587, 0.0019254, 152.165, 1.92445),
@@ -52,6 +66,11 @@ _ATOM_RADII = {"H": (0.120, 0.025, 0.053), "C": (0.170, 0.070, 0.067), "N": (0.1
                "K": (0.275, 0.220, 0.243), "Ca": (0.231, 0.180, 0.194), "Mn": (0.186, 0.140, 0.161),
                "Fe": (0.181, 0.140, 0.156), "Cu": (0.140, 0.135, 0.145), "Zn": (0.139, 0.135, 0.142)}
 _HEAVY = {"CH": "C", "CH2": "C", "CH3": "C", "NH": "N", "NH2": "N", "NH3": "N", "OH": "O", "SH": "S"}
+# an ion takes its element's radii and weight
+_ION_OF = {k: k.rstrip("+-0123456789") for k in XRAY if k[-1] in "+-"}
+_HEAVY.update(_ION_OF)
+for _ion, _el in _ION_OF.items():
+    SVG_RADIUS[_ion] = SVG_RADIUS[_el]
 VDW_RADIUS = {e: _ATOM_RADII[_HEAVY.get(e, e)][0] for e in SVG_RADIUS}
 EMP_RADIUS = {e: _ATOM_RADII[_HEAVY.get(e, e)][1] for e in SVG_RADIUS}
 CALC_RADIUS = {e: _ATOM_RADII[_HEAVY.get(e, e)][2] for e in SVG_RADIUS}
@@ -61,8 +80,11 @@ WEIGHT = {"H": 1.008, "C": 12.011, "N": 14.007, "O": 15.999, "P": 30.97376, "S":
           "Mg": 24.3050, "Cl": 35.45, "K": 39.0983, "Ca": 40.078, "Mn": 54.93804, "Fe": 55.845, "Cu": 63.546,
           "Zn": 65.38, "CH": 12.011, "CH2": 12.011, "CH3": 12.011, "NH": 14.007, "NH2": 14.007, "NH3": 14.007,
           "OH": 15.999, "SH": 32.06}
+WEIGHT.update({_ion: WEIGHT[_el] for _ion, _el in _ION_OF.items()})
 
-SPECIES = tuple(XRAY)  # table order, the order of ion rows
+# table order, the order of ion rows: elements by atomic number, each followed by its ions
+SPECIES = tuple(sorted(XRAY, key=lambda e: (list(XRAY).index(_HEAVY.get(e, e)) if e in _ION_OF else list(XRAY).index(e),
+                                            e in _ION_OF, e)))
 
 
 def ion_tables(elements):

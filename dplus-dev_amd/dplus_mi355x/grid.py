@@ -251,6 +251,30 @@ def excluded_volume_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05, fill_ho
     return _voxel_boxes(L.lib().dplus_excluded_volume_boxes, xyz, rad, probe, thickness, vox, fill_holes)
 
 
+def lua_symmetry_rows(script, params):
+    """Rows [k, 6] = (x, y, z, alpha, beta, gamma) (nm, degrees) that a symmetry
+    script's Populate(p, nlayers) returns for the layer parameters params
+    [nlayers][nlp] (LuaSymmetry::PreCalculate, Backend/Backend/Symmetry.cpp:463-521).
+    Runs on the host in the library's own interpreter; script errors raise DplusError.
+    Ragged layers are padded with zeros."""
+    layers = [np.atleast_1d(np.asarray(r, dtype=np.float64)) for r in params]
+    nl, nlp = len(layers), max((len(r) for r in layers), default=0)
+    P = np.zeros((nl, nlp))
+    for i, r in enumerate(layers):
+        P[i, :len(r)] = r
+    text = script.encode() if isinstance(script, str) else bytes(script)
+    n = ctypes.c_int()
+    rows = np.zeros(6 * 64)
+    for _ in range(2):
+        cap = len(rows) // 6
+        L.check(L.lib().dplus_lua_symmetry_rows(text, nl, nlp, L.dp(P) if P.size else None, cap, ctypes.byref(n),
+                                                L.dp(rows)))
+        if n.value <= cap:
+            return rows[:6 * n.value].reshape(-1, 6).copy()
+        rows = np.zeros(6 * n.value)
+    raise RuntimeError("the script's row count changed between calls")
+
+
 def synth_walk(seed, n, step=0.38, min_sep=0.12, axes=(3.0, 3.0, 3.0)):
     out = np.zeros(3 * n)
     L.check(L.lib().dplus_synth_walk(int(seed), int(n), float(step), float(min_sep), float(axes[0]),

@@ -276,6 +276,19 @@ int dplus_hydration_boxes(int n, const double *xyz, const double *rad, double pr
 int dplus_excluded_volume_boxes(int n, const double *xyz, const double *rad, double probe, double thickness,
                                 double vox, int fill_holes, int capacity, int *nbox, int *dims, double *com);
 
+/* ------------------------------------------------------------------ scripted symmetry */
+/* LuaSymmetry::PreCalculate (Backend/Backend/Symmetry.cpp:463-521) with the
+ * checks of VerifyAndObtainData (:579-605): runs a symmetry script (Lua 5.1
+ * text; the library carries its own interpreter for the subset such scripts
+ * use, without os and io) and calls Populate(p, nlayers) with p[i][j] =
+ * params[(i - 1) * nlp + j - 1] (1-based, j up to the script's Information.NLP,
+ * missing entries 0).  The result's rows are {x, y, z, alpha, beta, gamma}
+ * (nm, degrees), written row-major to rows[6 * k + c] when *nrows <= capacity;
+ * call again with capacity >= *nrows when it is larger.  A script error is
+ * ERROR_INVALIDARGS (21) with the script's message. */
+int dplus_lua_symmetry_rows(const char *script, int nlayers, int nlp, const double *params, int capacity, int *nrows,
+                            double *rows);
+
 /* ------------------------------------------------------------------ amplitude files */
 /* AmpGridAmplitude::ReadAmplitudeFromFile (Backend/Backend/Amplitude.cpp:3724-3796):
  * a ".amp" legacy stream (JacobianSphereGrid::ImportBinaryData, Grid.cpp:272-345)
@@ -292,7 +305,8 @@ int dplus_amp_write(dplus_grid *g, const char *path, const char *header);
 /* The job interface of PythonBackendWrapper (Backend/Backend/PythonBackendWrapper.h:15-46,
  * bound by PythonInterface/dplus/wrappers/BackendWrapper.pxd), for the model
  * types of the scattering hot path: "PDB" leaves, "AMP" leaves (amplitude
- * files) and Manual Symmetry (",26") nodes, orientation averaged by Monte
+ * files), Manual Symmetry (",26"), Space-filling Symmetry (",25") and
+ * Scripted Symmetry (a Lua script's rows) nodes, orientation averaged by Monte
  * Carlo, VEGAS or adaptive Gauss-Kronrod.  Strings returned through char**
  * are malloc'ed; release them with dplus_free. */
 typedef struct dplus_backend dplus_backend;

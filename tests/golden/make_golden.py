@@ -174,11 +174,91 @@ def voxel_cases():
     print("1jff", len(q))
 
 
+SCRIPTED_CASES = [f"Scripted_Symm_Impl_No_Charges_{h}_Hydr_{m}" for h in ("wo", "With")
+                  for m in ("MC", "VEGAS", "Hybrid_MC", "Hybrid_VEGAS", "Ga_Kr")]
+RING_CASES = [f"{e}_Ring{k}_{h}Solution_{s}_Solvation" for e in ("Impl", "Expl") for k in ("", "_DOL")
+              for h in ("", "Hybrid_") for s in ("with", "without")]
+MT_EXAMPLE = "/root/reference/Example Files/Examples/GPU/Slow_GPU/4_Microtubule"
+# the reference's symmetry scripts and the scripts under tests/golden/scripted/
+# that take the same layer parameters to the same rows
+SCRIPT_FOR = {"Space filling symmetry - Cartesian.lua": "cartesian_lattice.lua",
+              "Super Helical Left Hand MT Helix with in-ring twist.lua": "helical_ring.lua",
+              "Left_Hand_Helix.lua": "left_hand_helix.lua"}
+
+
+def scripted_cases():
+    """Scripted-symmetry fixtures under tests/golden/scripted/: the tubulin dimer
+    in the Cartesian-lattice script (MC, VEGAS, hybrid, GK, with and without the
+    hydration layer), the 13-unit tubulin ring as a helix script and as the
+    equivalent manual symmetry ("DOL") with implicit and explicit hydrogens, and
+    the microtubule example (672 copies from the left-handed helix script, VEGAS,
+    its model curve).  Each record holds the state, the curve, the PDB it names
+    (gzip'ed, shared when identical) and the script that replaces the
+    reference's (the scripts themselves are not copied)."""
+    import gzip
+    import hashlib
+    out = os.path.join(OUT, "scripted")
+    os.makedirs(out, exist_ok=True)
+    pdbs = {}
+
+    def store_pdb(path):
+        raw = open(path, "rb").read()
+        digest = hashlib.sha256(raw).hexdigest()
+        if digest not in pdbs:
+            name = {"453ecc065e91": "../symmetry/tubulin_dimer_no_charges.pdb.gz"}.get(digest[:12])
+            if name is None:
+                name = os.path.basename(path)[:-4].lower() + ".pdb.gz"
+                with gzip.GzipFile(os.path.join(out, name), "wb", mtime=0) as f:
+                    f.write(raw)
+            pdbs[digest] = name
+        return pdbs[digest], digest
+
+    def tree_files(m, acc):
+        if "Filename" in m:
+            acc.append((m["Type"], os.path.basename(m["Filename"].replace("\\", "/"))))
+        for c in m.get("Children", []):
+            tree_files(c, acc)
+        return acc
+
+    for name in SCRIPTED_CASES + RING_CASES:
+        d = os.path.join(GPU_REF, name)
+        state = json.load(open(os.path.join(d, name + ".state")))
+        files = tree_files(state["Domain"]["Populations"][0]["Models"][0], [])
+        pdb = [f for t, f in files if t == "PDB"][0]
+        script = [f for t, f in files if t == "Scripted Symmetry"]
+        pdb_name, digest = store_pdb(os.path.join(d, pdb))
+        std = [f for f in os.listdir(d) if f.startswith(name + "TestStandard")][0]
+        q, I, s, meta = read_standard(os.path.join(d, std))
+        rec = {"name": name, "source": f"reviewer_tests/files_for_tests/generate/gpu/short/{name}",
+               "pdb": pdb_name, "pdb_sha256": digest, "state": state, "q": q, "intensity": I, "sigma": s,
+               "meta": meta}
+        if script:
+            rec["reference_script"] = script[0]
+            rec["script"] = SCRIPT_FOR[script[0]]
+        with open(os.path.join(out, name + ".json"), "w") as f:
+            json.dump(rec, f)
+        print(name, len(q), pdb_name, rec.get("script"))
+    # the microtubule example: model curve of MT_State.state (program revision 4.0.5)
+    state = json.load(open(os.path.join(MT_EXAMPLE, "MT_State.state")))
+    pdb_name, digest = store_pdb(os.path.join(MT_EXAMPLE, "3j6f_Dimer_2_2_Added_H_GCentered.pdb"))
+    q, I, head = read_curve(os.path.join(MT_EXAMPLE, "MT_Model.out"))
+    rec = {"name": "Microtubule_Left_Hand_Helix", "source": "Example Files/Examples/GPU/Slow_GPU/4_Microtubule",
+           "pdb": pdb_name, "pdb_sha256": digest, "state": state, "q": q, "intensity": I,
+           "header": [h for h in head if h.startswith("#") and not h.startswith("###")],
+           "reference_script": "Left_Hand_Helix.lua", "script": SCRIPT_FOR["Left_Hand_Helix.lua"]}
+    with open(os.path.join(out, "Microtubule_Left_Hand_Helix.json"), "w") as f:
+        json.dump(rec, f)
+    print("microtubule", len(q), pdb_name)
+
+
 if __name__ == "__main__":
     import sys
     if "--voxel" in sys.argv:
         voxel_cases()
+    elif "--scripted" in sys.argv:
+        scripted_cases()
     else:
         main()
         symmetry_cases()
         voxel_cases()
+        scripted_cases()
