@@ -10,9 +10,10 @@ average by VEGAS (10 iterations x 1024 bins x 98 points).
 
 One step = one I(q) curve: amplitude grid of the resident atoms and boxes,
 spline coefficients, VEGAS average, I(q) back on the host.  The mixed-precision
-path is timed for --steps steps; the plain FP64 path (FP64 phases, sincos,
-accumulation) runs the same requests one at a time for comparison, and the
-maximum relative I(q) difference between the two is reported.
+path is timed for --steps steps; the FP64 path (FP64 phases, sincos,
+accumulation, with the mixed builder's algebra) runs the same requests one at a
+time for comparison, and the maximum relative I(q) difference between the two is
+reported; the plain FP64 builder (one node at a time) is timed beside it.
 """
 import argparse
 import json
@@ -177,7 +178,7 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, st
     serves the stream (at 2,000/s the queue never empties: the value is the
     saturated throughput and the latencies include queueing), then the same
     requests arriving at 70 % of that throughput (latency without a standing
-    queue); the plain FP64 path then runs the same requests one at a time
+    queue); the FP64 path then runs the same requests one at a time
     (fp64=False skips it).  batch_builds: small mixed builds share grouped
     builder launches."""
     from dplus_mi355x.serving import Engine
@@ -225,7 +226,7 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, st
         errs.append(float(np.max(np.abs(Im - I64) / np.abs(I64))))
     span64 = time.perf_counter() - t64
     eng64.close()
-    log(f"[serving] plain FP64, one request at a time: {n_requests / span64:.2f} curves/s; "
+    log(f"[serving] FP64, one request at a time: {n_requests / span64:.2f} curves/s; "
         f"max rel err mixed vs FP64 {max(errs):.2e}")
     return {"workload": f"{n_requests} Poisson requests at {rate:g}/s over {nstruct} seeded structures "
                         "(500..200,000 atoms), qmax U(2,8), 200..1,000 q, MC 65,536 dirs or VEGAS 5x4,096",
@@ -295,7 +296,7 @@ def fit_sessions(D, S, n_sessions=16, evals=4, directions=8192, nq=600, qmax=5.0
         t.join()
     span = time.perf_counter() - start
     lat = np.array(lat)
-    # accuracy against the plain FP64 path on the first two sessions' data point
+    # accuracy against the FP64 path on the first two sessions' data point
     errs = []
     for p, grids, sess in sessions[:2]:
         g64 = build(p, D.PRECISION_FP64)
@@ -324,6 +325,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=3)
+    ap.add_argument("--fp64-plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream and fit-session measurements")
     ap.add_argument("--serving-requests", type=int, default=512)
@@ -420,7 +422,8 @@ def main():
     log(f"[bench] mixed: {args.steps} curves in {elapsed:.3f} s; build {ms_build / max(nb, 1):.2f} ms/launch, "
         f"splines {ms_spl / max(ns, 1):.2f} ms, vegas {ms_vegas / max(args.steps, 1):.2f} ms/curve")
 
-    # ---- plain FP64 path on the same request
+    # ---- FP64 path on the same request: the paired builder's algebra in double
+    # (like for like), then the plain FP64 builder (one node at a time)
     lat64 = []
     for i in range(args.fp64_steps):
         t0 = time.perf_counter()
@@ -428,6 +431,18 @@ def main():
         lat64.append(time.perf_counter() - t0)
     err = float(np.max(np.abs(I32 - I64) / np.abs(I64)))
     log(f"[bench] fp64: {np.mean(lat64):.3f} s/curve; max rel I(q) mixed vs fp64 = {err:.2e}")
+    lat64p = []
+    D.set_fp64_builder(True)
+    try:
+        for i in range(args.fp64_plain_steps):
+            t0 = time.perf_counter()
+            I64p = curve(D, st, g64, W, seed=7)
+            lat64p.append(time.perf_counter() - t0)
+    finally:
+        D.set_fp64_builder(False)
+    err_plain = float(np.max(np.abs(I64p - I64) / np.abs(I64))) if lat64p else None
+    if lat64p:
+        log(f"[bench] fp64 plain builder: {np.mean(lat64p):.3f} s/curve; vs paired FP64 {err_plain:.2e}")
 
     # ---- the same grids averaged by adaptive Gauss-Kronrod (relative tolerance 1e-4, depth 8)
     gk = {}
@@ -524,6 +539,11 @@ def main():
                  "p99_ms": 1e3 * float(np.percentile(lat64, 99)), "steps": args.fp64_steps},
         "speedup_vs_fp64": value / fp64_value,
         "max_rel_err_vs_fp64": err,
+        # the plain FP64 builder (one node at a time, one double sincos per term) on the same curve
+        "fp64_plain": None if not lat64p else {
+            "value": 1.0 / float(np.mean(lat64p)), "unit": "curves/s", "steps": len(lat64p),
+            "max_rel_err_vs_fp64": err_plain},
+        "speedup_vs_fp64_plain": None if not lat64p else value * float(np.mean(lat64p)),
         "gauss_kronrod": gk_out,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,

@@ -26,8 +26,11 @@
 //    exceed kTile terms.  One launch builds up to kMaxBuildJobs grids, with a
 //    workgroup size (64..512 units) and atom-slice count chosen per grid
 //    class by the host's autotuner or occupancy model.
-//  * FP64 mode (build_atoms_fp64): the plain form -- double coordinates,
-//    phases and sincos, one node at a time, K nodes per thread.
+//  * FP64 mode (build_atoms_pairs64): the paired builder's algebra in double --
+//    double coordinates, phases, sincos (of q_x x + q_y y per node pair, of
+//    q_z z once per row and atom through LDS) and sums.  The plain form
+//    (build_atoms_fp64: one node at a time, K nodes per thread, one double
+//    sincos per term) stays selectable as the baseline FP64 path.
 
 #include <climits>
 #include <mutex>
@@ -244,6 +247,18 @@ __device__ __forceinline__ void store_pair(float4 *__restrict__ grid, const Pair
     }
 }
 
+__device__ __forceinline__ void store_pair64(double4 *__restrict__ grid, const PairUnit &pu, int p, double aR,
+                                             double aI, double bR, double bI, double scale) {
+    const long long k = pu.k0 + p, half = 3LL * pu.shell;
+    const double ar = aR * scale, ai = aI * scale, br = bR * scale, bi = bI * scale;
+    grid[index_from_indices(pu.shell, pu.j, k)] = make_double4(ar, ai, 0.0, 0.0);
+    grid[index_from_indices(pu.shell, pu.j, k + half)] = make_double4(br, bi, 0.0, 0.0);
+    if (2 * pu.j != half) {
+        grid[index_from_indices(pu.shell, half - pu.j, k + half)] = make_double4(ar, -ai, 0.0, 0.0);
+        grid[index_from_indices(pu.shell, half - pu.j, k)] = make_double4(br, -bi, 0.0, 0.0);
+    }
+}
+
 // job of workgroup `wg` (jobs are numbered in order of their first workgroup)
 __device__ __forceinline__ int batch_job(const BuildBatch &b, int wg, bool finishing) {
     int jb = 0;
@@ -405,6 +420,133 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs(BuildBatch batch) {
     }
 }
 
+// FP64 paired builder: the unit layout, slices and partials of build_atoms_pairs
+// with every quantity in double -- phases in radians, sin/cos by the double
+// sincos (full range reduction), cos/sin of q_z z shared by the row through
+// LDS, tile sums folded with the form factor or box weight in double.  The
+// values are the plain FP64 builder's up to rounding order.
+template <int kWg>
+__global__ __launch_bounds__(kWg) void build_atoms_pairs64(BuildBatch batch) {
+    constexpr int kRows = pair_rows<kWg>();
+    constexpr int kChunk64 = 32;
+    __shared__ double2 rowz[2][kRows][kChunk64];
+    __shared__ double rowqz[kRows];
+    const BuildJob &J = batch.job[batch_job(batch, (int)blockIdx.x, false)];
+    const double4 *__restrict__ atoms = J.atoms64;
+    const int4 *__restrict__ tiles = J.tiles;
+    const long long nunits = J.nunits;
+    const int local = (int)blockIdx.x - J.wg_begin;
+    const int slice = local / J.xblocks;
+    const long long u0 = (long long)(local - slice * J.xblocks) * kWg;
+    const long long u = u0 + threadIdx.x;
+    const bool live = u < nunits;
+    const long long uc = live ? u : nunits - 1;
+    const PairUnit pu = pair_unit(uc);
+    const int shell = pu.shell;
+    const long long j = pu.j, k0 = pu.k0;
+    int lrow, nrows;
+    {
+        const int s0 = shell_of_pair_unit(u0);
+        const long long g0 = rows_before(s0 - 1) + (u0 - pair_units_before(s0 - 1)) / s0;
+        const long long ul = min(u0 + (long long)kWg, nunits) - 1;
+        const int s1 = shell_of_pair_unit(ul);
+        const long long g1 = rows_before(s1 - 1) + (ul - pair_units_before(s1 - 1)) / s1;
+        lrow = (int)(rows_before(shell - 1) + j - g0);
+        nrows = (int)(g1 - g0 + 1);
+    }
+    const bool shared_z = nrows <= kRows;
+    double qx[kPair], qy[kPair], qz;
+    double aR[kPair], aI[kPair], bR[kPair], bI[kPair];
+    {
+        const double q = (double)shell * J.step;
+        double st, ct;
+        sincos(kPi * (double)j / (double)(3 * shell), &st, &ct);
+        qz = q * ct;
+#pragma unroll
+        for (int p = 0; p < kPair; p++) {
+            double sp, cp;
+            sincos(k2Pi * (double)(k0 + p) / (double)(6 * shell), &sp, &cp);
+            qx[p] = q * st * cp;
+            qy[p] = q * st * sp;
+            aR[p] = aI[p] = bR[p] = bI[p] = 0.0;
+        }
+    }
+    if (shared_z) {
+        rowqz[lrow] = qz;
+        __syncthreads();
+    }
+    int buf = 0;
+    const int t_end = J.slice_begin[slice + 1];
+    for (int t = J.slice_begin[slice]; t < t_end; t++) {
+        const int4 tl = tiles[t];
+        const double fion = tl.w == 0 ? J.ff[(size_t)shell * J.nion + tl.z] : 0.0;
+        double cc[kPair], ss[kPair], sc[kPair], cs[kPair];
+#pragma unroll
+        for (int p = 0; p < kPair; p++) cc[p] = ss[p] = sc[p] = cs[p] = 0.0;
+        const int a1 = tl.x + tl.y;
+        if (shared_z) {
+            for (int c0 = tl.x; c0 < a1; c0 += kChunk64) {
+                const int n = min(kChunk64, a1 - c0);
+                for (int e = threadIdx.x; e < nrows * kChunk64; e += kWg) {
+                    const int rr = e / kChunk64, c = e % kChunk64;
+                    if (c < n) {
+                        double sz, cz;
+                        sincos(rowqz[rr] * atoms[c0 + c].z, &sz, &cz);
+                        rowz[buf][rr][c] = make_double2(cz, sz);
+                    }
+                }
+                __syncthreads();
+                for (int c = 0; c < n; c++) {
+                    const double4 at = atoms[c0 + c];
+                    const double2 z = rowz[buf][lrow][c];
+#pragma unroll
+                    for (int p = 0; p < kPair; p++) {
+                        double su, cu;
+                        sincos(fma(qx[p], at.x, qy[p] * at.y), &su, &cu);
+                        cc[p] = fma(z.x, cu, cc[p]);
+                        ss[p] = fma(z.y, su, ss[p]);
+                        sc[p] = fma(z.y, cu, sc[p]);
+                        cs[p] = fma(z.x, su, cs[p]);
+                    }
+                }
+                buf ^= 1;
+            }
+        } else {
+            for (int a = tl.x; a < a1; a++) {
+                const double4 at = atoms[a];
+                double sz, cz;
+                sincos(qz * at.z, &sz, &cz);
+#pragma unroll
+                for (int p = 0; p < kPair; p++) {
+                    double su, cu;
+                    sincos(fma(qx[p], at.x, qy[p] * at.y), &su, &cu);
+                    cc[p] = fma(cz, cu, cc[p]);
+                    ss[p] = fma(sz, su, ss[p]);
+                    sc[p] = fma(sz, cu, sc[p]);
+                    cs[p] = fma(cz, su, cs[p]);
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < kPair; p++) {
+            const double f = tl.w == 0 ? fion : J.bcon[tl.z] * box_weight64(qx[p], qy[p], qz, J.bdims[tl.z], J.vox);
+            aR[p] = fma(f, cc[p] - ss[p], aR[p]);
+            aI[p] = fma(f, sc[p] + cs[p], aI[p]);
+            bR[p] = fma(f, cc[p] + ss[p], bR[p]);
+            bI[p] = fma(f, sc[p] - cs[p], bI[p]);
+        }
+    }
+    if (!live) return;
+    double *dst = J.partial + (size_t)slice * 12 * nunits + u;
+#pragma unroll
+    for (int p = 0; p < kPair; p++) {
+        dst[(4 * p + 0) * nunits] = aR[p];
+        dst[(4 * p + 1) * nunits] = aI[p];
+        dst[(4 * p + 2) * nunits] = bR[p];
+        dst[(4 * p + 3) * nunits] = bI[p];
+    }
+}
+
 // Sum of the atom slices of every pair unit of every job, in slice order,
 // scaled and stored; workgroup w serves units 256 (w - J.fin_begin) + .. of job J.
 __global__ __launch_bounds__(256) void pairs_finish(BuildBatch batch) {
@@ -419,7 +561,10 @@ __global__ __launch_bounds__(256) void pairs_finish(BuildBatch batch) {
         for (int y = 0; y < J.nslices; y++)
 #pragma unroll
             for (int m = 0; m < 4; m++) v[m] += partial[((size_t)y * 12 + 4 * p + m) * nunits + u];
-        store_pair(J.grid, pu, p, v[0], v[1], v[2], v[3], J.scale);
+        if (J.fp64)
+            store_pair64(J.grid64, pu, p, v[0], v[1], v[2], v[3], J.scale);
+        else
+            store_pair(J.grid, pu, p, v[0], v[1], v[2], v[3], J.scale);
     }
 }
 
@@ -538,17 +683,21 @@ long long pair_units(long long nhalf) {
     return pair_units_before(shell_of_half(nhalf) - 1);
 }
 
-// Resident workgroups of the paired builder of workgroup size wg on this device.
-static int pair_resident(int wg) {
-    static int cached[4] = {-1, -1, -1, -1};
+// Resident workgroups of the paired builder (mixed or FP64) of workgroup size wg on this device.
+static int pair_resident(int wg, bool fp64) {
+    static int cached[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
     static std::mutex mu;
-    const int slot = wg == 64 ? 0 : wg == 128 ? 1 : wg == 256 ? 2 : 3;
+    const int slot = (wg == 64 ? 0 : wg == 128 ? 1 : wg == 256 ? 2 : 3) + (fp64 ? 4 : 0);
     std::lock_guard<std::mutex> lk(mu);
     if (cached[slot] < 0) {
-        const void *fn = wg == 64    ? (const void *)build_atoms_pairs<false, 64>
-                         : wg == 128 ? (const void *)build_atoms_pairs<false, 128>
-                         : wg == 256 ? (const void *)build_atoms_pairs<false, 256>
-                                     : (const void *)build_atoms_pairs<false, 512>;
+        const void *fn = fp64 ? (wg == 64    ? (const void *)build_atoms_pairs64<64>
+                                 : wg == 128 ? (const void *)build_atoms_pairs64<128>
+                                 : wg == 256 ? (const void *)build_atoms_pairs64<256>
+                                             : (const void *)build_atoms_pairs64<512>)
+                              : (wg == 64    ? (const void *)build_atoms_pairs<false, 64>
+                                 : wg == 128 ? (const void *)build_atoms_pairs<false, 128>
+                                 : wg == 256 ? (const void *)build_atoms_pairs<false, 256>
+                                             : (const void *)build_atoms_pairs<false, 512>);
         int dev = 0, cus = 0, per_cu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
@@ -565,8 +714,8 @@ static int pair_resident(int wg) {
 // the device's resident workgroups; S slices turn W workgroups into W S
 // shorter ones.  S minimises ceil(W S / resident) / S, with 1 % per extra
 // slice for its partial sums.
-int pair_slices(long long nunits, int ntiles, int wg) {
-    const int resident = pair_resident(wg);
+int pair_slices(long long nunits, int ntiles, int wg, bool fp64) {
+    const int resident = pair_resident(wg, fp64);
     if (resident <= 0) return 1;
     const long long W = blocks_for(nunits, wg);
     int best = 1;
@@ -599,15 +748,26 @@ static void launch_pairs(int wg, const BuildBatch &b, unsigned nblocks, hipStrea
     }
 }
 
+template <int kWg>
+static void launch_pairs64(const BuildBatch &b, unsigned nblocks, hipStream_t s) {
+    build_atoms_pairs64<kWg><<<nblocks, kWg, 0, s>>>(b);
+}
+
+static bool plain_fp64_flag = getenv("DPLUS_FP64_PLAIN") != nullptr;
+bool fp64_plain_builder() { return plain_fp64_flag; }
+void set_fp64_plain_builder(bool plain) { plain_fp64_flag = plain; }
+
 void launch_build_batch(bool reduce, int wg, BuildJob *jobs, int njobs, hipStream_t s) {
     if (njobs < 1 || njobs > kMaxBuildJobs) throw Error(kErrInvalidArgs, "builder batch of 1..16 jobs");
     if (wg != 64 && wg != 128 && wg != 256 && wg != 512)
         throw Error(kErrInvalidArgs, "builder workgroup size must be 64, 128, 256 or 512");
+    const bool fp64 = jobs[0].fp64 != 0;
     BuildBatch b{};
     b.njobs = njobs;
     long long wgs = 0, fins = 0;
     for (int k = 0; k < njobs; k++) {
         BuildJob &J = jobs[k];
+        if ((J.fp64 != 0) != fp64) throw Error(kErrInvalidArgs, "a builder batch is all mixed or all FP64");
         if (J.nslices < 1 || J.nslices > kMaxSlices || J.nunits < 1)
             throw Error(kErrInvalidArgs, "builder job with no units or a slice count outside 1..8");
         J.xblocks = (int)blocks_for(J.nunits, wg);
@@ -618,10 +778,18 @@ void launch_build_batch(bool reduce, int wg, BuildJob *jobs, int njobs, hipStrea
         b.job[k] = J;
     }
     if (wgs > INT_MAX || fins > INT_MAX) throw Error(kErrInvalidArgs, "builder batch too large for one launch");
-    if (reduce)
+    if (fp64) {
+        switch (wg) {
+            case 64: launch_pairs64<64>(b, (unsigned)wgs, s); break;
+            case 128: launch_pairs64<128>(b, (unsigned)wgs, s); break;
+            case 256: launch_pairs64<256>(b, (unsigned)wgs, s); break;
+            default: launch_pairs64<512>(b, (unsigned)wgs, s); break;
+        }
+    } else if (reduce) {
         launch_pairs<true>(wg, b, (unsigned)wgs, s);
-    else
+    } else {
         launch_pairs<false>(wg, b, (unsigned)wgs, s);
+    }
     pairs_finish<<<(unsigned)fins, 256, 0, s>>>(b);
 }
 

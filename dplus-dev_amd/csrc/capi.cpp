@@ -491,14 +491,14 @@ void shape_record(int nshells, int ntiles, BuildShape shape) {
 
 BuildShape Structure::shape_for(const Grid &g, bool use_table) const {
     BuildShape sh;
-    if (use_table && shape_lookup(g.N, ntiles, &sh)) return sh;
+    // the shape table holds the mixed builder's tuned shapes; FP64 builds take the model
+    if (!g.fp64() && use_table && shape_lookup(g.N, ntiles, &sh)) return sh;
     sh.wg = 256;
-    sh.slices = pair_slices(pair_units(half_base(g.N + 1)), ntiles, sh.wg);
+    sh.slices = pair_slices(pair_units(half_base(g.N + 1)), ntiles, sh.wg, g.fp64());
     return sh;
 }
 
 BuildJob Structure::prepare(Grid &g, int slices, Pending &P, hipStream_t s) const {
-    if (g.fp64()) throw Error(kErrInvalidArgs, "the paired builder serves mixed-precision grids");
     std::vector<double> ff;
     form_factor_table(&desc, ion_counts, g.N, g.step, ff);
     // origin: sum of f~(0) over atoms + each box set's contrast * total volume
@@ -508,19 +508,22 @@ BuildJob Structure::prepare(Grid &g, int slices, Pending &P, hipStream_t s) cons
     if (boxes_on) P.origin += box_origin;
     // unreduced hardware sin/cos while every phase stays below 128 revolutions
     const double qtop = (double)(g.N + 1) * g.step;
-    P.reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
+    P.reduce = !g.fp64() && (qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr);
     P.ff.upload(ff.data(), ff.size() * sizeof(double), s);
     const long long nunits = pair_units(half_base(g.N + 1));
     const int S = std::max(1, std::min({slices, kMaxSlices, std::max(ntiles, 1)}));
     P.part.ensure((size_t)S * 12 * nunits * sizeof(double));
     BuildJob J{};
     J.atoms = (const float4 *)b32.p;
+    J.atoms64 = (const double4 *)b64.p;
+    J.fp64 = g.fp64() ? 1 : 0;
     J.tiles = (const int4 *)bt.p;
     J.ff = (const double *)P.ff.p;
     J.bdims = (const int4 *)bd.p;
     J.bcon = (const double *)bc.p;
     J.partial = (double *)P.part.p;
-    J.grid = (float4 *)g.dev.p;
+    J.grid = g.fp64() ? nullptr : (float4 *)g.dev.p;
+    J.grid64 = g.fp64() ? (double4 *)g.dev.p : nullptr;
     J.nunits = nunits;
     J.vox = desc.vox_step;
     J.step = g.step;
@@ -552,7 +555,9 @@ void Structure::finish(Grid &g, const Pending &P, bool host_copy, hipStream_t s)
 
 void Structure::build(Grid &g, bool host_copy, hipStream_t s, bool use_table) {
     const bool phase64 = getenv("DPLUS_PHASE64") != nullptr;
-    if (!g.fp64() && !phase64) {
+    // paired builder: every mixed grid (but the FP64-phase diagnostic) and, unless
+    // the plain form is selected, every FP64 grid
+    if (g.fp64() ? !fp64_plain_builder() : !phase64) {
         // per-build buffers (form factors of this grid's shells, slice partials): a
         // structure can be built into several grids at once on different streams
         Pending P;
@@ -1488,11 +1493,18 @@ int dplus_structure_build_shape(dplus_structure *sh, dplus_grid *gh, int *wg, in
         Grid *g = (Grid *)gh;
         if (!st || !g) throw Error(kErrInvalidArgs, "null argument");
         BuildShape b;
-        const bool t = shape_lookup(g->N, st->ntiles, &b);
+        const bool t = !g->fp64() && shape_lookup(g->N, st->ntiles, &b);
         if (!t) b = st->shape_for(*g);
         if (wg) *wg = b.wg;
         if (slices) *slices = b.slices;
         if (tuned) *tuned = t ? 1 : 0;
+    })
+}
+
+int dplus_set_fp64_builder(int plain) {
+    API_GUARD({
+        if (plain != 0 && plain != 1) throw Error(kErrInvalidArgs, "the FP64 builder form is 0 (paired) or 1 (plain)");
+        set_fp64_plain_builder(plain != 0);
     })
 }
 

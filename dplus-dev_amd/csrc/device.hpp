@@ -204,12 +204,15 @@ constexpr int kMaxSlices = 8;
 constexpr int kMaxBuildJobs = 16;
 struct BuildJob {
     const float4 *atoms;
+    const double4 *atoms64;  // FP64 jobs: the sources in double
     const int4 *tiles;
     const double *ff;
     const int4 *bdims;
     const double *bcon;  // contrast of each box shape
     double *partial;
     float4 *grid;
+    double4 *grid64;  // FP64 jobs: the grid's nodes
+    int fp64;         // 1: FP64 phases, sincos and sums into grid64
     long long nunits;
     double vox, step, scale;
     int nion, nslices;
@@ -270,8 +273,12 @@ std::vector<GroupDev> make_groups(const double *q, int nq, double step);
 // nunits units into, and one launch over several grids' jobs (njobs <=
 // kMaxBuildJobs, the same phase form `reduce` and workgroup size)
 long long pair_units(long long nhalf);
-int pair_slices(long long nunits, int ntiles, int wg);
+int pair_slices(long long nunits, int ntiles, int wg, bool fp64 = false);
+// one launch of up to kMaxBuildJobs jobs, all mixed or all FP64 (jobs[0].fp64)
 void launch_build_batch(bool reduce, int wg, BuildJob *jobs, int njobs, hipStream_t s);
+// FP64 grids: the paired builder (default) or the plain one node at a time
+bool fp64_plain_builder();
+void set_fp64_plain_builder(bool plain);
 // shape table of the autotuner, keyed by (grid shells, tile-count class)
 bool shape_lookup(int nshells, int ntiles, BuildShape *out);
 void shape_record(int nshells, int ntiles, BuildShape shape);

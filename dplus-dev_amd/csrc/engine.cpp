@@ -131,8 +131,8 @@ size_t request_bytes(const Request &r) {
         fp64 = r.precision == DPLUS_PRECISION_FP64;
         b += (size_t)P * (fp64 ? 32 : 16);
         // FP64 partial sums of the paired builder's atom slices (as many as a
-        // tuned launch shape may take)
-        if (!fp64) {
+        // tuned launch shape may take; FP64 grids too, unless built plainly)
+        if (!fp64 || !fp64_plain_builder()) {
             const long long units = pair_units(half_base(gs / 2 + 4));
             b += (size_t)kMaxSlices * 12 * units * sizeof(double);
         }

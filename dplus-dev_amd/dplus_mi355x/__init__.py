@@ -5,7 +5,7 @@ from ._lib import (CALC_ATOMIC_FORMFACTORS, CALC_DUMMY_SOLVENT, OA_GK, OA_MC, OA
                    PRECISION_MIXED, DplusError, device_info, lib)
 from ._lib import copy_probe, gather_probe, microbench_valu, timing_enable, timing_get, timing_reset
 from .grid import (Docking, Grid, Structure, azimuthal_average, build_grids, build_shapes_clear, build_shapes_load, build_shapes_save,
-                   dock_grid, dock_grid_space_filling, excluded_volume_boxes, hydration_boxes, index_from_indices, indices_from_index, lua_symmetry_rows,
+                   dock_grid, dock_grid_space_filling, excluded_volume_boxes, hydration_boxes, index_from_indices, indices_from_index, lua_symmetry_rows, set_fp64_builder,
                    orientation_average, synth_walk)
 
 

@@ -117,6 +117,7 @@ SIGNATURES = {
                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, _I, _I, _D]),
     "dplus_excluded_volume_boxes": (ctypes.c_int, [ctypes.c_int, _D, _D, ctypes.c_double, ctypes.c_double,
                                                    ctypes.c_double, ctypes.c_int, ctypes.c_int, _I, _I, _D]),
+    "dplus_set_fp64_builder": (ctypes.c_int, [ctypes.c_int]),
     "dplus_lua_symmetry_rows": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D, ctypes.c_int, _I,
                                                _D]),
     "dplus_structure_create": (ctypes.c_int, [ctypes.POINTER(PdbDesc), ctypes.POINTER(ctypes.c_void_p)]),

@@ -251,6 +251,12 @@ def excluded_volume_boxes(xyz, rad, probe=0.14, thickness=0.3, vox=0.05, fill_ho
     return _voxel_boxes(L.lib().dplus_excluded_volume_boxes, xyz, rad, probe, thickness, vox, fill_holes)
 
 
+def set_fp64_builder(plain):
+    """Form of later FP64 grid builds: False the paired builder (default), True the
+    plain form, one node at a time (dplus_set_fp64_builder)."""
+    L.check(L.lib().dplus_set_fp64_builder(1 if plain else 0))
+
+
 def lua_symmetry_rows(script, params):
     """Rows [k, 6] = (x, y, z, alpha, beta, gamma) (nm, degrees) that a symmetry
     script's Populate(p, nlayers) returns for the layer parameters params

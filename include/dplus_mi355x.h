@@ -153,6 +153,15 @@ int dplus_structure_build_shape(dplus_structure *s, dplus_grid *g, int *wg, int 
 int dplus_build_shapes_save(char **text, long long *len);
 int dplus_build_shapes_load(const char *text);
 int dplus_build_shapes_clear(void);
+/* The form of every later FP64 grid build in this process (no counterpart in
+ * the reference, whose FP64 path is its CPU calcAmplitude, PDBAmplitude.cpp:243-317):
+ * 0 (default) the paired builder's algebra in double -- Friedel halving, node
+ * pairs at phi and phi + pi sharing the sincos of q_x x + q_y y, the row's
+ * sincos of q_z z shared through LDS, atom slices; 1 the plain form, one node
+ * at a time with one double sincos per (node, source) term.  Both give the
+ * direct sum's values up to rounding order.  The environment variable
+ * DPLUS_FP64_PLAIN selects 1 at load. */
+int dplus_set_fp64_builder(int plain);
 
 /* ------------------------------------------------------------------ measurement */
 /* Per-kernel HIP-event timing on the library's stream: enable/disable, read the

@@ -83,6 +83,35 @@ def test_grid_build_edge_sizes(dplus, oracle, gs, natoms):
         assert relmax(g.data, ref) < tol
 
 
+@pytest.mark.parametrize("gs,natoms", [(6, 7), (24, 300), (70, 4000)])
+def test_fp64_paired_builder_matches_plain(dplus, gs, natoms):
+    """FP64 grids: the paired builder (the default FP64 form: Friedel halving,
+    phi / phi + pi node pairs sharing the sincos of q_x x + q_y y, the row's
+    sincos of q_z z through LDS, atom slices) and the plain builder (one node at
+    a time, one double sincos per term) give the same nodes and spline
+    coefficients to rounding, with hydration-layer and excluded-volume boxes."""
+    say(f"[case] FP64 paired vs plain builder, grid {gs}, {natoms} atoms")
+    xyz, ion, coef9, rad = structure(natoms, 30 + gs, radius=1.2 if natoms < 1000 else 2.6)
+    ld, lc = dplus.hydration_boxes(xyz, rad[ion], probe=0.14, thickness=0.28, vox=0.1)
+    ed, ec = dplus.excluded_volume_boxes(xyz, rad[ion], probe=0.14, thickness=0.28, vox=0.1)
+    st = dplus.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=dplus.CALC_ATOMIC_FORMFACTORS, box_dims=ld,
+                         box_com=lc, vox_step=0.1, box_contrast=33.3, excl_dims=ed, excl_com=ec,
+                         excl_contrast=-333.0)
+    g1 = dplus.Grid(gs, 5.0, 0)
+    st.build(g1, host_copy=True)
+    dplus.set_fp64_builder(True)
+    try:
+        g2 = dplus.Grid(gs, 5.0, 0)
+        st.build(g2, host_copy=True)
+    finally:
+        dplus.set_fp64_builder(False)
+    shape = st.build_shape(g1)
+    err, err_i = relmax(g1.data, g2.data), relmax(g1.interp, g2.interp)
+    say(f"    {len(ld)} layer and {len(ed)} excluded-volume boxes, shape {shape}; "
+        f"nodes {err:.2e}, splines {err_i:.2e}")
+    assert err < 1e-13 and err_i < 1e-12
+
+
 def test_splines_of_uploaded_data(dplus, oracle):
     say("[case] splines of uploaded random data")
     gs = 40
