@@ -335,6 +335,8 @@ def main():
                          "VEGAS curve, one MC average of 262,144 directions and the gather probe on that grid, then "
                          "the 1,000-copy helix docked MC at 32,768 directions and its gather probe: for rocprofv3")
     ap.add_argument("--no-docking", action="store_true", help="skip the 1,000-copy helix docking curve")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving stream's mixed legs (saturated and at 70 %% load): for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (the path shards by shell "
@@ -348,6 +350,9 @@ def main():
     log("[bench] device", info)
     if not info.get("ok"):
         raise SystemExit("no usable GPU")
+    if args.serving_only:
+        print(json.dumps({"serving": serving_stream(D, S, n_requests=args.serving_requests, fp64=False)}))
+        return
     W = make_workload(D, S)
     st = resident(D, W)
     g32 = D.Grid(W["gs"], W["qmax"], D.PRECISION_MIXED)

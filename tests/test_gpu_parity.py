@@ -125,6 +125,37 @@ def test_splines_of_uploaded_data(dplus, oracle):
     assert relmax(g.interp, oracle.grid_splines(gs, d)) < 1e-13
 
 
+@pytest.mark.parametrize("gs", [12, 250, 350])
+def test_splines_wave_rows_match_serial_sweeps(dplus, gs, monkeypatch):
+    """The one-wave-per-row spline kernel (affine-map scans of the forward and
+    backward sweeps across a row's lanes) against the serial one-thread-per-row
+    sweeps on random data: FP64 to 1e-13 of the largest coefficient, mixed to
+    1e-6 (the serial form rounds its forward values to FP32 between the sweeps).
+    Grid 350 has shells beyond the wave kernel's 1,024-node rows, which keep the
+    serial sweeps."""
+    say(f"[case] spline rows by waves vs serial sweeps, grid {gs}")
+    rng = np.random.default_rng(gs)
+    for prec, tol in ((0, 1e-13), (1, 1e-6)):
+        out = []
+        for serial in (False, True):
+            if serial:
+                monkeypatch.setenv("DPLUS_SPLINES_SERIAL", "1")
+            else:
+                monkeypatch.delenv("DPLUS_SPLINES_SERIAL", raising=False)
+            g = dplus.Grid(gs, 5.0, prec)
+            if not out:
+                d = rng.normal(size=2 * g.points)
+            g.data[:] = d
+            g.upload()
+            g.download()
+            out.append(g.interp.copy())
+            del g
+        monkeypatch.delenv("DPLUS_SPLINES_SERIAL", raising=False)
+        err = relmax(out[0], out[1])
+        say(f"    precision {prec}: max |wave - serial| / max {err:.2e}")
+        assert err < tol
+
+
 def test_get_sphr_matches_oracle(dplus, oracle, small):
     say("[case] get_sphr")
     s = small
