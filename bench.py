@@ -329,7 +329,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-stream and fit-session measurements")
     ap.add_argument("--serving-requests", type=int, default=512)
-    ap.add_argument("--session-evals", type=int, default=4)
+    ap.add_argument("--session-evals", type=int, default=300)
     ap.add_argument("--pmc-probe", "--probe", dest="pmc_probe", action="store_true",
                     help="one mixed build of the 50k-atom grid, a 256 MiB device copy (counter calibration), one "
                          "VEGAS curve, one MC average of 262,144 directions and the gather probe on that grid, then "

@@ -122,6 +122,19 @@ struct LeafGeo {
     double step;
 };
 
+// Gauss-Kronrod phi recursion on the device: a node of the phi trees of one
+// batch of theta tasks (roots first, 15 per theta task) and an open phi interval
+struct GkNode {
+    double value;
+    int parent;  // -1 for a root
+    int child0, child1;
+    int pending;  // children not yet resolved
+};
+struct GkInterval {
+    double pmin, pmax;
+    int node, maxd, mind, pad;
+};
+
 struct Grid {
     int grid_size = 0, N = 0, precision = DPLUS_PRECISION_MIXED;
     double qmax = 0, step = 0;
@@ -304,8 +317,16 @@ void azimuthal_average(const DockSet &D, int npts, const double *q, const double
 // Gauss-Kronrod levels are evaluated on the device in batches of at most this
 // many phi intervals (32 B task + 16 B sums each)
 constexpr size_t kGkBatch = (size_t)1 << 20;
-// upper bound of the device memory of one Gauss-Kronrod average over nq q values
-inline size_t gk_device_bytes(size_t nq) { return kGkBatch * 48 + nq * 32 + 4096; }
+// device phi recursion: at most kGkBatch open intervals per level and
+// kGkNodes tree nodes per theta level (beyond them the average runs its
+// recursion on the host, with the same results)
+constexpr size_t kGkNodes = (size_t)1 << 23;
+// upper bound of the device memory of one Gauss-Kronrod average over nq q values:
+// two interval levels (task + record), the Kronrod/Gauss sums, the node arena,
+// the phi integrals of a theta level and the counters
+inline size_t gk_device_bytes(size_t nq) {
+    return kGkBatch * (2 * (32 + sizeof(GkInterval)) + 16 + 8) + kGkNodes * sizeof(GkNode) + nq * 32 + 8192;
+}
 // space-filling lattice of a docked grid: basis vectors v[3k..3k+2] (nm) and
 // repetitions n[k]; on = 0 for none
 struct LatticeDev {

@@ -15,7 +15,9 @@
 // result does not depend on the thread count.
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <condition_variable>
 #include <functional>
 #include <memory>
@@ -33,6 +35,10 @@ void launch_docked_cart(bool fp64, const void *const *grids, const LeafGeo *geo,
 void launch_docked_gk(bool fp64, const void *const *grids, const LeafGeo *geo, const RotDev *rots, int nrot,
                       const double4 *trans, long long ntasks, const double4 *tasks, const GkNodes *nodes,
                       double *out, hipStream_t s);
+void launch_gk_decide(long long n, const GkInterval *cur, const double *kvgv, double eps, GkNode *nodes, int *cnt,
+                      int cap_int, int cap_nodes, const double4 *task, double4 *next_task, GkInterval *next,
+                      hipStream_t s);
+void launch_gk_tree_sums(GkNode *nodes, int lo, int hi, int nroots, double *phival, hipStream_t s);
 
 // Gauss-Kronrod 15/7 tables (Common/mathfuncs.h:100-125)
 static const double kX[15] = {
@@ -204,8 +210,205 @@ class Pool {
 };
 }  // namespace
 
+// q sin(theta_k), q cos(theta_k) of the 15 Kronrod nodes of a theta interval
+static void theta_nodes(ThetaTask &t, double qv) {
+    const double ba2 = (t.tmax - t.tmin) * 0.5, apb2 = (t.tmax + t.tmin) * 0.5;
+    for (int k = 0; k < 15; k++) {
+        const double th = apb2 + ba2 * kX[k];
+        t.qs[k] = qv * std::sin(th);
+        t.qc[k] = qv * std::cos(th);
+    }
+}
+
+// Kronrod and Gauss sums of a theta interval over its 15 phi integrals (GaussLegTheta, Amplitude.cpp:1424-1432)
+static void theta_sums(const ThetaTask &T, double &kv, double &gv) {
+    const double ba2 = (T.tmax - T.tmin) * 0.5, apb2 = (T.tmax + T.tmin) * 0.5;
+    kv = 0.0;
+    gv = 0.0;
+    for (int i = 0; i < 15; i++) {
+        const double sT = std::sin(apb2 + ba2 * kX[i]);
+        kv += T.phival[i] * kW[i] * ba2 * sT;
+        if (i % 2 == 1) gv += T.phival[i] * gW[i / 2] * ba2 * sT;
+    }
+}
+
+// |A(0)|^2 of the q = 0 entries (Amplitude.cpp:1479-1484); returns the evaluations
+static long long gk_zero_q(const DockSet &D, int nq, const double *q, double *out, hipStream_t s) {
+    std::vector<int> zero_q;
+    for (int i = 0; i < nq; i++)
+        if (q[i] == 0.0) zero_q.push_back(i);
+    if (zero_q.empty()) return 0;
+    std::vector<double> pts(3 * zero_q.size(), 0.0), vals(zero_q.size());
+    DevBuf bq, bo;
+    bq.upload(pts.data(), pts.size() * sizeof(double), s);
+    bo.ensure(vals.size() * sizeof(double));
+    KTimer kt("gk", s);
+    launch_docked_cart(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p, (const RotDev *)D.rotb.p,
+                       D.nrot, (const double4 *)D.trb.p, (int)vals.size(), (const double *)bq.p, (double *)bo.p, s);
+    check_hip(hipGetLastError(), "gk launch");
+    check_hip(hipMemcpyAsync(vals.data(), bo.p, vals.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    check_hip(hipStreamSynchronize(s), "gk batch");
+    for (size_t k = 0; k < zero_q.size(); k++) out[zero_q[k]] = vals[k];
+    return (long long)vals.size();
+}
+
+// The recursion with every phi interval on the device: the host keeps the theta
+// intervals (their nodes' sines and cosines, decisions and tree); each theta
+// level's 15 phi trees per interval are grown level by level on the device
+// (docked_gk_batch evaluates, gk_decide accepts and splits; gk_tree_sums adds
+// each tree's values level by level from the leaves), and their integrals come
+// back once per theta level.  Every sum is the host replay's:
+// the same phi intervals, Kronrod/Gauss sums, decisions and left + right tree
+// sums, so results and evaluation counts are equal.  Returns false when a level
+// outgrows the device arenas (the caller then runs the host replay).
+static bool gk_average_device(const DockSet &D, int nq, const double *q, double eps, long long max_depth,
+                              double *out, long long &evals, hipStream_t s) {
+    const int cap_int = (int)kGkBatch, cap_nodes = (int)kGkNodes;
+    std::vector<Tree> thetaTrees(nq);
+    std::vector<ThetaTask> level;
+    for (int i = 0; i < nq; i++) {
+        if (q[i] == 0.0) continue;
+        ThetaTask t;
+        t.q = i;
+        t.tmin = 0.0;
+        t.tmax = kPi / 2.0;
+        t.maxd = max_depth - 1;
+        t.mind = -1;
+        t.node = thetaTrees[i].add(-1);
+        theta_nodes(t, q[i]);
+        level.push_back(t);
+    }
+    if (level.empty()) return true;
+    GkNodes nodes;
+    for (int i = 0; i < 15; i++) {
+        nodes.x[i] = kX[i];
+        nodes.kw[i] = kW[i];
+    }
+    for (int i = 0; i < 7; i++) nodes.gw[i] = gW[i];
+    DevBuf bnodes, btask[2], biv[2], bkv, barena, bphi, bcnt;
+    bnodes.upload(&nodes, sizeof nodes, s);
+    for (int b = 0; b < 2; b++) {
+        btask[b].ensure((size_t)cap_int * sizeof(double4));
+        biv[b].ensure((size_t)cap_int * sizeof(GkInterval));
+    }
+    bkv.ensure((size_t)cap_int * 2 * sizeof(double));
+    barena.ensure((size_t)cap_nodes * sizeof(GkNode));
+    bphi.ensure((size_t)cap_int * sizeof(double));
+    bcnt.ensure(4 * sizeof(int));
+    // pinned staging, kept by the thread across averages (an engine worker runs many)
+    static thread_local Pinned<int> hcnt;
+    static thread_local Pinned<double> hphi;
+    static thread_local Pinned<double4> hroot_t;
+    static thread_local Pinned<GkInterval> hroot_i;
+    static thread_local Pinned<GkNode> hroot_n;
+    int *cnt = hcnt.get(4);
+    while (!level.empty()) {
+        const size_t T = level.size(), nroots = 15 * T;
+        if (nroots > (size_t)cap_int) return false;
+        double4 *rt = hroot_t.get(nroots);
+        GkInterval *ri = hroot_i.get(nroots);
+        GkNode *rn = hroot_n.get(nroots);
+        for (size_t t = 0; t < T; t++)
+            for (int k = 0; k < 15; k++) {
+                const size_t r = 15 * t + k;
+                const double pmin = 0.0, pmax = k2Pi;
+                rt[r] = make_double4(level[t].qs[k], level[t].qc[k], (pmax + pmin) * 0.5, (pmax - pmin) * 0.5);
+                ri[r] = GkInterval{pmin, pmax, (int)r, (int)level[t].maxd, (int)level[t].mind, 0};
+                rn[r] = GkNode{0.0, -1, -1, -1, 0};
+            }
+        cnt[0] = 0;
+        cnt[1] = (int)nroots;
+        cnt[2] = 0;
+        int cur = 0;
+        check_hip(hipMemcpyAsync(btask[0].p, rt, nroots * sizeof(double4), hipMemcpyHostToDevice, s), "H2D");
+        check_hip(hipMemcpyAsync(biv[0].p, ri, nroots * sizeof(GkInterval), hipMemcpyHostToDevice, s), "H2D");
+        check_hip(hipMemcpyAsync(barena.p, rn, nroots * sizeof(GkNode), hipMemcpyHostToDevice, s), "H2D");
+        check_hip(hipMemcpyAsync(bcnt.p, cnt, 3 * sizeof(int), hipMemcpyHostToDevice, s), "H2D");
+        long long n = (long long)nroots;
+        std::vector<int> bounds(1, 0);  // node ranges of the levels: roots first
+        bounds.push_back((int)nroots);
+        while (n > 0) {
+            {
+                KTimer kt("gk", s);
+                launch_docked_gk(D.fp64, (const void *const *)D.grids.p, (const LeafGeo *)D.geos.p,
+                                 (const RotDev *)D.rotb.p, D.nrot, (const double4 *)D.trb.p, n,
+                                 (const double4 *)btask[cur].p, (const GkNodes *)bnodes.p, (double *)bkv.p, s);
+                launch_gk_decide(n, (const GkInterval *)biv[cur].p, (const double *)bkv.p, eps, (GkNode *)barena.p,
+                                 (int *)bcnt.p, cap_int, cap_nodes, (const double4 *)btask[cur].p,
+                                 (double4 *)btask[1 - cur].p, (GkInterval *)biv[1 - cur].p, s);
+                check_hip(hipGetLastError(), "gk launch");
+            }
+            check_hip(hipMemcpyAsync(cnt, bcnt.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s), "D2H");
+            check_hip(hipStreamSynchronize(s), "gk level");
+            evals += 15 * n;
+            if (cnt[2]) return false;
+            n = cnt[0];
+            bounds.push_back(cnt[1]);
+            cnt[0] = 0;
+            check_hip(hipMemcpyAsync(bcnt.p, cnt, sizeof(int), hipMemcpyHostToDevice, s), "H2D");
+            cur = 1 - cur;
+        }
+        // tree sums from the deepest level to the roots
+        for (size_t b = bounds.size() - 1; b-- > 0;)
+            launch_gk_tree_sums((GkNode *)barena.p, bounds[b], bounds[b + 1], (int)nroots, (double *)bphi.p, s);
+        check_hip(hipGetLastError(), "gk tree sums");
+        double *phi = hphi.get(nroots);
+        check_hip(hipMemcpyAsync(phi, bphi.p, nroots * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        check_hip(hipStreamSynchronize(s), "gk phi integrals");
+        // theta decisions of this level, in order
+        std::vector<ThetaTask> next;
+        for (size_t t = 0; t < T; t++) {
+            ThetaTask &Tk = level[t];
+            for (int k = 0; k < 15; k++) Tk.phival[k] = phi[15 * t + k];
+            double kv, gv;
+            theta_sums(Tk, kv, gv);
+            Tree &tr = thetaTrees[Tk.q];
+            if (Tk.mind < 0 && (Tk.maxd < 0 || std::fabs(1.0 - (gv / kv)) <= eps || gv == kv)) {
+                const int r = tr.resolve(Tk.node, kv);
+                if (r >= 0) out[Tk.q] = tr.n[r].value / (2.0 * kPi);
+            } else {
+                const double mid = (Tk.tmax + Tk.tmin) / 2.0;
+                const int l = tr.add(Tk.node), rgt = tr.add(Tk.node);
+                tr.n[Tk.node].child[0] = l;
+                tr.n[Tk.node].child[1] = rgt;
+                tr.n[Tk.node].pending = 2;
+                for (int h = 0; h < 2; h++) {
+                    ThetaTask c;
+                    c.q = Tk.q;
+                    c.tmin = h == 0 ? Tk.tmin : mid;
+                    c.tmax = h == 0 ? mid : Tk.tmax;
+                    c.maxd = Tk.maxd - 1;
+                    c.mind = Tk.mind - 1;
+                    c.node = h == 0 ? l : rgt;
+                    theta_nodes(c, q[Tk.q]);
+                    next.push_back(c);
+                }
+            }
+        }
+        level.swap(next);
+    }
+    return true;
+}
+
+// the host replay (below) serves when the device arenas would overflow, when a
+// depth exceeds the device records' int range, or on request (DPLUS_GK_HOST)
+static void gk_average_host(const DockSet &D, int nq, const double *q, double eps, long long max_depth,
+                            double *out, dplus_oa_stats *st, hipStream_t s);
+
 void gk_average(const DockSet &D, int nq, const double *q, double eps, long long max_depth, double *out,
                 dplus_oa_stats *st, hipStream_t s) {
+    if (getenv("DPLUS_GK_HOST") == nullptr && max_depth <= (long long)INT_MAX) {
+        long long evals = gk_zero_q(D, nq, q, out, s);
+        if (gk_average_device(D, nq, q, eps, max_depth, out, evals, s)) {
+            if (st) st->evaluations = evals;
+            return;
+        }
+    }
+    gk_average_host(D, nq, q, eps, max_depth, out, st, s);
+}
+
+static void gk_average_host(const DockSet &D, int nq, const double *q, double eps, long long max_depth,
+                            double *out, dplus_oa_stats *st, hipStream_t s) {
     std::vector<Tree> thetaTrees(nq);
     std::vector<ThetaTask> tt;
     std::vector<Tree> phiTrees;

@@ -929,6 +929,85 @@ __global__ __launch_bounds__(15 * kGkTasksPerBlock) void docked_gk_batch(
     }
 }
 
+// Gauss-Kronrod phi recursion, one level per launch: each open interval is
+// accepted (its node takes its Kronrod sum) or split in two at its midpoint
+// (two new nodes and intervals at slots taken from counters, one atomic per
+// wave: cnt[0] next-level intervals, cnt[1] nodes, cnt[2] set on overflow).
+// Nodes are numbered level by level, so after the last level gk_tree_sums
+// walks the node ranges back to the roots: an inner node takes left + right,
+// and the roots' values are the phi integrals.  The accept rule and the
+// midpoint are GaussLegPhi's (Amplitude.cpp:1449-1467).
+__global__ __launch_bounds__(256) void gk_decide(long long n, const GkInterval *__restrict__ cur,
+                                                  const double *__restrict__ kvgv, double eps, GkNode *nodes,
+                                                  int *cnt, int cap_int, int cap_nodes, const double4 *__restrict__ task,
+                                                  double4 *__restrict__ next_task, GkInterval *__restrict__ next) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = t < n;
+    GkInterval I{0.0, 0.0, 0, 0, 0, 0};
+    double kv = 0.0, gv = 0.0;
+    bool accept = false;
+    if (live) {
+        I = cur[t];
+        kv = kvgv[2 * t];
+        gv = kvgv[2 * t + 1];
+        accept = I.mind < 0 && (I.maxd < 0 || fabs(1.0 - (gv / kv)) <= eps || gv == kv);
+    }
+    const bool split = live && !accept;
+    const unsigned long long mask = __ballot(split);
+    const int lane = threadIdx.x & 63;
+    int c = 0, w = 0;
+    if (mask) {
+        const int leader = __ffsll((long long)mask) - 1, nsplit = __popcll(mask);
+        int bc = 0, bw = 0;
+        if (lane == leader) {
+            bc = atomicAdd(&cnt[1], 2 * nsplit);
+            bw = atomicAdd(&cnt[0], 2 * nsplit);
+        }
+        bc = __shfl(bc, leader, 64);
+        bw = __shfl(bw, leader, 64);
+        const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        c = bc + 2 * rank;
+        w = bw + 2 * rank;
+    }
+    if (live && accept) nodes[I.node].value = kv;
+    if (!split) return;
+    if (c + 2 > cap_nodes || w + 2 > cap_int) {
+        atomicExch(&cnt[2], 1);
+        return;
+    }
+    nodes[c] = GkNode{0.0, I.node, -1, -1, 0};
+    nodes[c + 1] = GkNode{0.0, I.node, -1, -1, 0};
+    nodes[I.node].child0 = c;
+    nodes[I.node].child1 = c + 1;
+    const double4 tk = task[t];
+    const double mid = (I.pmax + I.pmin) * 0.5;
+    next[w] = GkInterval{I.pmin, mid, c, I.maxd - 1, I.mind - 1, 0};
+    next[w + 1] = GkInterval{mid, I.pmax, c + 1, I.maxd - 1, I.mind - 1, 0};
+    next_task[w] = make_double4(tk.x, tk.y, (mid + I.pmin) * 0.5, (mid - I.pmin) * 0.5);
+    next_task[w + 1] = make_double4(tk.x, tk.y, (I.pmax + mid) * 0.5, (I.pmax - mid) * 0.5);
+}
+
+// nodes [lo, hi) of one level: a split node takes left + right (its children,
+// in the next range, are final); a root writes its value to phival
+__global__ __launch_bounds__(256) void gk_tree_sums(GkNode *nodes, int lo, int hi, int nroots, double *phival) {
+    const int v = lo + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= hi) return;
+    GkNode &N = nodes[v];
+    if (N.child0 >= 0) N.value = nodes[N.child0].value + nodes[N.child1].value;
+    if (v < nroots) phival[v] = N.value;
+}
+
+void launch_gk_decide(long long n, const GkInterval *cur, const double *kvgv, double eps, GkNode *nodes, int *cnt,
+                      int cap_int, int cap_nodes, const double4 *task, double4 *next_task, GkInterval *next,
+                      hipStream_t s) {
+    gk_decide<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n, cur, kvgv, eps, nodes, cnt, cap_int, cap_nodes, task,
+                                                           next_task, next);
+}
+
+void launch_gk_tree_sums(GkNode *nodes, int lo, int hi, int nroots, double *phival, hipStream_t s) {
+    if (hi > lo) gk_tree_sums<<<(unsigned)((hi - lo + 255) / 256), 256, 0, s>>>(nodes, lo, hi, nroots, phival);
+}
+
 // ------------------------------------------------------------------ launchers
 void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot,
                const double4 *trans, const GroupDev *groups, const int *active, int nactive,

@@ -254,6 +254,31 @@ def test_gk_matches_oracle(dplus, oracle, small):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_gk_device_recursion_matches_host_replay(dplus, small, prec, monkeypatch):
+    """The Gauss-Kronrod phi recursion grown on the device (accept / split and the
+    left + right tree sums in gk_decide) against the host replay of the same
+    recursion: identical intensities and evaluation counts, on a single grid and
+    on a docked 12-copy helix, with depth limits and a q = 0 entry."""
+    say(f"[case] GK device recursion vs host replay precision={prec}")
+    import dplus_mi355x.synth as S
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    rot, nt, tr = S.rows_to_docking(S.helix_rows(12, 3.0, 0.9, 27.7))
+    q = np.concatenate([[0.0], np.linspace(0.05, 4.5, 40)])
+    for dock in (dplus.Docking.single(g), dplus.Docking([(g, rot, nt, tr)])):
+        for eps, depth in ((1e-4, 1 << 20), (1e-6, 5)):
+            monkeypatch.delenv("DPLUS_GK_HOST", raising=False)
+            a, sa = dplus.orientation_average(dock, q, method=dplus.OA_GK, eps=eps, gk_max_depth=depth)
+            monkeypatch.setenv("DPLUS_GK_HOST", "1")
+            b, sb = dplus.orientation_average(dock, q, method=dplus.OA_GK, eps=eps, gk_max_depth=depth)
+            monkeypatch.delenv("DPLUS_GK_HOST", raising=False)
+            say(f"    eps {eps} depth {depth}: {sa.evaluations} evaluations, max |device - host| "
+                f"{np.max(np.abs(a - b)):.2e}")
+            assert sa.evaluations == sb.evaluations
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_hybrid_docking_mc_matches_oracle(dplus, oracle, small, prec):
     say(f"[case] hybrid docking MC precision={prec}")
     import dplus_mi355x.synth as S
