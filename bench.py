@@ -276,6 +276,7 @@ def fit_sessions(D, S, n_sessions=16, evals=4, directions=8192, nq=600, qmax=5.0
         f"(copies {[p['copies'] for p in plans]})")
     lat = []
     lock = threading.Lock()
+    last = [time.perf_counter()]
 
     def run(sess_tuple, idx):
         p, grids, sess = sess_tuple
@@ -284,9 +285,12 @@ def fit_sessions(D, S, n_sessions=16, evals=4, directions=8192, nq=600, qmax=5.0
             pl = placements(p, float(r.uniform(0.9, 1.1)), float(r.uniform(-5.0, 5.0)))
             t = time.perf_counter()
             sess.evaluate(pl, fit_scale=1.0, fit_constant=0.0)
-            dt = (time.perf_counter() - t) * 1e3
+            now = time.perf_counter()
             with lock:
-                lat.append(dt)
+                lat.append((now - t) * 1e3)
+                if now - last[0] > 20.0:
+                    last[0] = now
+                    log(f"[sessions] {len(lat)} of {n_sessions * evals} evaluations")
 
     threads = [threading.Thread(target=run, args=(st, i)) for i, st in enumerate(sessions)]
     start = time.perf_counter()
