@@ -339,6 +339,8 @@ def main():
                          "VEGAS curve, one MC average of 262,144 directions and the gather probe on that grid, then "
                          "the 1,000-copy helix docked MC at 32,768 directions and its gather probe: for rocprofv3")
     ap.add_argument("--no-docking", action="store_true", help="skip the 1,000-copy helix docking curve")
+    ap.add_argument("--gk-only", action="store_true",
+                    help="the 50k-atom grid built once and averaged by Gauss-Kronrod three times: for rocprofv3")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving stream's mixed legs (saturated and at 70 %% load): for rocprofv3")
     args = ap.parse_args()
@@ -360,6 +362,15 @@ def main():
     W = make_workload(D, S)
     st = resident(D, W)
     g32 = D.Grid(W["gs"], W["qmax"], D.PRECISION_MIXED)
+    if args.gk_only:
+        st.build(g32)
+        runs = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            _, stg = D.orientation_average(D.Docking.single(g32), W["q"], method=D.OA_GK, eps=1e-4, gk_max_depth=8)
+            runs.append(time.perf_counter() - t0)
+        print(json.dumps({"gauss_kronrod_s": runs, "evaluations": stg.evaluations}))
+        return
     if args.pmc_probe:
         st.build(g32)
         nbytes = 256 * 2 ** 20
