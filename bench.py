@@ -167,8 +167,38 @@ def docking_curve(D, S, directions=2_000_000):
                          "ceiling": "dplus_gather_probe: the same lookups' node loads alone, 1,000 rotations"}}
 
 
+def plain_curve(D, S, steps=5):
+    """The 2,000-atom workload: a seeded walk in a 3 nm sphere (C/N/O/S), dummy-atom
+    solvent at 333 e/nm^3, grid 100, qmax 5, 500 q, MC over 10^6 directions (123
+    chunks of 8,192) with a fixed seed.  A curve is build + splines + MC from the
+    resident structure, in mixed precision and in FP64; median of `steps` curves."""
+    xyz, el = S.walk_structure(1, 2000, (3.0, 3.0, 3.0))
+    coef9, rad, ion, _ = S.structure_tables(el)
+    st = D.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=D.CALC_ATOMIC_FORMFACTORS | D.CALC_DUMMY_SOLVENT)
+    q = np.linspace(0.0, 5.0, 500)
+    out = {}
+    for name, prec in (("mixed", D.PRECISION_MIXED), ("fp64", D.PRECISION_FP64)):
+        g = D.Grid(100, 5.0, prec)
+        ts = []
+        for i in range(steps + 1):
+            t0 = time.perf_counter()
+            st.build(g)
+            I, stt = D.orientation_average(D.Docking.single(g), q, seed=1, max_dirs=10 ** 6, chunk=8192)
+            if i:
+                ts.append(time.perf_counter() - t0)
+        out[name] = (I, float(np.median(ts)), stt.dirs_used)
+    err = float(np.max(np.abs(out["mixed"][0] - out["fp64"][0]) / np.abs(out["fp64"][0])))
+    log(f"[plain] 2,000 atoms, grid 100, 500 q, MC {out['mixed'][2]} directions: mixed {out['mixed'][1] * 1e3:.1f} ms, "
+        f"FP64 {out['fp64'][1] * 1e3:.1f} ms per curve; max rel err {err:.2e}")
+    return {"workload": "2,000-atom SAW in a 3 nm sphere, solvent 333 e/nm^3, grid 100, qmax 5, 500 q, "
+                        "MC 10^6 directions (123 x 8,192), fixed seed",
+            "value": 1.0 / out["mixed"][1], "unit": "curves/s", "fp64_value": 1.0 / out["fp64"][1],
+            "speedup_vs_fp64": out["fp64"][1] / out["mixed"][1], "max_rel_err_vs_fp64": err,
+            "directions": int(out["mixed"][2]), "steps": steps}
+
+
 def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, streams=16, fp64=True,
-                   batch_builds=False):
+                   batch_builds=False, graphs=False, compare_modes=True, repeats=3):
     """Batched serving stream: n_requests with Poisson arrivals at `rate`/s; each
     request is its own seeded random-walk structure (or one of `pool_size`; atoms
     log-uniform in 500..200,000 at 50 atoms/nm^3, dummy-atom solvent), qmax
@@ -180,15 +210,17 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, st
     requests arriving at 70 % of that throughput (latency without a standing
     queue); the FP64 path then runs the same requests one at a time
     (fp64=False skips it).  batch_builds: small mixed builds share grouped
-    builder launches."""
+    builder launches.  graphs: each request's kernel chain runs as one captured
+    HIP graph; compare_modes also serves the saturated stream the other way
+    (eager launches vs graphs) and checks that every curve is bit-identical."""
     from dplus_mi355x.serving import Engine
     t0 = time.time()
     pool, reqs, arrivals = S.serving_workload(n_requests, pool_size, seed, rate)
     nstruct = n_requests if pool_size is None else pool_size
     log(f"[serving] {n_requests} requests over {nstruct} structures prepared in {time.time() - t0:.1f} s")
 
-    def run_mixed(arr):
-        eng = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=streams, batch_builds=batch_builds)
+    def run_mixed(arr, use_graphs=graphs):
+        eng = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=streams, batch_builds=batch_builds, graphs=use_graphs)
         tickets = []
         start = time.perf_counter()
         for (q, kw, _), at in zip(reqs, arr):
@@ -202,18 +234,46 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, st
         eng.close()
         return res, span, np.array([r["latency_ms"] for _, r in res]), st
 
-    mixed, span, lat, st = run_mixed(arrivals)
-    log(f"[serving] mixed: {n_requests / span:.2f} curves/s over {span:.2f} s, p50 {np.percentile(lat, 50):.1f} ms, "
-        f"p99 {np.percentile(lat, 99):.1f} ms, max concurrent {st['max_concurrent']}")
-    # the same requests arriving at 70 % of that throughput: latency without a standing queue
+    def summary(span_, lat_, st_):
+        return {"value": n_requests / span_, "p50_ms": float(np.percentile(lat_, 50)),
+                "p99_ms": float(np.percentile(lat_, 99)), "graph_launches": st_["graph_launches"],
+                "graph_instantiations": st_["graph_instantiations"], "graph_updates": st_["graph_updates"],
+                "graph_capture_ms": st_["graph_capture_ms"], "graph_exec_ms": st_["graph_exec_ms"]}
+
+    mode = "graphs" if graphs else "eager"
+    order = [mode] + ([("eager" if graphs else "graphs")] if compare_modes else [])
+    saturated, loaded_modes = {}, {}
+    for m in order:
+        # the saturated window is short (~0.5 s): the run of median span out of `repeats`
+        runs = sorted((run_mixed(arrivals, m == "graphs") for _ in range(repeats)), key=lambda x: x[1])
+        res_m, span_m, lat_m, st_m = runs[len(runs) // 2]
+        saturated[m] = summary(span_m, lat_m, st_m)
+        saturated[m]["values_all"] = [n_requests / x[1] for x in runs]
+        if not all(np.array_equal(a[0], b[0]) for x in runs for a, b in zip(res_m, x[0])):
+            saturated["repeat_mismatch"] = True
+        log(f"[serving] mixed ({m}): {n_requests / span_m:.2f} curves/s over {span_m:.2f} s, "
+            f"p50 {np.percentile(lat_m, 50):.1f} ms, p99 {np.percentile(lat_m, 99):.1f} ms, "
+            f"max concurrent {st_m['max_concurrent']}, graph launches {st_m['graph_launches']} "
+            f"(instantiated {st_m['graph_instantiations']}, updated {st_m['graph_updates']}; host ms capturing "
+            f"{st_m['graph_capture_ms']:.1f}, updating/instantiating {st_m['graph_exec_ms']:.1f})")
+        if m == mode:
+            mixed, span, lat, st = res_m, span_m, lat_m, st_m
+        else:
+            saturated["bit_identical"] = bool(all(np.array_equal(x[0], y[0]) for x, y in zip(mixed, res_m)))
+            log(f"[serving] curves bit-identical across modes: {saturated['bit_identical']}")
+    # the same requests arriving at 70 % of the primary mode's saturated throughput:
+    # latency without a standing queue, every mode at the same arrival rate
     rate70 = 0.7 * n_requests / span
-    _, span70, lat70, _ = run_mixed(arrivals * (rate / rate70))
-    loaded = {"rate": rate70, "value": n_requests / span70, "p50_ms": float(np.percentile(lat70, 50)),
-              "p99_ms": float(np.percentile(lat70, 99))}
-    log(f"[serving] mixed at {rate70:.0f}/s: {n_requests / span70:.2f} curves/s, p50 {loaded['p50_ms']:.1f} ms, "
-        f"p99 {loaded['p99_ms']:.1f} ms")
+    for m in order:
+        _, span70, lat70, _ = run_mixed(arrivals * (rate / rate70), m == "graphs")
+        loaded_modes[m] = {"rate": rate70, "value": n_requests / span70, "p50_ms": float(np.percentile(lat70, 50)),
+                           "p99_ms": float(np.percentile(lat70, 99))}
+        log(f"[serving] mixed ({m}) at {rate70:.0f}/s: {n_requests / span70:.2f} curves/s, "
+            f"p50 {loaded_modes[m]['p50_ms']:.1f} ms, p99 {loaded_modes[m]['p99_ms']:.1f} ms")
+    loaded = loaded_modes[mode]
     batching = {"batch_builds": bool(batch_builds), "batch_launches": st["batch_launches"],
-                "batched_builds": st["batched_builds"]}
+                "batched_builds": st["batched_builds"], "engine_mode": mode,
+                "modes": {"saturated": saturated, "at_70pct_load": loaded_modes}}
     if not fp64:
         return {"value": n_requests / span, "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99)), "streams": streams, "at_70pct_load": loaded, **batching}
@@ -529,6 +589,7 @@ def main():
         log(f"[bench] cpu baseline: {cpu}")
 
     docking = None if args.no_docking else docking_curve(D, S)
+    plain = plain_curve(D, S)
 
     serving = sessions = None
     if not args.no_serving:
@@ -583,6 +644,7 @@ def main():
         "oa_roofline": oa_roof,
         "cpu_baseline": cpu,
         "docking": docking,
+        "plain_2000_atoms": plain,
         "serving": serving,
         "fit_sessions": sessions,
     }

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <random>
 #include <tuple>
@@ -156,8 +157,8 @@ static thread_local hipStream_t t_alloc_stream = nullptr;
 hipStream_t alloc_stream() { return t_alloc_stream ? t_alloc_stream : Device::get().stream; }
 void set_alloc_stream(hipStream_t s) { t_alloc_stream = s; }
 
-std::shared_mutex &capture_mutex() {
-    static std::shared_mutex m;
+CaptureMutex &capture_mutex() {
+    static CaptureMutex m;
     return m;
 }
 
@@ -179,7 +180,7 @@ void DevBuf::release() {
         borrowed = false;
         return;
     }
-    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    std::shared_lock<CaptureMutex> lk(capture_mutex());
     free_locked(p, n, s);
 }
 void DevBuf::ensure(size_t bytes) {
@@ -196,7 +197,7 @@ void DevBuf::ensure(size_t bytes) {
         t_arena->used = off + bytes;
         return;
     }
-    std::shared_lock<std::shared_mutex> lk(capture_mutex());
+    std::shared_lock<CaptureMutex> lk(capture_mutex());
     if (borrowed) {
         p = nullptr;
         n = 0;
@@ -269,12 +270,15 @@ void Grid::ensure_host() {
 void launch_pack(bool fp64, const void *grid, long long n, double2 *data, double2 *coeffs, hipStream_t s);
 void launch_unpack(bool fp64, const double2 *data, long long n, void *grid, hipStream_t s);
 
-void Grid::splines(hipStream_t s) {
-    SplineTables *t = tables_for(N, s);
-    KTimer kt("splines", s);
+void Grid::splines(hipStream_t s, bool timed) {
+    const SplineTables *t = spline_tables ? (const SplineTables *)spline_tables : tables_for(N, s);
+    std::unique_ptr<KTimer> kt(timed ? new KTimer("splines", s) : nullptr);
     launch_splines(fp64(), dev.p, N, (const double *)t->tab.p, (const long long *)t->off.p, s);
     check_hip(hipGetLastError(), "splines launch");
 }
+
+// the tables live for the process, so the grid keeps their address
+void Grid::prepare_splines(hipStream_t s) { spline_tables = tables_for(N, s); }
 
 void Grid::download(hipStream_t s) {
     ensure_host();
@@ -543,9 +547,14 @@ BuildJob Structure::prepare(Grid &g, int slices, Pending &P, hipStream_t s) cons
     return J;
 }
 
-void Structure::finish(Grid &g, const Pending &P, bool host_copy, hipStream_t s) const {
+void Structure::enqueue_finish(Grid &g, const Pending &P, hipStream_t s, bool timed) const {
     launch_set_origin(g.fp64(), g.dev.p, P.origin * desc.scale, s);
-    g.splines(s);
+    check_hip(hipGetLastError(), "set_origin launch");
+    g.splines(s, timed);
+}
+
+void Structure::finish(Grid &g, const Pending &P, bool host_copy, hipStream_t s) const {
+    enqueue_finish(g, P, s, true);
     // the per-build buffers are released after the stream has drained
     if (host_copy)
         g.download(s);
@@ -865,28 +874,125 @@ void launch_mc_finish(const GroupDev *groups, int ngroups, int mstride, const do
 void launch_chi2(const double *I, const double *y, const double *sig, int n, double scale, double constant,
                  double *chi2, hipStream_t s);
 
-// chi^2 of the device intensities dI against host data, on the device
-static void chi2_on_device(const double *dI, int nq, Chi2Request *chi, hipStream_t s) {
-    DevBuf by, bs, bc;
-    by.upload(chi->y, nq * sizeof(double), s);
-    bs.upload(chi->sigma, nq * sizeof(double), s);
-    bc.ensure(sizeof(double));
-    launch_chi2(dI, (const double *)by.p, (const double *)bs.p, nq, chi->scale, chi->constant, (double *)bc.p, s);
-    check_hip(hipGetLastError(), "chi2 launch");
-    check_hip(hipMemcpyAsync(&chi->chi2, bc.p, sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-    chi->on_device = true;
+void check_q_range(const DockSet &D, int nq, const double *q) {
+    for (int i = 1; i < nq; i++)
+        if (q[i] < q[i - 1]) throw Error(kErrInvalidArgs, "q must be ascending");
+    if (nq <= 0) return;
+    if (!(q[0] >= 0)) throw Error(kErrInvalidArgs, "q must be non-negative");
+    if (q[nq - 1] > D.qmax * (1.0 + 1e-12) + 0.0) {
+        // the reference integrates up to the grid's qmax; beyond it the extra shells are extrapolation
+        if (q[nq - 1] > (D.grid_size / 2 + 1) * D.step) throw Error(kErrInvalidArgs, "q beyond the grid");
+    }
+}
+
+void FixedAverage::prepare(const DockSet &D, int nq_, const double *q, const dplus_oa_params *p,
+                           const Chi2Request *c, hipStream_t s) {
+    if (!(p->eps <= 0.0) || (p->method != DPLUS_OA_MC && p->method != DPLUS_OA_VEGAS))
+        throw Error(kErrInvalidArgs, "a fixed-count average is Monte Carlo or VEGAS with eps <= 0");
+    if (nq_ <= 0) throw Error(kErrInvalidArgs, "a fixed-count average needs q values");
+    check_q_range(D, nq_, q);
+    nq = nq_;
+    method = p->method;
+    seed = p->seed;
+    const std::vector<GroupDev> groups = make_groups(q, nq, D.step);
+    ng = (int)groups.size();
+    mstride = 1;
+    for (auto &g : groups) mstride = std::max(mstride, g.m);
+    std::vector<int> act(ng);
+    for (int i = 0; i < ng; i++) act[i] = i;
+    bq.upload(q, nq * sizeof(double), s);
+    bg.upload(groups.data(), ng * sizeof(GroupDev), s);
+    bact.upload(act.data(), ng * sizeof(int), s);
+    if (method == DPLUS_OA_MC) {
+        chunk = p->chunk > 0 ? p->chunk : 8192;
+        if (chunk % 256) throw Error(kErrInvalidArgs, "MC chunk must be a multiple of 256");
+        const long long nchunks = (std::max<long long>(1, p->max_dirs) + chunk - 1) / chunk;
+        // one launch covers every direction of every group
+        total = nchunks * chunk;
+        if (total / 256 > 2147483647LL) throw Error(kErrInvalidArgs, "too many directions");
+        nblk = (int)(total / 256);
+        bpart.ensure((size_t)ng * nblk * mstride * sizeof(double));
+        bout.ensure((size_t)ng * mstride * sizeof(double));
+        bI.ensure((size_t)nq * sizeof(double));
+    } else {
+        iters = std::max(1, p->vegas_iterations);
+        ppb = std::max(1, p->vegas_points_per_bin);
+        if (ppb > 256) throw Error(kErrInvalidArgs, "VEGAS points per bin must be <= 256");
+        std::vector<double> bounds((size_t)ng * 66);
+        for (int g = 0; g < ng; g++)
+            for (int i = 0; i <= 32; i++) {
+                bounds[(size_t)g * 66 + i] = (double)i / 32;
+                bounds[(size_t)g * 66 + 33 + i] = (double)i / 32;
+            }
+        bb.upload(bounds.data(), bounds.size() * sizeof(double), s);
+        bsum.ensure((size_t)ng * mstride * 1024 * sizeof(double));
+        bvar.ensure((size_t)ng * 1024 * sizeof(double));
+        bh.ensure((size_t)iters * nq * sizeof(double));
+    }
+    chi = c != nullptr;
+    if (chi) {
+        by.upload(c->y, nq * sizeof(double), s);
+        bsig.upload(c->sigma, nq * sizeof(double), s);
+        bchi.ensure(sizeof(double));
+        chi_scale = c->scale;
+        chi_constant = c->constant;
+    }
+}
+
+void FixedAverage::enqueue(const DockSet &D, double *I, double *chi2, hipStream_t s, bool timed) const {
+    const double *dI;
+    if (method == DPLUS_OA_MC) {
+        {
+            std::unique_ptr<KTimer> kt(timed ? new KTimer("mc", s) : nullptr);
+            launch_mc(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                      (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
+                      (const double *)bq.p, D.step, seed, 0, nblk, chunk, (double *)bpart.p, (double *)bout.p, s);
+        }
+        check_hip(hipGetLastError(), "mc launch");
+        launch_mc_finish((const GroupDev *)bg.p, ng, mstride, (const double *)bout.p, (double)total, (double *)bI.p, s);
+        check_hip(hipGetLastError(), "mc finish launch");
+        dI = (const double *)bI.p;
+    } else {
+        launch_fill_zero((double *)bh.p, (size_t)iters * nq, s);
+        check_hip(hipGetLastError(), "fill launch");
+        for (int k = 0; k < iters; k++) {
+            std::unique_ptr<KTimer> kt(timed ? new KTimer("vegas", s) : nullptr);
+            launch_vegas_iter(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
+                              (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
+                              (const double *)bq.p, D.step, seed, k, ppb, (double *)bb.p, (double *)bsum.p,
+                              (double *)bvar.p, (double *)bh.p, nq, s);
+            check_hip(hipGetLastError(), "vegas launch");
+        }
+        // every group runs every iteration: the result is the last iteration's running mean
+        dI = (const double *)bh.p + (size_t)(iters - 1) * nq;
+    }
+    if (chi) {
+        launch_chi2(dI, (const double *)by.p, (const double *)bsig.p, nq, chi_scale, chi_constant, (double *)bchi.p, s);
+        check_hip(hipGetLastError(), "chi2 launch");
+        check_hip(hipMemcpyAsync(chi2, bchi.p, sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    }
+    check_hip(hipMemcpyAsync(I, dI, nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
 }
 
 void orientation_average(const DockSet &D, int nq, const double *q, const dplus_oa_params *p, double *out,
                          dplus_oa_stats *st, hipStream_t s, Chi2Request *chi) {
     if (chi) chi->on_device = false;
-    for (int i = 1; i < nq; i++)
-        if (q[i] < q[i - 1]) throw Error(kErrInvalidArgs, "q must be ascending");
+    check_q_range(D, nq, q);
     if (nq <= 0) return;
-    if (q[0] < 0) throw Error(kErrInvalidArgs, "q must be non-negative");
-    if (q[nq - 1] > D.qmax * (1.0 + 1e-12) + 0.0) {
-        // the reference integrates up to the grid's qmax; beyond it the extra shells are extrapolation
-        if (q[nq - 1] > (D.grid_size / 2 + 1) * D.step) throw Error(kErrInvalidArgs, "q beyond the grid");
+    if (p->eps <= 0.0 && (p->method == DPLUS_OA_MC || p->method == DPLUS_OA_VEGAS)) {
+        // fixed count: every group takes every direction / iteration, no host decisions
+        FixedAverage F;
+        F.prepare(D, nq, q, p, chi, s);
+        F.enqueue(D, out, chi ? &chi->chi2 : nullptr, s, true);
+        check_hip(hipStreamSynchronize(s), p->method == DPLUS_OA_MC ? "mc" : "vegas");
+        if (chi) chi->on_device = true;
+        if (st) {
+            if (p->method == DPLUS_OA_MC)
+                st->dirs_used = F.total;
+            else
+                st->iterations_used = F.iters;
+        }
+        return;
     }
     const std::vector<GroupDev> groups = make_groups(q, nq, D.step);
     const int ng = (int)groups.size();
@@ -905,70 +1011,44 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
         std::vector<double> hist((size_t)nchunks * nq, 0.0), sum(nq, 0.0);
         DevBuf bpart, bout;
         long long used = 0;
-        if (p->eps <= 0.0) {
-            // one launch covers every direction of every group
-            const long long total = nchunks * chunk;
-            const long long nblk_ll = total / 256;
-            if (nblk_ll > 2147483647LL) throw Error(kErrInvalidArgs, "too many directions");
-            const int nblk = (int)nblk_ll;
-            bpart.ensure((size_t)ng * nblk * mstride * sizeof(double));
-            bout.ensure((size_t)ng * mstride * sizeof(double));
+        // chunked with the reference's three-back convergence test (GPUHybridCalc.cu:705-720)
+        const int nblk = chunk / 256;
+        bpart.ensure((size_t)ng * nblk * mstride * sizeof(double));
+        bout.ensure((size_t)ng * mstride * sizeof(double));
+        std::vector<int> active = act;
+        std::vector<int> done_k(ng, -1);
+        std::vector<double> r((size_t)ng * mstride);
+        for (long long k = 0; k < nchunks && !active.empty(); k++) {
+            bact.upload(active.data(), active.size() * sizeof(int), s);
             {
                 KTimer kt("mc", s);
                 launch_mc(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
-                          (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
-                          (const double *)bq.p, D.step, p->seed, 0, nblk, chunk, (double *)bpart.p, (double *)bout.p,
-                          s);
+                          (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
+                          (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k * chunk, nblk,
+                          chunk, (double *)bpart.p, (double *)bout.p, s);
             }
             check_hip(hipGetLastError(), "mc launch");
-            DevBuf bI;
-            bI.ensure((size_t)nq * sizeof(double));
-            launch_mc_finish((const GroupDev *)bg.p, ng, mstride, (const double *)bout.p, (double)total, (double *)bI.p, s);
-            check_hip(hipGetLastError(), "mc finish launch");
-            if (chi) chi2_on_device((const double *)bI.p, nq, chi, s);
-            check_hip(hipMemcpyAsync(out, bI.p, nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-            check_hip(hipStreamSynchronize(s), "mc");
-            used = total;
-        } else {
-            // chunked with the reference's three-back convergence test (GPUHybridCalc.cu:705-720)
-            const int nblk = chunk / 256;
-            bpart.ensure((size_t)ng * nblk * mstride * sizeof(double));
-            bout.ensure((size_t)ng * mstride * sizeof(double));
-            std::vector<int> active = act;
-            std::vector<int> done_k(ng, -1);
-            std::vector<double> r((size_t)ng * mstride);
-            for (long long k = 0; k < nchunks && !active.empty(); k++) {
-                bact.upload(active.data(), active.size() * sizeof(int), s);
-                {
-                    KTimer kt("mc", s);
-                    launch_mc(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
-                              (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
-                              (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k * chunk, nblk,
-                              chunk, (double *)bpart.p, (double *)bout.p, s);
+            check_hip(hipMemcpyAsync(r.data(), bout.p, active.size() * mstride * sizeof(double),
+                                     hipMemcpyDeviceToHost, s), "D2H");
+            check_hip(hipStreamSynchronize(s), "mc chunk");
+            std::vector<int> next;
+            for (size_t ai = 0; ai < active.size(); ai++) {
+                const GroupDev &G = groups[active[ai]];
+                for (int j = 0; j < G.m; j++) {
+                    sum[G.q0 + j] += r[ai * mstride + j];
+                    hist[(size_t)k * nq + G.q0 + j] = sum[G.q0 + j] / ((double)(k + 1) * chunk);
                 }
-                check_hip(hipGetLastError(), "mc launch");
-                check_hip(hipMemcpyAsync(r.data(), bout.p, active.size() * mstride * sizeof(double),
-                                         hipMemcpyDeviceToHost, s), "D2H");
-                check_hip(hipStreamSynchronize(s), "mc chunk");
-                std::vector<int> next;
-                for (size_t ai = 0; ai < active.size(); ai++) {
-                    const GroupDev &G = groups[active[ai]];
-                    for (int j = 0; j < G.m; j++) {
-                        sum[G.q0 + j] += r[ai * mstride + j];
-                        hist[(size_t)k * nq + G.q0 + j] = sum[G.q0 + j] / ((double)(k + 1) * chunk);
-                    }
-                    if (converged3(hist, (int)k, G.q0, G.m, nq, p->eps) || k == nchunks - 1)
-                        done_k[active[ai]] = (int)k;
-                    else
-                        next.push_back(active[ai]);
-                }
-                active.swap(next);
+                if (converged3(hist, (int)k, G.q0, G.m, nq, p->eps) || k == nchunks - 1)
+                    done_k[active[ai]] = (int)k;
+                else
+                    next.push_back(active[ai]);
             }
-            for (int gi = 0; gi < ng; gi++) {
-                const int k = done_k[gi];
-                used = std::max<long long>(used, (long long)(k + 1) * chunk);
-                for (int j = 0; j < groups[gi].m; j++) out[groups[gi].q0 + j] = hist[(size_t)k * nq + groups[gi].q0 + j];
-            }
+            active.swap(next);
+        }
+        for (int gi = 0; gi < ng; gi++) {
+            const int k = done_k[gi];
+            used = std::max<long long>(used, (long long)(k + 1) * chunk);
+            for (int j = 0; j < groups[gi].m; j++) out[groups[gi].q0 + j] = hist[(size_t)k * nq + groups[gi].q0 + j];
         }
         if (st) st->dirs_used = used;
     } else if (p->method == DPLUS_OA_VEGAS) {
@@ -1002,21 +1082,18 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
             }
             check_hip(hipGetLastError(), "vegas launch");
             used = k + 1;
-            if (p->eps > 0.0) {
-                check_hip(hipMemcpyAsync(hist.data() + (size_t)k * nq, (double *)bh.p + (size_t)k * nq,
-                                         nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-                check_hip(hipStreamSynchronize(s), "vegas iter");
-                std::vector<int> next;
-                for (int g : active) {
-                    if (converged3(hist, k, groups[g].q0, groups[g].m, nq, p->eps))
-                        done_k[g] = k;
-                    else
-                        next.push_back(g);
-                }
-                active.swap(next);
+            check_hip(hipMemcpyAsync(hist.data() + (size_t)k * nq, (double *)bh.p + (size_t)k * nq,
+                                     nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+            check_hip(hipStreamSynchronize(s), "vegas iter");
+            std::vector<int> next;
+            for (int g : active) {
+                if (converged3(hist, k, groups[g].q0, groups[g].m, nq, p->eps))
+                    done_k[g] = k;
+                else
+                    next.push_back(g);
             }
+            active.swap(next);
         }
-        if (chi && p->eps <= 0.0) chi2_on_device((const double *)bh.p + (size_t)(iters - 1) * nq, nq, chi, s);
         check_hip(hipMemcpyAsync(hist.data(), bh.p, hist.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         check_hip(hipStreamSynchronize(s), "vegas");
         for (int g = 0; g < ng; g++)

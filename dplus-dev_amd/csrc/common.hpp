@@ -14,6 +14,8 @@ namespace dplus {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double k2Pi = 6.28318530717958647692528676656;
+// shells whose spline rows (6i nodes <= 1024) take the one-wave-per-row kernel
+constexpr int kWaveShellMax = 170;
 
 // ---------------------------------------------------------------------------
 // Grid geometry (Backend/Backend/Grid.cpp:169-186, 1078-1111).

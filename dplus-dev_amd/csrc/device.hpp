@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include <condition_variable>
 #include <mutex>
 #include <shared_mutex>
 
@@ -51,10 +52,48 @@ struct Device {
 hipStream_t alloc_stream();
 void set_alloc_stream(hipStream_t s);
 // Graph capture against allocation: a thread capturing a stream holds this
-// exclusively from begin to instantiation; stream-ordered and pinned
+// exclusively from begin to end of the capture; stream-ordered and pinned
 // allocations and frees take it shared, so none runs during another thread's
-// capture (HIP rejects them there).
-std::shared_mutex &capture_mutex();
+// capture (HIP rejects them there).  A waiting capture goes first: once a
+// thread asks for it exclusively, new shared holders wait, so a capture is
+// not held back by a steady stream of allocations on other threads.
+class CaptureMutex {
+   public:
+    void lock() {
+        std::unique_lock<std::mutex> lk(m_);
+        writers_waiting_++;
+        cv_.wait(lk, [&] { return !writer_ && readers_ == 0; });
+        writers_waiting_--;
+        writer_ = true;
+    }
+    void unlock() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            writer_ = false;
+        }
+        cv_.notify_all();
+    }
+    void lock_shared() {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return !writer_ && writers_waiting_ == 0; });
+        readers_++;
+    }
+    void unlock_shared() {
+        bool last;
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            last = --readers_ == 0;
+        }
+        if (last) cv_.notify_all();
+    }
+
+   private:
+    std::mutex m_;
+    std::condition_variable cv_;
+    int readers_ = 0, writers_waiting_ = 0;
+    bool writer_ = false;
+};
+CaptureMutex &capture_mutex();
 
 // owning device allocation that only grows (or, while the calling thread has
 // a scratch arena installed, a region of that arena)
@@ -145,7 +184,13 @@ struct Grid {
     bool fp64() const { return precision == DPLUS_PRECISION_FP64; }
     size_t node_bytes() const { return fp64() ? sizeof(double4) : sizeof(float4); }
     void ensure_host();
-    void splines(hipStream_t s);
+    // spline coefficients from the node values (timed: HIP-event timing, never
+    // while capturing); prepare_splines makes the multiplier tables resident
+    // and keeps their address, so a captured splines() takes no lock and
+    // uploads nothing
+    void splines(hipStream_t s, bool timed = true);
+    void prepare_splines(hipStream_t s);
+    const void *spline_tables = nullptr;
     void download(hipStream_t s);
     void upload(hipStream_t s);
 };
@@ -272,6 +317,9 @@ struct Structure {
     };
     BuildJob prepare(Grid &g, int slices, Pending &bufs, hipStream_t s) const;
     void finish(Grid &g, const Pending &bufs, bool host_copy, hipStream_t s) const;
+    // finish's kernels alone (origin, splines): no upload, allocation or
+    // synchronisation once g.prepare_splines ran, so they can be captured
+    void enqueue_finish(Grid &g, const Pending &bufs, hipStream_t s, bool timed) const;
     // launch shape of the paired builder for grid g: the tuned one when the
     // shape table holds it, else the occupancy model's
     BuildShape shape_for(const Grid &g, bool use_table = true) const;
@@ -308,6 +356,29 @@ struct Chi2Request {
 };
 void orientation_average(const DockSet &D, int nq, const double *q, const dplus_oa_params *p, double *out,
                          dplus_oa_stats *st, hipStream_t s, Chi2Request *chi = nullptr);
+// the q checks of an orientation average: ascending, non-negative, at most
+// grid_size/2 + 1 shells
+void check_q_range(const DockSet &D, int nq, const double *q);
+// A fixed-count average (Monte Carlo or VEGAS with eps <= 0) in two steps:
+// prepare allocates and uploads everything it needs on stream s; enqueue then
+// issues its kernels and the download of I(q) (and chi^2) with no allocation
+// and no host synchronisation, so a caller can capture enqueue into a HIP
+// graph.  orientation_average runs every fixed-count average through it, so a
+// captured average gives the bits of an eager one.
+struct FixedAverage {
+    int nq = 0, ng = 0, mstride = 1, method = 0, iters = 0, ppb = 0, chunk = 0, nblk = 0;
+    long long total = 0;
+    uint64_t seed = 0;
+    bool chi = false;
+    double chi_scale = 1.0, chi_constant = 0.0;
+    DevBuf bq, bg, bact, bpart, bout, bI, bb, bsum, bvar, bh, by, bsig, bchi;
+    void prepare(const DockSet &D, int nq, const double *q, const dplus_oa_params *p, const Chi2Request *c,
+                 hipStream_t s);
+    // I: nq doubles, chi2: one double (with a chi^2 request), written when the
+    // stream reaches the downloads (pinned host memory for a captured graph);
+    // timed: HIP-event kernel timing (never while capturing)
+    void enqueue(const DockSet &D, double *I, double *chi2, hipStream_t s, bool timed) const;
+};
 void gk_average(const DockSet &D, int nq, const double *q, double eps, long long max_depth, double *out,
                 dplus_oa_stats *st, hipStream_t s);
 // azimuthal averages of |A|^2 at npts (q, theta) points over nphi stratified phi

@@ -19,6 +19,19 @@
 // request's stream waits on the launch's event), so small requests fill the
 // device together instead of launching one small grid each.
 //
+// Fixed-count requests (Monte Carlo or VEGAS with eps <= 0) are prepared
+// first -- resident structure, grid, form factors, slice partials,
+// placements, q and scratch allocated and uploaded on the slot's stream --
+// and then their whole kernel chain (builder, slice sums, origin, splines,
+// orientation average, intensity, chi^2, download into pinned memory) is
+// launched with a single synchronisation at the end.  Captured requests
+// (optional): the chain is captured into one HIP graph and launched.  Each
+// slot keeps instantiated graphs keyed by the chain's shape (method,
+// iterations, precision, phase form, builder workgroup, chi^2); a new
+// request updates the kept graph of its key in place (hipGraphExecUpdate)
+// and instantiates only when the update is refused.  The kernels and their
+// arguments are those of the eager launches, so both give the same bits.
+//
 // Fit session.  Resident subunit grids plus fixed q, data and sigma; each
 // evaluation re-docks the subunits with new placements (symmetry parameters of
 // a fitting iteration), averages by fixed-count Monte Carlo and returns I(q)
@@ -26,6 +39,7 @@
 // (placement upload, MC kernels, intensity, chi^2, result download), replayed
 // while the number of rotations stays the same and re-captured when it changes.
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -110,6 +124,7 @@ struct Request {
     std::vector<double> I;
     double chi2 = 0.0;
     bool chi_on_device = false;
+    bool graph = false;  // ran as a captured graph
     dplus_oa_stats st{};
     int code = 0;
     std::string message;
@@ -196,7 +211,26 @@ struct Engine {
         int code = 0;
         std::string message;
     };
-    bool batch_builds = false, autotune = false;
+    bool batch_builds = false, autotune = false, graphs = false;
+    // captured requests: per slot, pinned result staging and the instantiated
+    // graphs kept by chain key (least recently used dropped past kKeptGraphs)
+    static constexpr int kKeptGraphs = 16;
+    struct KeptGraph {
+        std::string key;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long last = 0;
+    };
+    struct Slot {
+        double *h_out = nullptr;
+        size_t cap = 0;
+        std::vector<KeptGraph> kept;
+        unsigned long long tick = 0;
+    };
+    std::vector<Slot> slots;
+    std::atomic<long long> graph_launches{0}, graph_instantiations{0}, graph_updates{0};
+    // host time of captured requests (ns): waiting for the capture lock and
+    // capturing; updating or instantiating the executable graph
+    std::atomic<long long> graph_capture_ns{0}, graph_exec_ns{0};
     double batch_terms = 1e10;
     std::mutex bmu;
     std::condition_variable bcv;
@@ -207,13 +241,14 @@ struct Engine {
     long long batch_launches = 0, batch_jobs = 0;
     std::mutex tune_mu;  // one autotuning sweep at a time
 
-    Engine(long long budget_bytes, int nstreams, bool batch, long long bterms, bool tune) {
+    Engine(long long budget_bytes, int nstreams, bool batch, long long bterms, bool tune, bool graph) {
         Device &d = Device::get();
         d.require();
         budget = budget_bytes > 0 ? (size_t)budget_bytes : (size_t)(0.85 * (double)d.mem);
         cache_cap = std::min<size_t>(budget / 16, (size_t)8 << 30);
         batch_builds = batch;
         autotune = tune;
+        graphs = graph;
         if (bterms > 0) batch_terms = (double)bterms;
         nstreams = std::max(1, std::min(16, nstreams));
         for (int i = 0; i < nstreams; i++) {
@@ -225,6 +260,7 @@ struct Engine {
             check_hip(hipStreamCreate(&bstream), "hipStreamCreate");
             bthread = std::thread([this] { batcher(); });
         }
+        slots.resize(nstreams);
         for (int i = 0; i < nstreams; i++) workers.emplace_back([this, i] { work(i); });
     }
 
@@ -245,6 +281,14 @@ struct Engine {
             bthread.join();
         }
         cache.clear();  // frees are ordered on the worker streams: release before destroying them
+        for (size_t i = 0; i < slots.size(); i++) {
+            (void)hipStreamSynchronize(streams[i]);
+            for (auto &k : slots[i].kept) (void)hipGraphExecDestroy(k.exec);
+            if (slots[i].h_out) {
+                std::shared_lock<CaptureMutex> lk(capture_mutex());
+                (void)hipHostFree(slots[i].h_out);
+            }
+        }
         for (auto s : streams) {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
@@ -406,7 +450,15 @@ struct Engine {
             }
             r->slot = w;
             r->t_start = Clock::now();
-            run(*r, s);
+            // a fixed-count request runs its whole chain from prepared buffers (as one
+            // captured graph when asked); the rest -- Gauss-Kronrod, convergence-tested
+            // averages, builds for the grouped batcher -- take the general path
+            const bool fixed = r->oa.eps <= 0.0 && (r->oa.method == DPLUS_OA_MC || r->oa.method == DPLUS_OA_VEGAS);
+            const bool batched = batch_builds && r->has_pdb && r->precision != DPLUS_PRECISION_FP64;
+            if (fixed && !batched)
+                run_fixed(*r, s, slots[w], graphs);
+            else
+                run(*r, s);
             r->t_end = Clock::now();
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -524,6 +576,204 @@ struct Engine {
             r.message = e.what();
         }
     }
+
+    // results of a fixed-count request from the slot's staging
+    static void finish_fixed(Request &r, const FixedAverage &F, const Slot &sl, bool graph) {
+        const int nq = (int)r.q.size();
+        r.I.assign(sl.h_out, sl.h_out + nq);
+        r.chi2 = r.want_chi ? sl.h_out[nq] : 0.0;
+        r.chi_on_device = r.want_chi;
+        r.graph = graph;
+        if (r.oa.method == DPLUS_OA_MC)
+            r.st.dirs_used = F.total;
+        else
+            r.st.iterations_used = F.iters;
+        r.code = 0;
+    }
+
+    // pinned staging of a slot's results (nq intensities, then chi^2)
+    static void slot_staging(Slot &sl, size_t n) {
+        if (n <= sl.cap) return;
+        std::shared_lock<CaptureMutex> lk(capture_mutex());
+        if (sl.h_out) (void)hipHostFree(sl.h_out);
+        sl.h_out = nullptr;
+        sl.cap = 0;
+        const size_t cap = std::max<size_t>(n, 2048);
+        check_hip(hipHostMalloc((void **)&sl.h_out, cap * sizeof(double), hipHostMallocDefault), "hipHostMalloc");
+        sl.cap = cap;
+    }
+
+    // the slot's executable graph for `graph`: the kept graph of its key updated
+    // in place, else a new instantiation that replaces it (or the least recently
+    // used of the kept graphs)
+    hipGraphExec_t slot_exec(Slot &sl, const std::string &key, hipGraph_t graph) {
+        sl.tick++;
+        auto it = std::find_if(sl.kept.begin(), sl.kept.end(), [&](const KeptGraph &k) { return k.key == key; });
+        if (it != sl.kept.end()) {
+            hipGraphNode_t bad = nullptr;
+            hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+            if (hipGraphExecUpdate(it->exec, graph, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+                it->last = sl.tick;
+                graph_updates++;
+                return it->exec;
+            }
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(it->exec);
+            sl.kept.erase(it);
+        }
+        KeptGraph k;
+        k.key = key;
+        k.last = sl.tick;
+        check_hip(hipGraphInstantiate(&k.exec, graph, nullptr, nullptr, 0), "graph instantiate");
+        graph_instantiations++;
+        if ((int)sl.kept.size() >= kKeptGraphs) {
+            auto old = std::min_element(sl.kept.begin(), sl.kept.end(),
+                                        [](const KeptGraph &a, const KeptGraph &b) { return a.last < b.last; });
+            (void)hipGraphExecDestroy(old->exec);
+            sl.kept.erase(old);
+        }
+        sl.kept.push_back(k);
+        return k.exec;
+    }
+
+    // a fixed-count request: everything allocated and uploaded first, then the
+    // kernel chain launched with one synchronisation at the end -- eagerly, or
+    // captured into a graph that is instantiated (or an instantiation of the
+    // slot updated) and launched; the same kernels and arguments either way
+    void run_fixed(Request &r, hipStream_t s, Slot &sl, bool graph_mode) {
+        try {
+            std::shared_ptr<Structure> S;
+            std::unique_ptr<Grid> G;
+            Grid *g = r.grid;
+            Structure::Pending P;
+            BuildJob J{};
+            BuildShape sh;
+            bool build_in_chain = false;
+            if (r.has_pdb) {
+                S = structure_for(r, s);
+                G.reset(new Grid(r.grid_size, r.qmax, r.precision));
+                g = G.get();
+                // the paired builder (mixed, and FP64 unless the plain form is selected)
+                // joins the chain; the plain FP64 builder, the FP64-phase diagnostic and
+                // an autotuning sweep build the grid before it
+                const bool paired = g->fp64() ? !fp64_plain_builder() : getenv("DPLUS_PHASE64") == nullptr;
+                BuildShape tuned;
+                if (!paired) {
+                    S->build(*g, false, s, autotune);
+                } else if (!g->fp64() && autotune && !shape_lookup(g->N, S->ntiles, &tuned)) {
+                    std::lock_guard<std::mutex> lk(tune_mu);
+                    if (!shape_lookup(g->N, S->ntiles, &tuned))
+                        S->autotune(*g, s, nullptr, nullptr);  // times the shapes and builds g with the best
+                    else
+                        build_in_chain = true;
+                } else {
+                    build_in_chain = true;
+                }
+                if (build_in_chain) {
+                    sh = S->shape_for(*g, autotune);
+                    J = S->prepare(*g, sh.slices, P, s);
+                    g->prepare_splines(s);
+                }
+            }
+            const double single[4] = {0, 0, 0, 1}, origin[3] = {0, 0, 0};
+            const int one = 1;
+            dplus_leaf leaf{};
+            leaf.grid = (dplus_grid *)g;
+            if (r.rot.empty()) {
+                leaf.nrot = 1;
+                leaf.rot = single;
+                leaf.ntrans = &one;
+                leaf.trans = origin;
+            } else {
+                leaf.nrot = (int)(r.rot.size() / 4);
+                leaf.rot = r.rot.data();
+                leaf.ntrans = r.ntrans.data();
+                leaf.trans = r.trans.data();
+            }
+            DockSet D(1, &leaf, s);
+            Chi2Request chi;
+            if (r.want_chi) {
+                chi.y = r.y.data();
+                chi.sigma = r.sigma.data();
+                chi.scale = r.fit_scale;
+                chi.constant = r.fit_constant;
+            }
+            const int nq = (int)r.q.size();
+            FixedAverage F;
+            F.prepare(D, nq, r.q.data(), &r.oa, r.want_chi ? &chi : nullptr, s);
+            slot_staging(sl, (size_t)nq + 1);
+            // the chain's shape: kernels (and so the graph's nodes) follow from these
+            std::string key = std::to_string(r.oa.method) + "/" + std::to_string(F.iters) + "/" +
+                              std::to_string(F.ppb) + (D.fp64 ? "d" : "f") + (D.single ? "s" : "r") +
+                              (r.want_chi ? "c" : "-");
+            // (grid sizes, launch sizes and buffers may differ within a key: an update
+            // takes them; rows beyond the one-wave splines add a kernel)
+            if (build_in_chain)
+                key += "/b" + std::to_string(sh.wg) + (P.reduce ? "r" : "u") + (g->fp64() ? "d" : "f") +
+                       (g->N > kWaveShellMax ? "L" : "S");
+            if (!graph_mode) {
+                if (build_in_chain) {
+                    {
+                        KTimer kt("build", s);
+                        launch_build_batch(P.reduce, sh.wg, &J, 1, s);
+                        check_hip(hipGetLastError(), "build_atoms launch");
+                    }
+                    S->enqueue_finish(*g, P, s, true);
+                }
+                F.enqueue(D, sl.h_out, sl.h_out + nq, s, true);
+                check_hip(hipStreamSynchronize(s), "request");
+                timing_collect();
+                finish_fixed(r, F, sl, false);
+                return;
+            }
+            hipGraph_t graph = nullptr;
+            const Clock::time_point t0 = Clock::now();
+            {
+                // no allocation anywhere in the process while this stream captures
+                std::unique_lock<CaptureMutex> lk(capture_mutex());
+                check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+                // a failure inside the capture still ends it, so the stream stays usable
+                struct EndOnThrow {
+                    hipStream_t s;
+                    bool armed = true;
+                    ~EndOnThrow() {
+                        if (!armed) return;
+                        hipGraph_t gg = nullptr;
+                        if (hipStreamEndCapture(s, &gg) == hipSuccess && gg) (void)hipGraphDestroy(gg);
+                    }
+                } end_on_throw{s};
+                if (build_in_chain) {
+                    launch_build_batch(P.reduce, sh.wg, &J, 1, s);
+                    check_hip(hipGetLastError(), "build_atoms launch");
+                    S->enqueue_finish(*g, P, s, false);
+                }
+                F.enqueue(D, sl.h_out, sl.h_out + nq, s, false);
+                end_on_throw.armed = false;
+                check_hip(hipStreamEndCapture(s, &graph), "end capture");
+            }
+            struct DropGraph {
+                hipGraph_t g;
+                ~DropGraph() {
+                    if (g) (void)hipGraphDestroy(g);
+                }
+            } drop{graph};
+            const Clock::time_point t1 = Clock::now();
+            hipGraphExec_t exec = slot_exec(sl, key, graph);
+            const Clock::time_point t2 = Clock::now();
+            graph_capture_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+            graph_exec_ns += std::chrono::duration_cast<std::chrono::nanoseconds>(t2 - t1).count();
+            check_hip(hipGraphLaunch(exec, s), "graph launch");
+            check_hip(hipStreamSynchronize(s), "request");
+            graph_launches++;
+            finish_fixed(r, F, sl, true);
+        } catch (const Error &e) {
+            r.code = e.code;
+            r.message = e.what();
+        } catch (const std::exception &e) {
+            r.code = kErrGeneral;
+            r.message = e.what();
+        }
+    }
 };
 
 // ------------------------------------------------------------------ fit session
@@ -562,11 +812,11 @@ struct Session {
     }
 
     static void host_alloc(void **p, size_t bytes) {
-        std::shared_lock<std::shared_mutex> lk(capture_mutex());
+        std::shared_lock<CaptureMutex> lk(capture_mutex());
         check_hip(hipHostMalloc(p, bytes, hipHostMallocDefault), "hipHostMalloc");
     }
     static void host_free(void *p) {
-        std::shared_lock<std::shared_mutex> lk(capture_mutex());
+        std::shared_lock<CaptureMutex> lk(capture_mutex());
         (void)hipHostFree(p);
     }
 
@@ -591,7 +841,7 @@ struct Session {
         if (exec) (void)hipGraphExecDestroy(exec), exec = nullptr;
         if (graph) (void)hipGraphDestroy(graph), graph = nullptr;
         // no allocation anywhere in the process while this stream captures
-        std::unique_lock<std::shared_mutex> lk(capture_mutex());
+        std::unique_lock<CaptureMutex> lk(capture_mutex());
         check_hip(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
         // a failure inside the capture still ends it, so the stream stays usable
         struct EndOnThrow {
@@ -649,7 +899,7 @@ int dplus_engine_create(const dplus_engine_config *cfg, dplus_engine **out) {
         const long long budget = cfg ? cfg->hbm_budget_bytes : 0;
         const int streams = cfg && cfg->streams > 0 ? cfg->streams : 16;
         *out = (dplus_engine *)new Engine(budget, streams, cfg && cfg->batch_builds != 0, cfg ? cfg->batch_terms : 0,
-                                          cfg && cfg->autotune != 0);
+                                          cfg && cfg->autotune != 0, cfg && cfg->graphs != 0);
     })
 }
 
@@ -771,6 +1021,7 @@ int dplus_engine_wait(dplus_engine *eh, long long ticket, double *intensity, dpl
             res->reserved_bytes = (long long)r->need;
             res->slot = r->slot;
             res->oa = r->st;
+            res->graph = r->graph ? 1 : 0;
         }
         if (r->code) throw Error(r->code, r->message);
         if (intensity) std::copy(r->I.begin(), r->I.end(), intensity);
@@ -811,6 +1062,19 @@ int dplus_engine_batch_stats(dplus_engine *eh, long long *launches, long long *b
         std::lock_guard<std::mutex> lk(e->bmu);
         if (launches) *launches = e->batch_launches;
         if (builds) *builds = e->batch_jobs;
+    })
+}
+
+int dplus_engine_graph_stats(dplus_engine *eh, long long *launches, long long *instantiations, long long *updates,
+                             double *capture_ms, double *exec_ms) {
+    ENGINE_GUARD({
+        Engine *e = (Engine *)eh;
+        if (!e) throw Error(kErrInvalidArgs, "null engine");
+        if (launches) *launches = e->graph_launches.load();
+        if (instantiations) *instantiations = e->graph_instantiations.load();
+        if (updates) *updates = e->graph_updates.load();
+        if (capture_ms) *capture_ms = 1e-6 * (double)e->graph_capture_ns.load();
+        if (exec_ms) *exec_ms = 1e-6 * (double)e->graph_exec_ns.load();
     })
 }
 

@@ -133,7 +133,7 @@ struct Pinned {
     size_t cap = 0;
     V *get(size_t n) {
         if (n > cap) {
-            std::shared_lock<std::shared_mutex> lk(capture_mutex());
+            std::shared_lock<CaptureMutex> lk(capture_mutex());
             if (p) (void)hipHostFree(p);
             p = nullptr;
             cap = 0;
@@ -145,7 +145,7 @@ struct Pinned {
     }
     ~Pinned() {
         if (p) {
-            std::shared_lock<std::shared_mutex> lk(capture_mutex());
+            std::shared_lock<CaptureMutex> lk(capture_mutex());
             (void)hipHostFree(p);
         }
     }

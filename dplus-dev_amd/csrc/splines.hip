@@ -58,7 +58,6 @@ static void spline_table(int n, double *t) {
     for (int i = 0; i < n; i++) dg[i] = (double)diag[i];
 }
 
-constexpr int kWaveShellMax = 170;  // shells of the one-wave kernel: rows of n = 6i <= 1024 nodes
 
 // the same five tables for the one-wave kernel, lane-interleaved: with
 // m = ceil(n / 64), entry e * 64 + l of each block of 64 m holds the serial

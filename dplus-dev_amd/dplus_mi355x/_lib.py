@@ -61,7 +61,7 @@ class OAStats(ctypes.Structure):
 
 class EngineConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_longlong), ("streams", ctypes.c_int), ("batch_builds", ctypes.c_int),
-                ("batch_terms", ctypes.c_longlong), ("autotune", ctypes.c_int)]
+                ("batch_terms", ctypes.c_longlong), ("autotune", ctypes.c_int), ("graphs", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -77,7 +77,7 @@ class RequestResult(ctypes.Structure):
     _fields_ = [
         ("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double),
         ("chi2", ctypes.c_double), ("chi2_on_device", ctypes.c_int), ("slot", ctypes.c_int),
-        ("reserved_bytes", ctypes.c_longlong), ("oa", OAStats),
+        ("reserved_bytes", ctypes.c_longlong), ("oa", OAStats), ("graph", ctypes.c_int),
     ]
 
 
@@ -164,6 +164,7 @@ SIGNATURES = {
     "dplus_engine_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 5),
     "dplus_engine_cache_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
     "dplus_engine_batch_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 2),
+    "dplus_engine_graph_stats": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_longlong)] * 3 + [_D, _D]),
     "dplus_session_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _D, _D, _D,
                                             ctypes.c_ulonglong, ctypes.c_longlong, ctypes.POINTER(ctypes.c_void_p)]),
     "dplus_session_evaluate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Leaf), ctypes.c_int, ctypes.c_double,

@@ -14,12 +14,14 @@ class Engine:
     (I(q), result dict).  batch_builds groups small mixed builds (at most
     batch_terms canonical-node x source terms; 0: 1e10) into shared builder
     launches; autotune tunes each new (grid, tile class) build shape on first
-    sight."""
+    sight; graphs runs each fixed-count request (MC or VEGAS with eps <= 0) as
+    one captured HIP graph of its whole kernel chain."""
 
-    def __init__(self, hbm_budget_bytes=0, streams=16, batch_builds=False, batch_terms=0, autotune=False):
+    def __init__(self, hbm_budget_bytes=0, streams=16, batch_builds=False, batch_terms=0, autotune=False,
+                 graphs=False):
         h = ctypes.c_void_p()
         cfg = L.EngineConfig(int(hbm_budget_bytes), int(streams), 1 if batch_builds else 0, int(batch_terms),
-                             1 if autotune else 0)
+                             1 if autotune else 0, 1 if graphs else 0)
         L.check(L.lib().dplus_engine_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
         self._nq = {}
@@ -86,6 +88,10 @@ class Engine:
         b = [ctypes.c_longlong() for _ in range(2)]
         L.check(L.lib().dplus_engine_batch_stats(self._h, *[ctypes.byref(x) for x in b]))
         out.update(zip(("batch_launches", "batched_builds"), (x.value for x in b)))
+        g = [ctypes.c_longlong() for _ in range(3)] + [ctypes.c_double() for _ in range(2)]
+        L.check(L.lib().dplus_engine_graph_stats(self._h, *[ctypes.byref(x) for x in g]))
+        out.update(zip(("graph_launches", "graph_instantiations", "graph_updates", "graph_capture_ms",
+                        "graph_exec_ms"), (x.value for x in g)))
         return out
 
     def close(self):

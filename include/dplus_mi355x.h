@@ -385,6 +385,11 @@ typedef struct {
                                    whose (grid, tile class) the table lacks is autotuned on
                                    first sight (dplus_structure_autotune); 0: the occupancy
                                    model's shapes */
+    int graphs;                 /* 1: a fixed-count request (Monte Carlo or VEGAS with eps <= 0)
+                                   runs its kernel chain (builder, splines, orientation
+                                   average, chi^2, result download) as one captured HIP graph,
+                                   updating an instantiation its slot keeps for the chain's
+                                   shape; the same bits as the eager launches */
 } dplus_engine_config;
 
 typedef struct {
@@ -415,6 +420,7 @@ typedef struct {
     int slot;           /* stream the request ran on */
     long long reserved_bytes;
     dplus_oa_stats oa;
+    int graph;          /* 1 when the request ran as a captured HIP graph */
 } dplus_request_result;
 
 int dplus_engine_create(const dplus_engine_config *cfg, dplus_engine **out);
@@ -438,6 +444,12 @@ int dplus_engine_cache_stats(dplus_engine *e, long long *hits, long long *misses
 /* grouped builder launches of the engine (no counterpart in the reference):
  * launches made and builds they carried */
 int dplus_engine_batch_stats(dplus_engine *e, long long *launches, long long *builds);
+/* captured requests of the engine (no counterpart in the reference): graph
+ * launches, instantiations and in-place updates of kept instantiations, and
+ * the host milliseconds summed over requests spent capturing (waits for the
+ * process-wide capture lock included) and updating or instantiating */
+int dplus_engine_graph_stats(dplus_engine *e, long long *launches, long long *instantiations, long long *updates,
+                             double *capture_ms, double *exec_ms);
 
 /* ------------------------------------------------------------------ fit sessions */
 /* Fitting-loop evaluation over resident subunit grids (the hybrid docking of

@@ -477,20 +477,21 @@ def test_autotune_shape_table(dplus):
 
 
 def test_2000_atoms_grid100_500q_mc_and_gk_vs_debye(dplus, oracle):
-    """The 2,000-atom workload as specified (3 nm sphere, grid 100, qmax 5,
-    500 q, MC over 2^20 directions, fixed seed) against the FP64 Debye sum:
-    the MC curve within its sampling error, Gauss-Kronrod (no sampling error)
-    within the grid's interpolation error, and mixed within 1e-4 of FP64."""
-    say("[case] 2000 atoms, grid 100, 500 q, MC 2^20 directions and GK vs Debye")
+    """The 2,000-atom workload as specified (3 nm sphere, solvent 333 e/nm^3,
+    grid 100, qmax 5, 500 q, MC over 10^6 directions in chunks of 8,192, fixed
+    seed) against the FP64 Debye sum: the MC curve within its sampling error,
+    Gauss-Kronrod (no sampling error) within the grid's interpolation error,
+    and mixed within 1e-4 of FP64."""
+    say("[case] 2000 atoms, grid 100, 500 q, MC 10^6 directions and GK vs Debye")
     xyz, ion, coef9, rad = structure(2000, 1, radius=3.0)
     q = np.linspace(0.0, 5.0, 500)
-    deb = oracle.debye(xyz, ion, oracle_ions(oracle, coef9, rad, ion), q)
+    deb = oracle.debye(xyz, ion, oracle_ions(oracle, coef9, rad, ion, sol=333.0), q)
     curves = {}
     for prec in (0, 1):
-        g = gpu_grid(dplus, 100, 5.0, prec, xyz, ion, coef9, rad)
-        curves[prec], st = dplus.orientation_average(dplus.Docking.single(g), q, seed=1, max_dirs=1 << 20,
+        g = gpu_grid(dplus, 100, 5.0, prec, xyz, ion, coef9, rad, sol=333.0)
+        curves[prec], st = dplus.orientation_average(dplus.Docking.single(g), q, seed=1, max_dirs=10 ** 6,
                                                      chunk=8192)
-        assert st.dirs_used == 1 << 20
+        assert st.dirs_used == 123 * 8192  # 10^6 rounded up to whole chunks
         if prec == 0:
             sub = q[::25]
             gk, _ = dplus.orientation_average(dplus.Docking.single(g), sub, method=dplus.OA_GK, eps=1e-6)

@@ -71,6 +71,35 @@ def test_engine_same_alone_and_in_the_mix(dplus):
             assert r1["chi2_on_device"] == 1
 
 
+def test_engine_graph_requests_match_eager(dplus):
+    from dplus_mi355x.serving import Engine
+    say("[case] engine: 10 requests eager vs captured graphs (1 stream twice, then 4 streams)")
+    reqs = requests(dplus)
+    e1 = Engine(streams=1)
+    eager = [e1.wait(e1.submit(q, **kw)) for q, kw in reqs]
+    e1.close()
+    assert not any(r["graph"] for _, r in eager)
+    # one slot: the second pass updates the instantiations the first pass kept
+    g1 = Engine(streams=1, graphs=True)
+    passes = [[g1.wait(g1.submit(q, **kw)) for q, kw in reqs] for _ in range(2)]
+    st1 = g1.stats()
+    g1.close()
+    g4 = Engine(streams=4, graphs=True)
+    tickets = [g4.submit(q, **kw) for q, kw in reqs]
+    passes.append([g4.wait(t) for t in tickets])
+    st4 = g4.stats()
+    g4.close()
+    say(f"    graph stats one slot {st1}, four slots {st4}")
+    assert st1["graph_launches"] == 2 * len(reqs) and st1["graph_updates"] >= len(reqs)
+    assert st4["graph_launches"] == len(reqs)
+    for res in passes:
+        for (Ie, re), (Ig, rg), (q, kw) in zip(eager, res, reqs):
+            assert rg["graph"] == 1
+            assert np.array_equal(Ie, Ig), (kw.get("method"), kw["grid_size"], kw["precision"])
+            assert re["chi2"] == rg["chi2"] and re["chi2_on_device"] == rg["chi2_on_device"]
+            assert (re["dirs_used"], re["iterations_used"]) == (rg["dirs_used"], rg["iterations_used"])
+
+
 def test_engine_matches_oracle(dplus, oracle):
     from dplus_mi355x.serving import Engine
     say("[case] engine request against the FP64 oracle")
