@@ -198,7 +198,7 @@ def plain_curve(D, S, steps=5):
 
 
 def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, streams=16, fp64=True,
-                   batch_builds=False, graphs=False, compare_modes=True, repeats=3):
+                   batch_builds=False, graphs=True, compare_modes=True, repeats=3):
     """Batched serving stream: n_requests with Poisson arrivals at `rate`/s; each
     request is its own seeded random-walk structure (or one of `pool_size`; atoms
     log-uniform in 500..200,000 at 50 atoms/nm^3, dummy-atom solvent), qmax
@@ -251,8 +251,12 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, st
         saturated[m]["values_all"] = [n_requests / x[1] for x in runs]
         if not all(np.array_equal(a[0], b[0]) for x in runs for a, b in zip(res_m, x[0])):
             saturated["repeat_mismatch"] = True
+        qms = np.array([r["queue_ms"] for _, r in res_m])
+        rms = np.array([r["run_ms"] for _, r in res_m])
         log(f"[serving] mixed ({m}): {n_requests / span_m:.2f} curves/s over {span_m:.2f} s, "
-            f"p50 {np.percentile(lat_m, 50):.1f} ms, p99 {np.percentile(lat_m, 99):.1f} ms, "
+            f"p50 {np.percentile(lat_m, 50):.1f} ms, p99 {np.percentile(lat_m, 99):.1f} ms "
+            f"(queue p50 {np.percentile(qms, 50):.1f}, run p50 {np.percentile(rms, 50):.1f} / p99 "
+            f"{np.percentile(rms, 99):.1f} ms), "
             f"max concurrent {st_m['max_concurrent']}, graph launches {st_m['graph_launches']} "
             f"(instantiated {st_m['graph_instantiations']}, updated {st_m['graph_updates']}; host ms capturing "
             f"{st_m['graph_capture_ms']:.1f}, updating/instantiating {st_m['graph_exec_ms']:.1f})")
@@ -265,11 +269,13 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, st
     # latency without a standing queue, every mode at the same arrival rate
     rate70 = 0.7 * n_requests / span
     for m in order:
-        _, span70, lat70, _ = run_mixed(arrivals * (rate / rate70), m == "graphs")
+        res70, span70, lat70, _ = run_mixed(arrivals * (rate / rate70), m == "graphs")
         loaded_modes[m] = {"rate": rate70, "value": n_requests / span70, "p50_ms": float(np.percentile(lat70, 50)),
                            "p99_ms": float(np.percentile(lat70, 99))}
+        rms = np.array([r["run_ms"] for _, r in res70])
         log(f"[serving] mixed ({m}) at {rate70:.0f}/s: {n_requests / span70:.2f} curves/s, "
-            f"p50 {loaded_modes[m]['p50_ms']:.1f} ms, p99 {loaded_modes[m]['p99_ms']:.1f} ms")
+            f"p50 {loaded_modes[m]['p50_ms']:.1f} ms, p99 {loaded_modes[m]['p99_ms']:.1f} ms "
+            f"(run p50 {np.percentile(rms, 50):.1f} / p99 {np.percentile(rms, 99):.1f} ms)")
     loaded = loaded_modes[mode]
     batching = {"batch_builds": bool(batch_builds), "batch_launches": st["batch_launches"],
                 "batched_builds": st["batched_builds"], "engine_mode": mode,
@@ -403,6 +409,9 @@ def main():
                     help="the 50k-atom grid built once and averaged by Gauss-Kronrod three times: for rocprofv3")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving stream's mixed legs (saturated and at 70 %% load): for rocprofv3")
+    ap.add_argument("--serving-streams", type=int, default=16, help="engine slots (streams) of the serving stream")
+    ap.add_argument("--serving-eager", action="store_true",
+                    help="the serving stream's primary mode is eager launches (captured graphs beside it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (the path shards by shell "
@@ -417,7 +426,8 @@ def main():
     if not info.get("ok"):
         raise SystemExit("no usable GPU")
     if args.serving_only:
-        print(json.dumps({"serving": serving_stream(D, S, n_requests=args.serving_requests, fp64=False)}))
+        print(json.dumps({"serving": serving_stream(D, S, n_requests=args.serving_requests, fp64=False,
+                                                    streams=args.serving_streams, graphs=not args.serving_eager)}))
         return
     W = make_workload(D, S)
     st = resident(D, W)
@@ -525,14 +535,20 @@ def main():
         log(f"[bench] fp64 plain builder: {np.mean(lat64p):.3f} s/curve; vs paired FP64 {err_plain:.2e}")
 
     # ---- the same grids averaged by adaptive Gauss-Kronrod (relative tolerance 1e-4, depth 8)
+    # (one untimed average first: the first one grows the device memory pool by the GK arenas)
     gk = {}
+    D.orientation_average(D.Docking.single(g32), W["q"], method=D.OA_GK, eps=1e-4, gk_max_depth=8)
     for name, g in (("mixed", g32), ("fp64", g64)):
-        t0 = time.perf_counter()
-        Igk, stg = D.orientation_average(D.Docking.single(g), W["q"], method=D.OA_GK, eps=1e-4, gk_max_depth=8)
-        gk[name] = (Igk, time.perf_counter() - t0, stg.evaluations)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            Igk, stg = D.orientation_average(D.Docking.single(g), W["q"], method=D.OA_GK, eps=1e-4, gk_max_depth=8)
+            ts.append(time.perf_counter() - t0)
+        gk[name] = (Igk, float(np.median(ts)), stg.evaluations)
     gk_err = float(np.max(np.abs(gk["mixed"][0] - gk["fp64"][0]) / np.abs(gk["fp64"][0])))
     gk_out = {"tolerance": 1e-4, "max_depth": 8, "mixed_s": gk["mixed"][1], "fp64_s": gk["fp64"][1],
-              "evaluations": gk["mixed"][2], "curve_s_mixed": gk["mixed"][1] + ms_build / max(nb, 1) * 1e-3,
+              "timing": "median of 3 per mode after one untimed average",
+              "evaluations": gk["mixed"][2], "evaluations_fp64": gk["fp64"][2], "curve_s_mixed": gk["mixed"][1] + ms_build / max(nb, 1) * 1e-3,
               "max_rel_err_vs_fp64": gk_err}
     log(f"[bench] GK (tol 1e-4): mixed {gk['mixed'][1]:.3f} s, fp64 {gk['fp64'][1]:.3f} s per average "
         f"({gk['mixed'][2]} integrand evaluations); max rel I(q) mixed vs fp64 = {gk_err:.2e}")
@@ -593,7 +609,8 @@ def main():
 
     serving = sessions = None
     if not args.no_serving:
-        serving = serving_stream(D, S, n_requests=args.serving_requests)
+        serving = serving_stream(D, S, n_requests=args.serving_requests, streams=args.serving_streams,
+                                 graphs=not args.serving_eager)
         sessions = fit_sessions(D, S, evals=args.session_evals)
 
     value = args.steps / elapsed

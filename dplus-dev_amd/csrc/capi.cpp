@@ -218,7 +218,41 @@ void DevBuf::ensure(size_t bytes) {
 }
 void DevBuf::upload(const void *src, size_t bytes, hipStream_t s) {
     ensure(bytes);
-    if (bytes) check_hip(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    if (!bytes) return;
+    if (borrowed && t_arena && t_arena->stage((size_t)((char *)p - t_arena->base), src, bytes)) return;
+    check_hip(hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+}
+
+bool Arena::stage(size_t dev, const void *src, size_t bytes) {
+    if (!staging) return false;
+    // a run grows over the alignment padding between consecutive uploads (at most
+    // 255 bytes, holding no buffer: a buffer carved between two uploads puts at
+    // least 256 bytes between them), so staging mirrors the device layout
+    size_t pad = 0;
+    bool extend = false;
+    if (!runs.empty()) {
+        const Run &r = runs.back();
+        const size_t end = r.dev + r.bytes;
+        if (dev >= end && dev - end < 256) {
+            pad = dev - end;
+            extend = true;
+        }
+    }
+    if (pad + bytes > staging_size - staged) return false;
+    memset(staging + staged, 0, pad);
+    memcpy(staging + staged + pad, src, bytes);
+    if (extend)
+        runs.back().bytes += pad + bytes;
+    else
+        runs.push_back({dev, staged, bytes});
+    staged += pad + bytes;
+    return true;
+}
+
+void Arena::flush(hipStream_t s) {
+    for (const Run &r : runs)
+        check_hip(hipMemcpyAsync(base + r.dev, staging + r.host, r.bytes, hipMemcpyHostToDevice, s), "H2D");
+    runs.clear();
 }
 
 // spline multiplier tables, one set per shell count, kept for the process
@@ -248,7 +282,7 @@ static SplineTables *tables_for(int nshells, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ grid
-Grid::Grid(int gs, double qm, int prec) {
+Grid::Grid(int gs, double qm, int prec, bool zero) {
     if (gs % 2) gs++;
     if (gs < 2 || gs > 4000 || !(qm > 0)) throw Error(kErrInvalidArgs, "invalid grid size or qmax");
     grid_size = gs;
@@ -259,7 +293,7 @@ Grid::Grid(int gs, double qm, int prec) {
     P = total_points(gs);
     Device::get().require();
     dev.ensure((size_t)P * node_bytes());
-    check_hip(hipMemsetAsync(dev.p, 0, (size_t)P * node_bytes(), alloc_stream()), "hipMemset");
+    if (zero) check_hip(hipMemsetAsync(dev.p, 0, (size_t)P * node_bytes(), alloc_stream()), "hipMemset");
 }
 
 void Grid::ensure_host() {
@@ -1115,6 +1149,43 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
     }
 }
 
+// Device bytes the Monte Carlo / VEGAS orientation average takes from a
+// scratch arena: every DevBuf of DockSet and of FixedAverage / the
+// orientation average's MC and VEGAS branches, in their sizes, each 256-byte
+// aligned (and the chi^2 inputs of a fixed-count average); *upload_bytes: the
+// aligned sizes of the uploaded ones.
+size_t oa_arena_bytes(int nleaf, long long nrot, long long ntr, int nq, const double *q, double step,
+                      const dplus_oa_params *p, bool chi, size_t *upload_bytes) {
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    size_t up = 0;
+    auto upl = [&](size_t b) {
+        up += al(b);
+        return al(b);
+    };
+    size_t b = upl((size_t)nleaf * sizeof(void *)) + upl((size_t)nleaf * sizeof(LeafGeo)) +
+               upl((size_t)std::max<long long>(1, nrot) * sizeof(RotDev)) +
+               upl((size_t)std::max<long long>(1, ntr) * sizeof(double4));
+    if (nq > 0) {
+        const std::vector<GroupDev> groups = make_groups(q, nq, step);
+        const size_t ng = groups.size();
+        size_t mstride = 1;
+        for (auto &g : groups) mstride = std::max<size_t>(mstride, (size_t)g.m);
+        b += upl((size_t)nq * 8) + upl(ng * sizeof(GroupDev)) + upl(ng * sizeof(int));
+        if (p->method == DPLUS_OA_MC) {
+            const long long chunk = p->chunk > 0 ? p->chunk : 8192;
+            const long long nchunks = (std::max<long long>(1, p->max_dirs) + chunk - 1) / chunk;
+            const size_t nblk = (size_t)(p->eps <= 0.0 ? nchunks * chunk / 256 : chunk / 256);
+            b += al(ng * nblk * mstride * 8) + al(ng * mstride * 8) + (p->eps <= 0.0 ? al((size_t)nq * 8) : 0);
+        } else {
+            const size_t iters = (size_t)std::max(1, p->vegas_iterations);
+            b += upl(ng * 66 * 8) + al(ng * mstride * 1024 * 8) + al(ng * 1024 * 8) + al(iters * nq * 8);
+        }
+        if (chi) b += upl((size_t)nq * 8) + upl((size_t)nq * 8) + al(8);
+    }
+    if (upload_bytes) *upload_bytes = up;
+    return b;
+}
+
 }  // namespace dplus
 
 using namespace dplus;
@@ -1384,33 +1455,6 @@ int dplus_orientation_average(int nleaf, const dplus_leaf *leaves, int nq, const
         timing_collect();
         if (st) st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     })
-}
-
-// Device bytes the Monte Carlo / VEGAS orientation average takes from a
-// scratch arena: every DevBuf of DockSet and of orientation_average's MC and
-// VEGAS branches, in their sizes, each 256-byte aligned.
-static size_t oa_arena_bytes(int nleaf, long long nrot, long long ntr, int nq, const double *q, double step,
-                             const dplus_oa_params *p) {
-    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    size_t b = al((size_t)nleaf * sizeof(void *)) + al((size_t)nleaf * sizeof(LeafGeo)) +
-               al((size_t)std::max<long long>(1, nrot) * sizeof(RotDev)) +
-               al((size_t)std::max<long long>(1, ntr) * sizeof(double4));
-    if (nq <= 0) return b;
-    const std::vector<GroupDev> groups = make_groups(q, nq, step);
-    const size_t ng = groups.size();
-    size_t mstride = 1;
-    for (auto &g : groups) mstride = std::max<size_t>(mstride, (size_t)g.m);
-    b += al((size_t)nq * 8) + al(ng * sizeof(GroupDev)) + al(ng * sizeof(int));
-    if (p->method == DPLUS_OA_MC) {
-        const long long chunk = p->chunk > 0 ? p->chunk : 8192;
-        const long long nchunks = (std::max<long long>(1, p->max_dirs) + chunk - 1) / chunk;
-        const size_t nblk = (size_t)(p->eps <= 0.0 ? nchunks * chunk / 256 : chunk / 256);
-        b += al(ng * nblk * mstride * 8) + al(ng * mstride * 8) + (p->eps <= 0.0 ? al((size_t)nq * 8) : 0);
-    } else {
-        const size_t iters = (size_t)std::max(1, p->vegas_iterations);
-        b += al(ng * 66 * 8) + al(ng * mstride * 1024 * 8) + al(ng * 1024 * 8) + al(iters * nq * 8);
-    }
-    return b;
 }
 
 int dplus_orientation_average_noalloc(int nleaf, const dplus_leaf *leaves, int nq, const double *q,

@@ -111,15 +111,37 @@ struct DevBuf {
     void upload(const void *src, size_t bytes, hipStream_t s);
 };
 
-// Scratch arena of the allocation-free entry point: while a thread has one
-// installed, DevBuf::ensure carves its buffers from [base, base + size) in
-// call order (256-byte aligned) instead of allocating, failing with
-// ERROR_INVALID_ARGS when the arena is exhausted, and release() frees nothing.
+// Scratch arena (the allocation-free entry point, the request engine's
+// slots): while a thread has one installed, DevBuf::ensure carves its buffers
+// from [base, base + size) in call order (256-byte aligned) instead of
+// allocating, failing with ERROR_INVALID_ARGS when the arena is exhausted, and
+// release() frees nothing.  With a pinned staging buffer, DevBuf::upload
+// copies into staging instead of issuing a copy, and flush() sends the staged
+// bytes with one copy per run of uploads that are contiguous on the device
+// (nothing reads them on the device before flush).
 struct Arena {
     char *base = nullptr;
     size_t size = 0, used = 0;
+    char *staging = nullptr;  // pinned host memory of staging_size bytes, or none
+    size_t staging_size = 0, staged = 0;
+    struct Run {
+        size_t dev, host, bytes;
+    };
+    std::vector<Run> runs;
+    void reset() {
+        used = staged = 0;
+        runs.clear();
+    }
+    // stage bytes for device offset dev (false: no room, copy directly)
+    bool stage(size_t dev, const void *src, size_t bytes);
+    void flush(hipStream_t s);
 };
 void set_arena(Arena *a);  // nullptr: allocate again
+// device bytes of a Monte Carlo or VEGAS average's buffers (docked leaves,
+// q, groups, partial sums, bins, chi^2 inputs) carved from an arena, each
+// 256-byte aligned; host bytes they upload
+size_t oa_arena_bytes(int nleaf, long long nrot, long long ntr, int nq, const double *q, double step,
+                      const dplus_oa_params *p, bool chi = false, size_t *upload_bytes = nullptr);
 // the allocation-free orientation average's device array (scratch.hip): its
 // size, the address of this device's copy (looked up on first use, with the
 // device current), and the lock a call holds while it uses it
@@ -180,7 +202,9 @@ struct Grid {
     long long P = 0;
     DevBuf dev;  // P nodes of (Re, Im, dRe, dIm) in float4 or double4
     std::vector<double> host_data, host_interp;
-    Grid(int gs, double qmax, int precision);
+    // zero = false leaves the nodes unset: for a grid a build fills at once (a
+    // build writes every node and every spline coefficient)
+    Grid(int gs, double qmax, int precision, bool zero = true);
     bool fp64() const { return precision == DPLUS_PRECISION_FP64; }
     size_t node_bytes() const { return fp64() ? sizeof(double4) : sizeof(float4); }
     void ensure_host();

@@ -215,18 +215,27 @@ struct Engine {
     // captured requests: per slot, pinned result staging and the instantiated
     // graphs kept by chain key (least recently used dropped past kKeptGraphs)
     static constexpr int kKeptGraphs = 16;
+    static constexpr int kMaxSlots = 32;  // worker threads, each with its own stream
     struct KeptGraph {
         std::string key;
         hipGraphExec_t exec = nullptr;
         unsigned long long last = 0;
     };
+    // each slot also keeps a device arena and its pinned staging for the small
+    // buffers of its requests (leaf tables, q, groups, partial sums, bins, chi^2
+    // inputs), grown to the largest request it has run: no allocation per
+    // request, and their uploads go up in one or two copies
     struct Slot {
         double *h_out = nullptr;
         size_t cap = 0;
         std::vector<KeptGraph> kept;
         unsigned long long tick = 0;
+        Arena arena;
+        DevBuf arena_dev;
+        char *arena_pin = nullptr;
+        size_t pin_cap = 0;
     };
-    std::vector<Slot> slots;
+    std::deque<Slot> slots;  // never relocated (slots hold device buffers)
     std::atomic<long long> graph_launches{0}, graph_instantiations{0}, graph_updates{0};
     // host time of captured requests (ns): waiting for the capture lock and
     // capturing; updating or instantiating the executable graph
@@ -250,7 +259,7 @@ struct Engine {
         autotune = tune;
         graphs = graph;
         if (bterms > 0) batch_terms = (double)bterms;
-        nstreams = std::max(1, std::min(16, nstreams));
+        nstreams = std::max(1, std::min(kMaxSlots, nstreams));
         for (int i = 0; i < nstreams; i++) {
             hipStream_t s;
             check_hip(hipStreamCreate(&s), "hipStreamCreate");
@@ -284,10 +293,11 @@ struct Engine {
         for (size_t i = 0; i < slots.size(); i++) {
             (void)hipStreamSynchronize(streams[i]);
             for (auto &k : slots[i].kept) (void)hipGraphExecDestroy(k.exec);
-            if (slots[i].h_out) {
-                std::shared_lock<CaptureMutex> lk(capture_mutex());
-                (void)hipHostFree(slots[i].h_out);
-            }
+            std::shared_lock<CaptureMutex> lk(capture_mutex());
+            if (slots[i].h_out) (void)hipHostFree(slots[i].h_out);
+            if (slots[i].arena_pin) (void)hipHostFree(slots[i].arena_pin);
+            lk.unlock();
+            slots[i].arena_dev.release();  // ordered on the slot's stream, before it is destroyed
         }
         for (auto s : streams) {
             (void)hipStreamSynchronize(s);
@@ -591,6 +601,27 @@ struct Engine {
         r.code = 0;
     }
 
+    // the slot's arena sized for dev_bytes of buffers and host_bytes of staged
+    // uploads, emptied for the next request (its previous request has finished)
+    static void slot_arena(Slot &sl, size_t dev_bytes, size_t host_bytes) {
+        if (dev_bytes > sl.arena_dev.n)
+            sl.arena_dev.ensure(std::max(dev_bytes, std::max<size_t>(2 * sl.arena_dev.n, (size_t)4 << 20)));
+        if (host_bytes > sl.pin_cap) {
+            std::shared_lock<CaptureMutex> lk(capture_mutex());
+            if (sl.arena_pin) (void)hipHostFree(sl.arena_pin);
+            sl.arena_pin = nullptr;
+            sl.pin_cap = 0;
+            const size_t cap = std::max(host_bytes, std::max<size_t>(2 * sl.pin_cap, (size_t)1 << 20));
+            check_hip(hipHostMalloc((void **)&sl.arena_pin, cap, hipHostMallocDefault), "hipHostMalloc");
+            sl.pin_cap = cap;
+        }
+        sl.arena.base = (char *)sl.arena_dev.p;
+        sl.arena.size = sl.arena_dev.n;
+        sl.arena.staging = sl.arena_pin;
+        sl.arena.staging_size = sl.pin_cap;
+        sl.arena.reset();
+    }
+
     // pinned staging of a slot's results (nq intensities, then chi^2)
     static void slot_staging(Slot &sl, size_t n) {
         if (n <= sl.cap) return;
@@ -651,7 +682,7 @@ struct Engine {
             bool build_in_chain = false;
             if (r.has_pdb) {
                 S = structure_for(r, s);
-                G.reset(new Grid(r.grid_size, r.qmax, r.precision));
+                G.reset(new Grid(r.grid_size, r.qmax, r.precision, false));
                 g = G.get();
                 // the paired builder (mixed, and FP64 unless the plain form is selected)
                 // joins the chain; the plain FP64 builder, the FP64-phase diagnostic and
@@ -690,7 +721,6 @@ struct Engine {
                 leaf.ntrans = r.ntrans.data();
                 leaf.trans = r.trans.data();
             }
-            DockSet D(1, &leaf, s);
             Chi2Request chi;
             if (r.want_chi) {
                 chi.y = r.y.data();
@@ -699,8 +729,26 @@ struct Engine {
                 chi.constant = r.fit_constant;
             }
             const int nq = (int)r.q.size();
+            // leaf tables, q, groups, scratch and chi^2 inputs from the slot's arena,
+            // their uploads staged and sent together
+            long long ntr = 0;
+            for (int k = 0; k < leaf.nrot; k++) ntr += leaf.ntrans[k];
+            size_t up = 0;
+            const size_t need =
+                oa_arena_bytes(1, leaf.nrot, ntr, nq, r.q.data(), g->step, &r.oa, r.want_chi, &up);
+            slot_arena(sl, need, up);
+            std::unique_ptr<DockSet> Dp;
             FixedAverage F;
-            F.prepare(D, nq, r.q.data(), &r.oa, r.want_chi ? &chi : nullptr, s);
+            {
+                struct Install {
+                    explicit Install(Arena *a) { set_arena(a); }
+                    ~Install() { set_arena(nullptr); }
+                } install(&sl.arena);
+                Dp.reset(new DockSet(1, &leaf, s));
+                F.prepare(*Dp, nq, r.q.data(), &r.oa, r.want_chi ? &chi : nullptr, s);
+            }
+            sl.arena.flush(s);
+            const DockSet &D = *Dp;
             slot_staging(sl, (size_t)nq + 1);
             // the chain's shape: kernels (and so the graph's nodes) follow from these
             std::string key = std::to_string(r.oa.method) + "/" + std::to_string(F.iters) + "/" +

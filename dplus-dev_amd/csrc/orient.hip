@@ -391,16 +391,19 @@ __global__ __launch_bounds__(256) void gather_probe(const float4 *__restrict__ g
 }
 
 // sum the block partials of every active group in a fixed order:
-// out[gi * mstride + j]
-__global__ void mc_reduce(const GroupDev *__restrict__ groups, const int *__restrict__ active,
-                          const double *__restrict__ partial, int nblk, int mstride,
-                          double *__restrict__ out) {
+// out[gi * mstride + j]; one wave per q (lane l adds blocks l, l + 64, ... in
+// turn, then the wave's fixed shuffle tree), four waves per group
+__global__ __launch_bounds__(256) void mc_reduce(const GroupDev *__restrict__ groups, const int *__restrict__ active,
+                                                 const double *__restrict__ partial, int nblk, int mstride,
+                                                 double *__restrict__ out) {
     const int gi = blockIdx.x;
     const int m = groups[active[gi]].m;
-    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    for (int j = threadIdx.x >> 6; j < m; j += blockDim.x >> 6) {
         double s = 0.0;
-        for (int b = 0; b < nblk; b++) s += partial[((size_t)gi * nblk + b) * mstride + j];
-        out[(size_t)gi * mstride + j] = s;
+        for (int b = lane; b < nblk; b += 64) s += partial[((size_t)gi * nblk + b) * mstride + j];
+        s = wave_sum(s);
+        if (lane == 0) out[(size_t)gi * mstride + j] = s;
     }
 }
 
@@ -651,9 +654,11 @@ __device__ __forceinline__ double vegas_weight(const double *var, double total, 
 }
 
 // serial sweep of the re-binning: new interior boundaries at equal shares of
-// the weights; boundaries the sweep does not reach keep their previous value
-__device__ void vegas_sweep(const double *w, double *b) {
-    double tw = 0.0, nb[kVB];
+// the weights; boundaries the sweep does not reach keep their previous value.
+// w, b and the scratch nb (kVB entries) are LDS arrays: the sweep's chain of
+// dependent steps then waits on LDS, not on global or private memory.
+__device__ void vegas_sweep(const double *w, double *b, double *nb) {
+    double tw = 0.0;
     for (int i = 0; i < kVB; i++) tw += w[i];
     for (int i = 0; i < kVB; i++) nb[i] = b[i + 1];
     double xold, xnew = 0., dw = 0., pts = tw / kVB;
@@ -692,9 +697,14 @@ __global__ __launch_bounds__(1024) void vegas_reduce(const GroupDev *__restrict_
     __shared__ double marg[2][kVB];
     __shared__ double wgt[2][kVB];
     __shared__ double tot[2];
+    __shared__ double sb[2][kVB + 1], snb[2][kVB];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     sv[bin] = binvar[(size_t)gi * 1024 + bin];
     svol[bin] = (bt[ti + 1] - bt[ti]) * (bp[pj + 1] - bp[pj]) / (double)ppb;
+    if (threadIdx.x < 2 * (kVB + 1)) {
+        const int ax = threadIdx.x / (kVB + 1), i = threadIdx.x % (kVB + 1);
+        sb[ax][i] = (ax == 0 ? bt : bp)[i];
+    }
     __syncthreads();
     for (int j = w; j < G.m; j += 16) {
         const double *bs = binsum + ((size_t)gi * mstride + j) * 1024;
@@ -727,7 +737,12 @@ __global__ __launch_bounds__(1024) void vegas_reduce(const GroupDev *__restrict_
         wgt[ax][i] = tot[ax] == 0. ? 0. : vegas_weight(marg[ax], tot[ax], i);
     }
     __syncthreads();
-    if (threadIdx.x < 2 && tot[threadIdx.x] != 0.) vegas_sweep(wgt[threadIdx.x], threadIdx.x == 0 ? bt : bp);
+    if (threadIdx.x < 2 && tot[threadIdx.x] != 0.) vegas_sweep(wgt[threadIdx.x], sb[threadIdx.x], snb[threadIdx.x]);
+    __syncthreads();
+    if (threadIdx.x < 2 * (kVB + 1)) {
+        const int ax = threadIdx.x / (kVB + 1), i = threadIdx.x % (kVB + 1);
+        (ax == 0 ? bt : bp)[i] = sb[ax][i];
+    }
 }
 
 // ------------------------------------------------------------------ single points
@@ -1028,7 +1043,7 @@ void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *r
     else
         DPLUS_MC(float, float4, false);
 #undef DPLUS_MC
-    mc_reduce<<<nactive, 64, 0, s>>>(groups, active, partial, nblk, mstride, out);
+    mc_reduce<<<nactive, 256, 0, s>>>(groups, active, partial, nblk, mstride, out);
 }
 
 // the gather-ceiling probe over layers [layer0, layer0 + nlayers) of a mixed grid

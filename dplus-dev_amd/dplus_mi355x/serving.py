@@ -18,7 +18,7 @@ class Engine:
     one captured HIP graph of its whole kernel chain."""
 
     def __init__(self, hbm_budget_bytes=0, streams=16, batch_builds=False, batch_terms=0, autotune=False,
-                 graphs=False):
+                 graphs=True):
         h = ctypes.c_void_p()
         cfg = L.EngineConfig(int(hbm_budget_bytes), int(streams), 1 if batch_builds else 0, int(batch_terms),
                              1 if autotune else 0, 1 if graphs else 0)

@@ -376,7 +376,8 @@ typedef struct dplus_engine dplus_engine;
 
 typedef struct {
     long long hbm_budget_bytes; /* admission budget; <= 0: 85 % of device memory */
-    int streams;                /* concurrent requests (1..16); <= 0: 16 */
+    int streams;                /* concurrent requests (1..32), one worker thread and HIP
+                                   stream each; <= 0: 16 */
     int batch_builds;           /* 1: mixed PDB builds of at most batch_terms terms that are
                                    ready together run as one grouped builder launch (up to
                                    16 grids of different structures and sizes) */

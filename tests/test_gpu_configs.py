@@ -196,18 +196,19 @@ def test_serving_stream_512_distinct_structures(dplus, oracle):
     _, reqs, _ = S.serving_workload(512)
     say(f"[case] serving: 512 requests, {sum(n for *_, n in reqs)} atoms in 512 distinct structures "
         f"(prepared in {time.time() - t0:.1f} s)")
-    say("  [step] mixed, one request at a time on one stream")
-    e1 = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=1)
+    say("  [step] mixed, one request at a time on one stream, eager launches")
+    e1 = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=1, graphs=False)
     alone = [e1.wait(e1.submit(q, precision=dplus.PRECISION_MIXED, **kw))[0] for q, kw, _ in reqs]
     e1.close()
-    say("  [step] mixed, all submitted at once on 16 streams")
-    e16 = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=16)
+    say("  [step] mixed, all submitted at once on 16 streams, one captured graph per request")
+    e16 = Engine(hbm_budget_bytes=200 * 10 ** 9, streams=16, graphs=True)
     tickets = [e16.submit(q, precision=dplus.PRECISION_MIXED, **kw) for q, kw, _ in reqs]
     mix = [e16.wait(t)[0] for t in tickets]
     st = e16.stats()
     e16.close()
-    say(f"    max concurrent {st['max_concurrent']}, cache hits {st['cache_hits']}")
-    assert st["max_concurrent"] > 1
+    say(f"    max concurrent {st['max_concurrent']}, cache hits {st['cache_hits']}, graph launches "
+        f"{st['graph_launches']} (updated {st['graph_updates']})")
+    assert st["max_concurrent"] > 1 and st["graph_launches"] == len(reqs)
     for i, (a, m) in enumerate(zip(alone, mix)):
         assert np.array_equal(a, m), i
     say("  [step] FP64, 16 streams")

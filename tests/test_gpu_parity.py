@@ -476,6 +476,30 @@ def test_autotune_shape_table(dplus):
         dplus.build_shapes_clear()
 
 
+@pytest.mark.parametrize("prec,gs,plain", [(0, 40, False), (0, 42, False), (1, 40, False), (1, 42, False),
+                                           (1, 40, True)])
+def test_build_writes_every_node(dplus, prec, gs, plain):
+    """A build writes every node of its grid and every spline coefficient (the
+    request engine builds into unzeroed grids): a grid whose nodes all hold NaN
+    before the build gives the bits of a build into a zeroed grid."""
+    say(f"[case] build over NaN nodes: precision {prec}, grid {gs}, plain FP64 builder {plain}")
+    xyz, ion, coef9, rad = structure(1500, 9)
+    dplus.set_fp64_builder(plain)
+    try:
+        ref = gpu_grid(dplus, gs, 4.0, prec, xyz, ion, coef9, rad)
+        ref.download()
+        g = dplus.Grid(gs, 4.0, prec)
+        g.data[:] = np.nan
+        g.upload()  # every node value NaN, so every spline coefficient but the origin's (always 0)
+        g.download()
+        assert np.isnan(g.data).all() and np.isnan(g.interp[2:]).all()
+        g.build_pdb(xyz, ion, coef9, rad, sol_ed=334.0, flags=dplus.CALC_ATOMIC_FORMFACTORS | dplus.CALC_DUMMY_SOLVENT)
+        g.download()
+    finally:
+        dplus.set_fp64_builder(False)
+    assert np.array_equal(g.data, ref.data) and np.array_equal(g.interp, ref.interp)
+
+
 def test_2000_atoms_grid100_500q_mc_and_gk_vs_debye(dplus, oracle):
     """The 2,000-atom workload as specified (3 nm sphere, solvent 333 e/nm^3,
     grid 100, qmax 5, 500 q, MC over 10^6 directions in chunks of 8,192, fixed

@@ -75,7 +75,7 @@ def test_engine_graph_requests_match_eager(dplus):
     from dplus_mi355x.serving import Engine
     say("[case] engine: 10 requests eager vs captured graphs (1 stream twice, then 4 streams)")
     reqs = requests(dplus)
-    e1 = Engine(streams=1)
+    e1 = Engine(streams=1, graphs=False)
     eager = [e1.wait(e1.submit(q, **kw)) for q, kw in reqs]
     e1.close()
     assert not any(r["graph"] for _, r in eager)
