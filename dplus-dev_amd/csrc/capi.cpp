@@ -399,11 +399,16 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
     const BoxSet sets[2] = {{d->box_contrast != 0.0 ? d->nbox : 0, d->box_dims, d->box_com, d->box_contrast},
                             {d->excl_contrast != 0.0 ? d->nexcl : 0, d->excl_dims, d->excl_com, d->excl_contrast}};
     const bool boxes_on = sets[0].n > 0 || sets[1].n > 0;
-    // atoms sorted by ion (ExtractRelevantCoeffs, PDBReaderLib.cpp:1443-1509), then
-    // each box set sorted by shape (SortSolventBoxEntry, PDBAmplitude.cpp:161-188)
+    // atoms sorted by ion (ExtractRelevantCoeffs, PDBReaderLib.cpp:1443-1509), stably:
+    // a counting sort over the ion index; then each box set sorted by shape
+    // (SortSolventBoxEntry, PDBAmplitude.cpp:161-188)
     std::vector<int> order(d->natoms);
-    for (int a = 0; a < d->natoms; a++) order[a] = a;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return d->ion[x] < d->ion[y]; });
+    {
+        std::vector<int> start(d->nion + 1, 0);
+        for (int a = 0; a < d->natoms; a++) start[d->ion[a] + 1]++;
+        for (int i = 0; i < d->nion; i++) start[i + 1] += start[i];
+        for (int a = 0; a < d->natoms; a++) order[start[d->ion[a]]++] = a;
+    }
     std::vector<int> border[2];
     for (int si = 0; si < 2; si++) {
         const BoxSet &B = sets[si];
@@ -495,11 +500,28 @@ Structure::Structure(const dplus_pdb_desc *d, hipStream_t s) {
     rmax = 0.0;
     for (size_t i = 0; i < nsrc; i++)
         rmax = std::max(rmax, std::sqrt(at64[i].x * at64[i].x + at64[i].y * at64[i].y + at64[i].z * at64[i].z));
-    b32.upload(at32.data(), at32.size() * sizeof(float4), s);
-    b64.upload(at64.data(), at64.size() * sizeof(double4), s);
-    bt.upload(tiles.data(), std::max<size_t>(1, tiles.size()) * sizeof(int4), s);
-    bd.upload(bdims.data(), bdims.size() * sizeof(int4), s);
-    bc.upload(bcon.data(), bcon.size() * sizeof(double), s);
+    // one allocation and one upload: [sources float4 | sources double4 | tiles | box shapes | contrasts]
+    if (tiles.empty()) tiles.push_back(make_int4(0, 0, 0, 0));
+    // (each section 256-byte aligned)
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o64 = al(at32.size() * sizeof(float4));
+    const size_t ot = o64 + al(at64.size() * sizeof(double4));
+    const size_t od = ot + al(tiles.size() * sizeof(int4));
+    const size_t oc = od + al(bdims.size() * sizeof(int4));
+    const size_t total = oc + bcon.size() * sizeof(double);
+    std::vector<char> host(total, 0);
+    memcpy(host.data(), at32.data(), at32.size() * sizeof(float4));
+    memcpy(host.data() + o64, at64.data(), at64.size() * sizeof(double4));
+    memcpy(host.data() + ot, tiles.data(), tiles.size() * sizeof(int4));
+    memcpy(host.data() + od, bdims.data(), bdims.size() * sizeof(int4));
+    memcpy(host.data() + oc, bcon.data(), bcon.size() * sizeof(double));
+    blob.upload(host.data(), total, s);
+    char *base = (char *)blob.p;
+    d32 = (const float4 *)base;
+    d64 = (const double4 *)(base + o64);
+    dtiles = (const int4 *)(base + ot);
+    ddims = (const int4 *)(base + od);
+    dcon = (const double *)(base + oc);
     check_hip(hipStreamSynchronize(s), "structure upload");
 }
 
@@ -552,13 +574,13 @@ BuildJob Structure::prepare(Grid &g, int slices, Pending &P, hipStream_t s) cons
     const int S = std::max(1, std::min({slices, kMaxSlices, std::max(ntiles, 1)}));
     P.part.ensure((size_t)S * 12 * nunits * sizeof(double));
     BuildJob J{};
-    J.atoms = (const float4 *)b32.p;
-    J.atoms64 = (const double4 *)b64.p;
+    J.atoms = d32;
+    J.atoms64 = d64;
     J.fp64 = g.fp64() ? 1 : 0;
-    J.tiles = (const int4 *)bt.p;
+    J.tiles = dtiles;
     J.ff = (const double *)P.ff.p;
-    J.bdims = (const int4 *)bd.p;
-    J.bcon = (const double *)bc.p;
+    J.bdims = ddims;
+    J.bcon = dcon;
     J.partial = (double *)P.part.p;
     J.grid = g.fp64() ? nullptr : (float4 *)g.dev.p;
     J.grid64 = g.fp64() ? (double4 *)g.dev.p : nullptr;
@@ -628,8 +650,8 @@ void Structure::build(Grid &g, bool host_copy, hipStream_t s, bool use_table) {
         KTimer kt("build", s);
         const double qtop = (double)(g.N + 1) * g.step;
         const bool reduce = qtop * rmax / k2Pi >= 128.0 || getenv("DPLUS_REDUCE_PHASE") != nullptr;
-        launch_build_atoms(g.fp64(), phase64, reduce, b32.p, b64.p, (const int4 *)bt.p, ntiles,
-                           (const double *)bff.p, desc.nion, (const int4 *)bd.p, (const double *)bc.p, desc.vox_step,
+        launch_build_atoms(g.fp64(), phase64, reduce, d32, d64, dtiles, ntiles,
+                           (const double *)bff.p, desc.nion, ddims, dcon, desc.vox_step,
                            g.step, nhalf, g.dev.p, desc.scale, s);
         check_hip(hipGetLastError(), "build_atoms launch");
     }

@@ -324,7 +324,13 @@ struct Structure {
     int ntiles = 0;
     std::vector<int> tile_len;  // terms of each tile
     long long nsources = 0;
-    DevBuf b32, b64, bt, bd, bc;  // immutable after construction
+    // device copies, immutable after construction: one allocation holding the
+    // sources in float and double, the tile table, box shapes and contrasts
+    DevBuf blob;
+    const float4 *d32 = nullptr;
+    const double4 *d64 = nullptr;
+    const int4 *dtiles = nullptr, *ddims = nullptr;
+    const double *dcon = nullptr;
     Structure(const dplus_pdb_desc *d, hipStream_t s);
     // use_table: the launch shape from the autotuner's table where it has the grid's
     // class (shapes timed on a device to themselves), else the occupancy model's

@@ -518,7 +518,7 @@ struct Engine {
                 }
         }
         std::shared_ptr<Structure> S(new Structure(&r.desc, s));
-        const size_t bytes = (size_t)(r.desc.natoms + r.desc.nbox + r.desc.nexcl) * 48 + (size_t)(S->ntiles + 1) * 16 + 4096;
+        const size_t bytes = S->blob.n + 4096;
         std::lock_guard<std::mutex> lk(cache_mu);
         cache_misses++;
         for (auto &c : cache)
