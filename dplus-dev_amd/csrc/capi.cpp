@@ -133,7 +133,7 @@ Device::Device() {
         ok = false;
         return;
     }
-    hipDeviceProp_t p;
+    hipDeviceProp_t p{};
     if (hipGetDeviceProperties(&p, id) == hipSuccess) {
         name = p.gcnArchName;
         cus = p.multiProcessorCount;
