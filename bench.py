@@ -571,10 +571,12 @@ def main():
     nhalf = 3 * (N + 1) ** 2 * N
     sources = len(W["xyz"]) + len(W["dims"])
     terms = float(nhalf) * sources
-    # executed per atom and phi / phi + pi node pair (two terms): q_x x + q_y y (3 flops), its sin/cos
-    # and four FMAs (8 flops); sin/cos of q_z z come from LDS, evaluated once per (row, atom) per
-    # workgroup, whose units span a few rows (under 2 % of the pair work, not counted)
-    flop_eq = 11.0 / 2.0 + w_t
+    # executed per atom and unit (three phi / phi + pi node pairs 120 degrees apart: six terms):
+    # q_x x + q_y y of two pairs (6 flops), their sin/cos (4 transcendentals), the third pair's
+    # e^{i u2} = conj(e^{i u0} e^{i u1}) (6 flops) and twelve FMAs (24 flops); sin/cos of q_z z
+    # come from LDS, evaluated once per (row, atom) per workgroup, whose units span a few rows
+    # (under 2 % of the work, not counted)
+    flop_eq = 36.0 / 6.0 + (4.0 / 6.0) * w_t
     t_launch = ms_build / max(nb, 1) * 1e-3
     model_achieved = terms * flop_eq / t_launch / 1e12
     # counted: the builder launch's VALU wave instructions from rocprofv3 --pmc of this

@@ -195,9 +195,14 @@ __global__ __launch_bounds__(256) void build_atoms_phase64(
 // z = q_z z_a and u = q_x x_a + q_y y_a their phases are z + u and z - u, and
 //   e^{i(z+u)} = (cz cu - sz su) + i (sz cu + cz su)
 //   e^{i(z-u)} = (cz cu + sz su) + i (sz cu - cz su).
-// One unit (thread) owns three such pairs; per atom it takes sin/cos of u once
-// per pair and sums the four products cz cu, sz su, sz cu, cz su: one
-// transcendental per term.  cz, sz depend only on the row and the atom, so a
+// One unit (thread) owns three such pairs at azimuths phi, phi + 2 pi / 3 and
+// phi + 4 pi / 3 (indices k0, k0 + 2i, k0 + 4i of the row's 6i: the unit's six
+// nodes are k0 + m i, m = 0..5).  Their in-plane phases sum to zero,
+// u0 + u1 + u2 = 0 (cos phi + cos(phi + 2 pi/3) + cos(phi + 4 pi/3) = 0, and the
+// same for sin), so e^{i u2} = conj(e^{i u0} e^{i u1}): per atom the unit takes
+// sin/cos of u0 and u1 only and forms the third pair's cos/sin with four
+// products, then sums the four products cz cu, sz su, sz cu, cz su of each
+// pair: two transcendentals per six terms.  cz, sz depend only on the row and the atom, so a
 // workgroup (whose kWg units span at most kRows rows for all but the smallest
 // shells) evaluates them once per (row, atom) into LDS, chunk by chunk,
 // double-buffered with one barrier per chunk; a workgroup spanning more rows
@@ -217,7 +222,8 @@ __global__ __launch_bounds__(256) void build_atoms_phase64(
 constexpr int kPair = 3;
 constexpr int kChunk = 64;
 
-// pair unit u -> its row and first azimuthal index
+// pair unit u -> its row and first azimuthal index (k0 < i: the unit's pairs
+// are at k0, k0 + 2i, k0 + 4i)
 struct PairUnit {
     int shell;
     long long j, k0;
@@ -228,33 +234,36 @@ __device__ __forceinline__ PairUnit pair_unit(long long u) {
     pu.shell = shell_of_pair_unit(u);
     const long long r = u - pair_units_before(pu.shell - 1);
     pu.j = r / pu.shell;
-    pu.k0 = 3 * (r % pu.shell);
+    pu.k0 = r % pu.shell;
     return pu;
 }
 
+// azimuthal index of pair p of a unit
+__device__ __forceinline__ long long pair_k(const PairUnit &pu, int p) { return pu.k0 + 2LL * pu.shell * p; }
+
 // the four nodes of pair p of a unit: (j, k), (j, k + 3i) and, off the
-// equator, their antipodes (3i - j, k + 3i), (3i - j, k)
+// equator, their antipodes (3i - j, k + 3i), (3i - j, k); azimuths mod 6i
 __device__ __forceinline__ void store_pair(float4 *__restrict__ grid, const PairUnit &pu, int p, double aR,
                                            double aI, double bR, double bI, double scale) {
-    const long long k = pu.k0 + p, half = 3LL * pu.shell;
+    const long long half = 3LL * pu.shell, k = pair_k(pu, p), kb = k + half >= 2 * half ? k - half : k + half;
     const float ar = (float)(aR * scale), ai = (float)(aI * scale);
     const float br = (float)(bR * scale), bi = (float)(bI * scale);
     grid[index_from_indices(pu.shell, pu.j, k)] = make_float4(ar, ai, 0.f, 0.f);
-    grid[index_from_indices(pu.shell, pu.j, k + half)] = make_float4(br, bi, 0.f, 0.f);
+    grid[index_from_indices(pu.shell, pu.j, kb)] = make_float4(br, bi, 0.f, 0.f);
     if (2 * pu.j != half) {
-        grid[index_from_indices(pu.shell, half - pu.j, k + half)] = make_float4(ar, -ai, 0.f, 0.f);
+        grid[index_from_indices(pu.shell, half - pu.j, kb)] = make_float4(ar, -ai, 0.f, 0.f);
         grid[index_from_indices(pu.shell, half - pu.j, k)] = make_float4(br, -bi, 0.f, 0.f);
     }
 }
 
 __device__ __forceinline__ void store_pair64(double4 *__restrict__ grid, const PairUnit &pu, int p, double aR,
                                              double aI, double bR, double bI, double scale) {
-    const long long k = pu.k0 + p, half = 3LL * pu.shell;
+    const long long half = 3LL * pu.shell, k = pair_k(pu, p), kb = k + half >= 2 * half ? k - half : k + half;
     const double ar = aR * scale, ai = aI * scale, br = bR * scale, bi = bI * scale;
     grid[index_from_indices(pu.shell, pu.j, k)] = make_double4(ar, ai, 0.0, 0.0);
-    grid[index_from_indices(pu.shell, pu.j, k + half)] = make_double4(br, bi, 0.0, 0.0);
+    grid[index_from_indices(pu.shell, pu.j, kb)] = make_double4(br, bi, 0.0, 0.0);
     if (2 * pu.j != half) {
-        grid[index_from_indices(pu.shell, half - pu.j, k + half)] = make_double4(ar, -ai, 0.0, 0.0);
+        grid[index_from_indices(pu.shell, half - pu.j, kb)] = make_double4(ar, -ai, 0.0, 0.0);
         grid[index_from_indices(pu.shell, half - pu.j, k)] = make_double4(br, -bi, 0.0, 0.0);
     }
 }
@@ -265,6 +274,53 @@ __device__ __forceinline__ int batch_job(const BuildBatch &b, int wg, bool finis
     for (int k = 1; k < b.njobs; k++)
         if (wg >= (finishing ? b.job[k].fin_begin : b.job[k].wg_begin)) jb = k;
     return jb;
+}
+
+// one atom's terms for a unit's three pairs: sin/cos of u0 and u1 (phases in
+// revolutions), e^{i u2} = conj(e^{i u0} e^{i u1}), then per pair the four
+// products with the row's cz, sz.  t2 = s0 c1 + c0 s1 = -sin u2.
+template <bool kReduce>
+__device__ __forceinline__ void pair_terms(const float qx[kPair], const float qy[kPair], const float4 &at, float cz,
+                                           float sz, float cc[kPair], float ss[kPair], float sc[kPair],
+                                           float cs[kPair]) {
+    float cu[2], su[2];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        const float ph = fmaf(qx[p], at.x, qy[p] * at.y);
+        const float v = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
+        cu[p] = __builtin_amdgcn_cosf(v);
+        su[p] = __builtin_amdgcn_sinf(v);
+        cc[p] = fmaf(cz, cu[p], cc[p]);
+        ss[p] = fmaf(sz, su[p], ss[p]);
+        sc[p] = fmaf(sz, cu[p], sc[p]);
+        cs[p] = fmaf(cz, su[p], cs[p]);
+    }
+    const float c2 = fmaf(cu[0], cu[1], -su[0] * su[1]);
+    const float t2 = fmaf(su[0], cu[1], cu[0] * su[1]);
+    cc[2] = fmaf(cz, c2, cc[2]);
+    ss[2] = fmaf(-sz, t2, ss[2]);
+    sc[2] = fmaf(sz, c2, sc[2]);
+    cs[2] = fmaf(-cz, t2, cs[2]);
+}
+
+__device__ __forceinline__ void pair_terms64(const double qx[kPair], const double qy[kPair], const double4 &at,
+                                             double cz, double sz, double cc[kPair], double ss[kPair],
+                                             double sc[kPair], double cs[kPair]) {
+    double cu[2], su[2];
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        sincos(fma(qx[p], at.x, qy[p] * at.y), &su[p], &cu[p]);
+        cc[p] = fma(cz, cu[p], cc[p]);
+        ss[p] = fma(sz, su[p], ss[p]);
+        sc[p] = fma(sz, cu[p], sc[p]);
+        cs[p] = fma(cz, su[p], cs[p]);
+    }
+    const double c2 = fma(cu[0], cu[1], -su[0] * su[1]);
+    const double t2 = fma(su[0], cu[1], cu[0] * su[1]);
+    cc[2] = fma(cz, c2, cc[2]);
+    ss[2] = fma(-sz, t2, ss[2]);
+    sc[2] = fma(sz, c2, sc[2]);
+    cs[2] = fma(-cz, t2, cs[2]);
 }
 
 // LDS rows of a workgroup of kWg units (kWg units of shell i span at most
@@ -296,7 +352,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs(BuildBatch batch) {
     const long long uc = live ? u : nunits - 1;
     const PairUnit pu = pair_unit(uc);
     const int shell = pu.shell;
-    const long long j = pu.j, k0 = pu.k0;
+    const long long j = pu.j;
     // rows of this workgroup: global row index of the first and last unit
     int lrow, nrows;
     {
@@ -320,7 +376,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs(BuildBatch batch) {
 #pragma unroll
         for (int p = 0; p < kPair; p++) {
             double sp, cp;
-            sincos(k2Pi * (double)(k0 + p) / (double)(6 * shell), &sp, &cp);
+            sincos(k2Pi * (double)pair_k(pu, p) / (double)(6 * shell), &sp, &cp);
             qx[p] = (float)(q * st * cp * inv2pi);
             qy[p] = (float)(q * st * sp * inv2pi);
             aR[p] = aI[p] = bR[p] = bI[p] = 0.0;
@@ -358,16 +414,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs(BuildBatch batch) {
                 for (int c = 0; c < n; c++) {
                     const float4 at = atoms[c0 + c];
                     const float2 z = rowz[buf][lrow][c];
-#pragma unroll
-                    for (int p = 0; p < kPair; p++) {
-                        const float ph = fmaf(qx[p], at.x, qy[p] * at.y);
-                        const float v = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
-                        const float cu = __builtin_amdgcn_cosf(v), su = __builtin_amdgcn_sinf(v);
-                        cc[p] = fmaf(z.x, cu, cc[p]);
-                        ss[p] = fmaf(z.y, su, ss[p]);
-                        sc[p] = fmaf(z.y, cu, sc[p]);
-                        cs[p] = fmaf(z.x, su, cs[p]);
-                    }
+                    pair_terms<kReduce>(qx, qy, at, z.x, z.y, cc, ss, sc, cs);
                 }
                 buf ^= 1;
             }
@@ -377,16 +424,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs(BuildBatch batch) {
                 const float4 at = atoms[a];
                 const float zz = kReduce ? __builtin_amdgcn_fractf(qz * at.z) : qz * at.z;
                 const float cz = __builtin_amdgcn_cosf(zz), sz = __builtin_amdgcn_sinf(zz);
-#pragma unroll
-                for (int p = 0; p < kPair; p++) {
-                    const float ph = fmaf(qx[p], at.x, qy[p] * at.y);
-                    const float v = kReduce ? __builtin_amdgcn_fractf(ph) : ph;
-                    const float cu = __builtin_amdgcn_cosf(v), su = __builtin_amdgcn_sinf(v);
-                    cc[p] = fmaf(cz, cu, cc[p]);
-                    ss[p] = fmaf(sz, su, ss[p]);
-                    sc[p] = fmaf(sz, cu, sc[p]);
-                    cs[p] = fmaf(cz, su, cs[p]);
-                }
+                pair_terms<kReduce>(qx, qy, at, cz, sz, cc, ss, sc, cs);
             }
         }
         // box tiles: contrast * prod_a L_a sinc(q_a L_a / 2); the z edge is common to the row
@@ -443,7 +481,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs64(BuildBatch batch) {
     const long long uc = live ? u : nunits - 1;
     const PairUnit pu = pair_unit(uc);
     const int shell = pu.shell;
-    const long long j = pu.j, k0 = pu.k0;
+    const long long j = pu.j;
     int lrow, nrows;
     {
         const int s0 = shell_of_pair_unit(u0);
@@ -465,7 +503,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs64(BuildBatch batch) {
 #pragma unroll
         for (int p = 0; p < kPair; p++) {
             double sp, cp;
-            sincos(k2Pi * (double)(k0 + p) / (double)(6 * shell), &sp, &cp);
+            sincos(k2Pi * (double)pair_k(pu, p) / (double)(6 * shell), &sp, &cp);
             qx[p] = q * st * cp;
             qy[p] = q * st * sp;
             aR[p] = aI[p] = bR[p] = bI[p] = 0.0;
@@ -499,15 +537,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs64(BuildBatch batch) {
                 for (int c = 0; c < n; c++) {
                     const double4 at = atoms[c0 + c];
                     const double2 z = rowz[buf][lrow][c];
-#pragma unroll
-                    for (int p = 0; p < kPair; p++) {
-                        double su, cu;
-                        sincos(fma(qx[p], at.x, qy[p] * at.y), &su, &cu);
-                        cc[p] = fma(z.x, cu, cc[p]);
-                        ss[p] = fma(z.y, su, ss[p]);
-                        sc[p] = fma(z.y, cu, sc[p]);
-                        cs[p] = fma(z.x, su, cs[p]);
-                    }
+                    pair_terms64(qx, qy, at, z.x, z.y, cc, ss, sc, cs);
                 }
                 buf ^= 1;
             }
@@ -516,15 +546,7 @@ __global__ __launch_bounds__(kWg) void build_atoms_pairs64(BuildBatch batch) {
                 const double4 at = atoms[a];
                 double sz, cz;
                 sincos(qz * at.z, &sz, &cz);
-#pragma unroll
-                for (int p = 0; p < kPair; p++) {
-                    double su, cu;
-                    sincos(fma(qx[p], at.x, qy[p] * at.y), &su, &cu);
-                    cc[p] = fma(cz, cu, cc[p]);
-                    ss[p] = fma(sz, su, ss[p]);
-                    sc[p] = fma(sz, cu, sc[p]);
-                    cs[p] = fma(cz, su, cs[p]);
-                }
+                pair_terms64(qx, qy, at, cz, sz, cc, ss, sc, cs);
             }
         }
 #pragma unroll
