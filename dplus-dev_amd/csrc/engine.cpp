@@ -196,6 +196,10 @@ struct Engine {
     std::mutex cache_mu;
     std::vector<Cached> cache;
     size_t cache_cap = 0, cache_bytes = 0;
+    // the slots' arenas: a fixed share of the budget (like the cache, kept out of
+    // request admission), split evenly over the slots; a request whose small
+    // buffers exceed its slot's share allocates them per request instead
+    size_t arena_cap = 0, slot_arena_max = 0;
     unsigned long long cache_tick = 0;
     long long cache_hits = 0, cache_misses = 0;
     int busy = 0;
@@ -255,6 +259,7 @@ struct Engine {
         d.require();
         budget = budget_bytes > 0 ? (size_t)budget_bytes : (size_t)(0.85 * (double)d.mem);
         cache_cap = std::min<size_t>(budget / 16, (size_t)8 << 30);
+        arena_cap = std::min<size_t>(budget / 8, (size_t)1 << 30);
         batch_builds = batch;
         autotune = tune;
         graphs = graph;
@@ -270,6 +275,7 @@ struct Engine {
             bthread = std::thread([this] { batcher(); });
         }
         slots.resize(nstreams);
+        slot_arena_max = arena_cap / (size_t)nstreams;
         for (int i = 0; i < nstreams; i++) workers.emplace_back([this, i] { work(i); });
     }
 
@@ -431,7 +437,7 @@ struct Engine {
         while (!pending.empty()) {
             Request *r = pending.front();
             if ((int)admitted.size() + busy >= (int)streams.size()) break;
-            if (reserved + r->need + cache_cap > budget) break;
+            if (reserved + r->need + cache_cap + arena_cap > budget) break;
             pending.pop_front();
             reserved += r->need;
             peak_reserved = std::max(peak_reserved, reserved);
@@ -603,9 +609,10 @@ struct Engine {
 
     // the slot's arena sized for dev_bytes of buffers and host_bytes of staged
     // uploads, emptied for the next request (its previous request has finished)
-    static void slot_arena(Slot &sl, size_t dev_bytes, size_t host_bytes) {
+    void slot_arena(Slot &sl, size_t dev_bytes, size_t host_bytes) const {
         if (dev_bytes > sl.arena_dev.n)
-            sl.arena_dev.ensure(std::max(dev_bytes, std::max<size_t>(2 * sl.arena_dev.n, (size_t)4 << 20)));
+            sl.arena_dev.ensure(std::min(slot_arena_max,
+                                         std::max(dev_bytes, std::max<size_t>(2 * sl.arena_dev.n, (size_t)4 << 20))));
         if (host_bytes > sl.pin_cap) {
             std::shared_lock<CaptureMutex> lk(capture_mutex());
             if (sl.arena_pin) (void)hipHostFree(sl.arena_pin);
@@ -736,18 +743,20 @@ struct Engine {
             size_t up = 0;
             const size_t need =
                 oa_arena_bytes(1, leaf.nrot, ntr, nq, r.q.data(), g->step, &r.oa, r.want_chi, &up);
-            slot_arena(sl, need, up);
+            // (rounded up as slot_arena grows it; past the slot's share: allocated per request)
+            const bool in_arena = std::max(need, (size_t)4 << 20) <= slot_arena_max;
+            if (in_arena) slot_arena(sl, need, up);
             std::unique_ptr<DockSet> Dp;
             FixedAverage F;
             {
                 struct Install {
                     explicit Install(Arena *a) { set_arena(a); }
                     ~Install() { set_arena(nullptr); }
-                } install(&sl.arena);
+                } install(in_arena ? &sl.arena : nullptr);
                 Dp.reset(new DockSet(1, &leaf, s));
                 F.prepare(*Dp, nq, r.q.data(), &r.oa, r.want_chi ? &chi : nullptr, s);
             }
-            sl.arena.flush(s);
+            if (in_arena) sl.arena.flush(s);
             const DockSet &D = *Dp;
             slot_staging(sl, (size_t)nq + 1);
             // the chain's shape: kernels (and so the graph's nodes) follow from these
@@ -1019,7 +1028,7 @@ int dplus_engine_submit(dplus_engine *eh, const dplus_request *rq, long long *ti
         }
         r->oa = rq->oa;
         r->need = request_bytes(*r);
-        if (r->need + e->cache_cap > e->budget)
+        if (r->need + e->cache_cap + e->arena_cap > e->budget)
             throw Error(kErrOutOfMemory, "the request needs " + std::to_string(r->need) +
                                              " device bytes, more than the engine's budget of " +
                                              std::to_string(e->budget));
