@@ -198,7 +198,7 @@ def plain_curve(D, S, steps=5):
 
 
 def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, streams=16, fp64=True,
-                   batch_builds=False, graphs=True, compare_modes=True, repeats=3):
+                   batch_builds=False, graphs=True, compare_modes=True, repeats=5):
     """Batched serving stream: n_requests with Poisson arrivals at `rate`/s; each
     request is its own seeded random-walk structure (or one of `pool_size`; atoms
     log-uniform in 500..200,000 at 50 atoms/nm^3, dummy-atom solvent), qmax
@@ -244,7 +244,7 @@ def serving_stream(D, S, n_requests=512, rate=2000.0, pool_size=None, seed=4, st
     order = [mode] + ([("eager" if graphs else "graphs")] if compare_modes else [])
     saturated, loaded_modes = {}, {}
     for m in order:
-        # the saturated window is short (~0.5 s): the run of median span out of `repeats`
+        # the saturated window is short (~0.4 s): the run of median span out of `repeats`
         runs = sorted((run_mixed(arrivals, m == "graphs") for _ in range(repeats)), key=lambda x: x[1])
         res_m, span_m, lat_m, st_m = runs[len(runs) // 2]
         saturated[m] = summary(span_m, lat_m, st_m)

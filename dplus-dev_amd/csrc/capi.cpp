@@ -995,8 +995,7 @@ void FixedAverage::prepare(const DockSet &D, int nq_, const double *q, const dpl
     }
 }
 
-void FixedAverage::enqueue(const DockSet &D, double *I, double *chi2, hipStream_t s, bool timed) const {
-    const double *dI;
+void FixedAverage::enqueue_kernels(const DockSet &D, hipStream_t s, bool timed) const {
     if (method == DPLUS_OA_MC) {
         {
             std::unique_ptr<KTimer> kt(timed ? new KTimer("mc", s) : nullptr);
@@ -1007,7 +1006,6 @@ void FixedAverage::enqueue(const DockSet &D, double *I, double *chi2, hipStream_
         check_hip(hipGetLastError(), "mc launch");
         launch_mc_finish((const GroupDev *)bg.p, ng, mstride, (const double *)bout.p, (double)total, (double *)bI.p, s);
         check_hip(hipGetLastError(), "mc finish launch");
-        dI = (const double *)bI.p;
     } else {
         launch_fill_zero((double *)bh.p, (size_t)iters * nq, s);
         check_hip(hipGetLastError(), "fill launch");
@@ -1020,14 +1018,17 @@ void FixedAverage::enqueue(const DockSet &D, double *I, double *chi2, hipStream_
             check_hip(hipGetLastError(), "vegas launch");
         }
         // every group runs every iteration: the result is the last iteration's running mean
-        dI = (const double *)bh.p + (size_t)(iters - 1) * nq;
     }
     if (chi) {
-        launch_chi2(dI, (const double *)by.p, (const double *)bsig.p, nq, chi_scale, chi_constant, (double *)bchi.p, s);
+        launch_chi2(result(), (const double *)by.p, (const double *)bsig.p, nq, chi_scale, chi_constant,
+                    (double *)bchi.p, s);
         check_hip(hipGetLastError(), "chi2 launch");
-        check_hip(hipMemcpyAsync(chi2, bchi.p, sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
     }
-    check_hip(hipMemcpyAsync(I, dI, nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+}
+
+void FixedAverage::download(double *I, double *chi2, hipStream_t s) const {
+    if (chi) check_hip(hipMemcpyAsync(chi2, bchi.p, sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+    check_hip(hipMemcpyAsync(I, result(), nq * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
 }
 
 void orientation_average(const DockSet &D, int nq, const double *q, const dplus_oa_params *p, double *out,

@@ -407,7 +407,17 @@ struct FixedAverage {
     // I: nq doubles, chi2: one double (with a chi^2 request), written when the
     // stream reaches the downloads (pinned host memory for a captured graph);
     // timed: HIP-event kernel timing (never while capturing)
-    void enqueue(const DockSet &D, double *I, double *chi2, hipStream_t s, bool timed) const;
+    void enqueue(const DockSet &D, double *I, double *chi2, hipStream_t s, bool timed) const {
+        enqueue_kernels(D, s, timed);
+        download(I, chi2, s);
+    }
+    // the kernels alone, and the downloads alone
+    void enqueue_kernels(const DockSet &D, hipStream_t s, bool timed) const;
+    void download(double *I, double *chi2, hipStream_t s) const;
+    // device intensities the kernels leave
+    const double *result() const {
+        return method == DPLUS_OA_MC ? (const double *)bI.p : (const double *)bh.p + (size_t)(iters - 1) * nq;
+    }
 };
 void gk_average(const DockSet &D, int nq, const double *q, double eps, long long max_depth, double *out,
                 dplus_oa_stats *st, hipStream_t s);
