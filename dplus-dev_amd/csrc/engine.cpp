@@ -614,11 +614,11 @@ struct Engine {
             sl.arena_dev.ensure(std::min(slot_arena_max,
                                          std::max(dev_bytes, std::max<size_t>(2 * sl.arena_dev.n, (size_t)4 << 20))));
         if (host_bytes > sl.pin_cap) {
+            const size_t cap = std::max(host_bytes, std::max<size_t>(2 * sl.pin_cap, (size_t)1 << 20));
             std::shared_lock<CaptureMutex> lk(capture_mutex());
             if (sl.arena_pin) (void)hipHostFree(sl.arena_pin);
             sl.arena_pin = nullptr;
             sl.pin_cap = 0;
-            const size_t cap = std::max(host_bytes, std::max<size_t>(2 * sl.pin_cap, (size_t)1 << 20));
             check_hip(hipHostMalloc((void **)&sl.arena_pin, cap, hipHostMallocDefault), "hipHostMalloc");
             sl.pin_cap = cap;
         }
