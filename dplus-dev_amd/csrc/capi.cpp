@@ -728,7 +728,7 @@ void pdb_build(Grid &g, const dplus_pdb_desc *d, bool host_copy, hipStream_t s) 
 
 // ------------------------------------------------------------------ orientation averages
 void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
-               const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
+               const ZCopyDev *zc, int nz, const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
                double step, uint64_t seed, long long dir0, int nblk, int chunk, double *partial, double *out,
                hipStream_t s);
 void launch_vegas_iter(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
@@ -773,6 +773,46 @@ void euler_rt(double a, double b, double c, double m[9]) {
                           -(c2 * s3), c1 * c3 - s1 * s2 * s3, c3 * s1 + c1 * s2 * s3,
                           s2, -(c2 * s1), c1 * c2};
     for (int i = 0; i < 9; i++) m[i] = rt[i];
+}
+
+bool z_copies(const RotDev *rots, int nrot, const double4 *tr, std::vector<ZCopyDev> &out) {
+    // DPLUS_ZCOPY=0 keeps such dockings on the general kernel (for A/B checks)
+    const char *env = getenv("DPLUS_ZCOPY");
+    if (env && std::strcmp(env, "0") == 0) return false;
+    std::vector<ZCopyDev> zc;
+    for (int r = 0; r < nrot; r++) {
+        const double *m = rots[r].m;
+        // R^T = Rz(alpha)^T = [[c, s, 0], [-s, c, 0], [0, 0, 1]]: the z row and
+        // column exactly as Euler angles (0, 0, alpha) give them, so the general
+        // lookup would take the direction's own polar angle
+        if (m[2] != 0.0 || m[5] != 0.0 || m[6] != 0.0 || m[7] != 0.0 || m[8] != 1.0) return false;
+        if (std::fabs(m[3] + m[1]) > 1e-12 || std::fabs(m[4] - m[0]) > 1e-12) return false;
+        double a = std::atan2(m[1], m[0]);
+        if (a < 0.0) a += 2.0 * kPi;
+        if (a >= 2.0 * kPi) a = 0.0;
+        for (int t = 0; t < rots[r].nt; t++) {
+            const double4 v = tr[rots[r].t0 + t];
+            zc.push_back(ZCopyDev{a, {v.x, v.y, v.z}, rots[r].scale, rots[r].leaf, 0});
+        }
+    }
+    std::stable_sort(zc.begin(), zc.end(), [](const ZCopyDev &x, const ZCopyDev &y) {
+        return x.leaf != y.leaf ? x.leaf < y.leaf : x.alpha < y.alpha;
+    });
+    // a chunk of 64 copies spanning d alpha covers at most d alpha * 6 l / (2 pi) + 3
+    // nodes of a shell-l row (its two ends and the last node's right neighbour):
+    // within the 64-node window while l <= 61 * 2 pi / (6 d alpha); one node
+    // of margin for rounding.  Chunks spanning two leaves take the direct
+    // lookup in the kernel and do not bound it.
+    double lfit = 1e9;
+    for (size_t c0 = 0; c0 < zc.size(); c0 += 64) {
+        const size_t last = std::min(zc.size() - 1, c0 + 63);
+        if (zc[c0].leaf != zc[last].leaf) continue;
+        const double da = zc[last].alpha - zc[c0].alpha;
+        if (da > 0.0) lfit = std::min(lfit, std::floor(60.0 * 2.0 * kPi / (6.0 * da)));
+    }
+    if (!zc.empty()) zc[0].aux = (int)std::min(lfit, 1e9);
+    out.swap(zc);
+    return true;
 }
 
 // GridSymmetry::OrganizeParameters/PreCalculate (DefaultModels/Symmetries.cpp:29-97):
@@ -923,6 +963,12 @@ void DockSet::finish(std::vector<Grid *> &gl, std::vector<RotDev> &rots, std::ve
     geos.upload(geo.data(), geo.size() * sizeof(LeafGeo), s);
     rotb.upload(rots.data(), std::max<size_t>(1, rots.size()) * sizeof(RotDev), s);
     trb.upload(tr.data(), tr.size() * sizeof(double4), s);
+    std::vector<ZCopyDev> zc;
+    nz = 0;
+    if (!single && z_copies(rots.data(), nrot, tr.data(), zc) && (int)zc.size() >= kZCopyMin) {
+        nz = (int)zc.size();
+        zcb.upload(zc.data(), zc.size() * sizeof(ZCopyDev), s);
+    }
 }
 
 void launch_mc_finish(const GroupDev *groups, int ngroups, int mstride, const double *out, double total, double *I,
@@ -1000,7 +1046,7 @@ void FixedAverage::enqueue_kernels(const DockSet &D, hipStream_t s, bool timed) 
         {
             std::unique_ptr<KTimer> kt(timed ? new KTimer("mc", s) : nullptr);
             launch_mc(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
-                      (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
+                      (const double4 *)D.trb.p, (const ZCopyDev *)D.zcb.p, D.nz, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride,
                       (const double *)bq.p, D.step, seed, 0, nblk, chunk, (double *)bpart.p, (double *)bout.p, s);
         }
         check_hip(hipGetLastError(), "mc launch");
@@ -1080,7 +1126,7 @@ void orientation_average(const DockSet &D, int nq, const double *q, const dplus_
             {
                 KTimer kt("mc", s);
                 launch_mc(D.fp64, D.single, (const void *const *)D.grids.p, (const RotDev *)D.rotb.p, D.nrot,
-                          (const double4 *)D.trb.p, (const GroupDev *)bg.p, (const int *)bact.p,
+                          (const double4 *)D.trb.p, (const ZCopyDev *)D.zcb.p, D.nz, (const GroupDev *)bg.p, (const int *)bact.p,
                           (int)active.size(), mstride, (const double *)bq.p, D.step, p->seed, k * chunk, nblk,
                           chunk, (double *)bpart.p, (double *)bout.p, s);
             }
@@ -1187,7 +1233,8 @@ size_t oa_arena_bytes(int nleaf, long long nrot, long long ntr, int nq, const do
     };
     size_t b = upl((size_t)nleaf * sizeof(void *)) + upl((size_t)nleaf * sizeof(LeafGeo)) +
                upl((size_t)std::max<long long>(1, nrot) * sizeof(RotDev)) +
-               upl((size_t)std::max<long long>(1, ntr) * sizeof(double4));
+               upl((size_t)std::max<long long>(1, ntr) * sizeof(double4)) +
+               upl((size_t)std::max<long long>(1, ntr) * sizeof(ZCopyDev));
     if (nq > 0) {
         const std::vector<GroupDev> groups = make_groups(q, nq, step);
         const size_t ng = groups.size();

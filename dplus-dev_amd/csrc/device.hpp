@@ -165,6 +165,29 @@ struct RotDev {
 // R^T of R = Rx(a) Ry(b) Rz(c), row-major
 void euler_rt(double a, double b, double c, double m[9]);
 
+// One placed copy of a leaf turned about z alone (R = Rz(alpha)): R^T q has the
+// polar angle of q and the azimuth phi - alpha, so the copy's lookup is the
+// leaf's grid at (theta, phi - alpha).  One record per translation; alpha in
+// [0, 2 pi).  aux of the first record: the largest shell whose row segments
+// fit the copy-major kernel's collapse window for every chunk of 64 records
+// (the shell groups up to it take the collapsed lookup).
+struct ZCopyDev {
+    double alpha;
+    double t[3];
+    double scale;
+    int leaf, aux;
+};
+
+// the copies of rotation records that all turn about z alone, one per
+// translation, ordered by (leaf, alpha) so that neighbouring records look up
+// neighbouring nodes of the same theta rows, with out[0].aux set; false (out
+// untouched) when a record has another rotation
+bool z_copies(const RotDev *rots, int nrot, const double4 *tr, std::vector<ZCopyDev> &out);
+
+// the copy-major MC kernel takes a docking whose copies all turn about z when
+// it has at least this many of them (a wave's lanes are its copies)
+constexpr int kZCopyMin = 32;
+
 // one shell group: q-values q0..q0+m-1 lie in (layer*step, (layer+1)*step]
 struct GroupDev {
     int layer, q0, m, pad;
@@ -236,7 +259,10 @@ struct DockSet {
     // one leaf placed once, unrotated, at the origin: the orientation-average
     // kernels look up the drawn angles directly
     bool single = false;
-    DevBuf grids, geos, rotb, trb;
+    // nz > 0: every copy turns about z alone; zcb holds the nz ZCopyDev records
+    // (z_copies) for the copy-major MC kernel
+    int nz = 0;
+    DevBuf grids, geos, rotb, trb, zcb;
     DockSet(int nleaf, const dplus_leaf *leaves, hipStream_t s);
     // placements sharing grid, rotation and scale become one rotation record
     // with several translations

@@ -55,7 +55,7 @@
 namespace dplus {
 
 void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot, const double4 *trans,
-               const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
+               const ZCopyDev *zc, int nz, const GroupDev *groups, const int *active, int nactive, int mstride, const double *qs,
                double step, uint64_t seed, long long dir0, int nblk, int chunk, double *partial, double *out,
                hipStream_t s);
 void launch_mc_finish(const GroupDev *groups, int ngroups, int mstride, const double *out, double total, double *I,
@@ -171,7 +171,7 @@ size_t request_bytes(const Request &r) {
     } else {
         b += gk_device_bytes(r.q.size());
     }
-    b += (r.rot.size() / 4 + 1) * sizeof(RotDev) + (r.trans.size() / 3 + 1) * sizeof(double4);
+    b += (r.rot.size() / 4 + 1) * sizeof(RotDev) + (r.trans.size() / 3 + 1) * (sizeof(double4) + sizeof(ZCopyDev));
     return b;
 }
 
@@ -844,24 +844,26 @@ struct Session {
     int chunk = 0;  // MC stratification period (directions)
     unsigned long long seed = 0;
     std::vector<GroupDev> groups;
-    DevBuf bgrids, bq, bg, bact, bpart, bout, bI, by, bsig, bchi, bparam, brot, btr;
+    DevBuf bgrids, bq, bg, bact, bpart, bout, bI, by, bsig, bchi, bparam, brot, btr, bz;
     size_t cap_rot = 0, cap_tr = 0;
     RotDev *h_rot = nullptr;
     double4 *h_tr = nullptr;
+    ZCopyDev *h_z = nullptr;  // cap_tr records: the copies when all turn about z
     double *h_param = nullptr, *h_out = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int graph_nrot = -1;
+    int graph_nrot = -1, graph_nz = -1;
     long long evaluations = 0, captures = 0;
 
     ~Session() {
         if (s) (void)hipStreamSynchronize(s);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        for (void *p : {(void *)h_rot, (void *)h_tr, (void *)h_param, (void *)h_out})
+        for (void *p : {(void *)h_rot, (void *)h_tr, (void *)h_z, (void *)h_param, (void *)h_out})
             if (p) host_free(p);
         bgrids.release(), bq.release(), bg.release(), bact.release(), bpart.release(), bout.release();
         bI.release(), by.release(), bsig.release(), bchi.release(), bparam.release(), brot.release(), btr.release();
+        bz.release();
         if (s) {
             (void)hipStreamSynchronize(s);
             (void)hipStreamDestroy(s);
@@ -887,14 +889,17 @@ struct Session {
         }
         if (ntr > cap_tr) {
             if (h_tr) host_free(h_tr);
+            if (h_z) host_free(h_z);
             cap_tr = std::max<size_t>(ntr, 2 * cap_tr);
             host_alloc((void **)&h_tr, cap_tr * sizeof(double4));
+            host_alloc((void **)&h_z, cap_tr * sizeof(ZCopyDev));
             btr.ensure(cap_tr * sizeof(double4));
+            bz.ensure(cap_tr * sizeof(ZCopyDev));
             graph_nrot = -1;
         }
     }
 
-    void capture(int nrot) {
+    void capture(int nrot, int nz) {
         if (exec) (void)hipGraphExecDestroy(exec), exec = nullptr;
         if (graph) (void)hipGraphDestroy(graph), graph = nullptr;
         // no allocation anywhere in the process while this stream captures
@@ -912,9 +917,11 @@ struct Session {
         } end_on_throw{s};
         check_hip(hipMemcpyAsync(brot.p, h_rot, cap_rot * sizeof(RotDev), hipMemcpyHostToDevice, s), "H2D");
         check_hip(hipMemcpyAsync(btr.p, h_tr, cap_tr * sizeof(double4), hipMemcpyHostToDevice, s), "H2D");
+        if (nz > 0)
+            check_hip(hipMemcpyAsync(bz.p, h_z, cap_tr * sizeof(ZCopyDev), hipMemcpyHostToDevice, s), "H2D");
         check_hip(hipMemcpyAsync(bparam.p, h_param, 2 * sizeof(double), hipMemcpyHostToDevice, s), "H2D");
         launch_mc(fp64, false, (const void *const *)bgrids.p, (const RotDev *)brot.p, nrot, (const double4 *)btr.p,
-                  (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride, (const double *)bq.p, step, seed, 0, nblk,
+                  (const ZCopyDev *)bz.p, nz, (const GroupDev *)bg.p, (const int *)bact.p, ng, mstride, (const double *)bq.p, step, seed, 0, nblk,
                   chunk, (double *)bpart.p, (double *)bout.p, s);
         launch_mc_finish((const GroupDev *)bg.p, ng, mstride, (const double *)bout.p, (double)total, (double *)bI.p, s);
         launch_chi2_param((const double *)bI.p, (const double *)by.p, (const double *)bsig.p, nq,
@@ -928,6 +935,7 @@ struct Session {
         graph = gph;
         check_hip(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
         graph_nrot = nrot;
+        graph_nz = nz;
         captures++;
     }
 };
@@ -1244,7 +1252,14 @@ int dplus_session_evaluate(dplus_session *sh, const dplus_leaf *placements, int 
         }
         S->h_param[0] = fit_scale;
         S->h_param[1] = fit_constant;
-        if (S->graph_nrot != (int)nrot) S->capture((int)nrot);
+        // copies that all turn about z run the copy-major MC kernel (a graph of its own)
+        std::vector<ZCopyDev> zc;
+        int nz = 0;
+        if (z_copies(S->h_rot, (int)nrot, S->h_tr, zc) && (int)zc.size() >= kZCopyMin) {
+            nz = (int)zc.size();
+            std::copy(zc.begin(), zc.end(), S->h_z);
+        }
+        if (S->graph_nrot != (int)nrot || S->graph_nz != nz) S->capture((int)nrot, nz);
         check_hip(hipGraphLaunch(S->exec, S->s), "graph launch");
         check_hip(hipStreamSynchronize(S->s), "session evaluation");
         if (intensity) std::copy(S->h_out, S->h_out + S->nq, intensity);

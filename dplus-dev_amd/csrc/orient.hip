@@ -307,6 +307,304 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T
     }
 }
 
+// ------------------------------------------------------------------ MC, copies turned about z
+// lane i's value of v, for a wave-uniform i
+__device__ __forceinline__ double lane_value(double v, int i) {
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, i);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), i);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// the sum over the wave's 64 lanes, in every lane: DPP exchanges within each row
+// of 16 lanes (quad swaps, half-row and row mirrors), then the four row sums
+__device__ __forceinline__ float wave_total(float v) {
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x141, 0xF, 0xF, false));
+    v += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x140, 0xF, 0xF, false));
+    const int b = __float_as_int(v);
+    return (__int_as_float(__builtin_amdgcn_readlane(b, 0)) + __int_as_float(__builtin_amdgcn_readlane(b, 16))) +
+           (__int_as_float(__builtin_amdgcn_readlane(b, 32)) + __int_as_float(__builtin_amdgcn_readlane(b, 48)));
+}
+__device__ __forceinline__ double wave_total(double v) { return wave_sum(v); }
+
+// weights of the 4-point spline (four_point's end derivatives, then the cubic
+// Hermite at t) over values y0..y3: the spline at t is sum_r w[r] y[r]
+template <typename T>
+__device__ __forceinline__ void spline4_weights(T t, T w[4]) {
+    const T a = (T)(7. / 15.), b = (T)(14. / 15.), c = (T)(4. / 15.), e = (T)(2. / 15.);
+    const T t2 = t * t, t3 = t2 * t;
+    const T h00 = (T)1 - (T)3 * t2 + (T)2 * t3, h01 = (T)3 * t2 - (T)2 * t3, h10 = t - (T)2 * t2 + t3,
+            h11 = t3 - t2;
+    w[0] = h10 * (a - b) + h11 * (c - e);
+    w[1] = h00 + h10 * (c - a) + h11 * (e - b);
+    w[2] = h01 + h10 * (b - e) + h11 * (a - c);
+    w[3] = h10 * (e - c) + h11 * (b - a);
+}
+
+// the four planes' amplitudes (layer - 1 .. layer + 2) of one lookup at (th, ph),
+// as group_planes takes them, without the q derivatives
+template <typename T, typename T4>
+__device__ __forceinline__ void direct_planes(const T4 *__restrict__ g, int layer, double th, double ph,
+                                              cplx<T> amps[4]) {
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        const int lqi = layer - 1 + p;
+        if (lqi == -1) {
+            double nth, nph;
+            antipode(th, ph, nth, nph);
+            amps[p] = plane_lookup<T, T4, false>(g + 1, 1, nth * kInvPi, nph * kInv2Pi);
+        } else if (lqi == 0) {
+            const T4 o = g[0];
+            amps[p] = {o.x, o.y};
+        } else {
+            amps[p] = plane_lookup<T, T4, false>(g + shell_base(lqi), lqi, th * kInvPi, ph * kInv2Pi);
+        }
+    }
+}
+
+// direct_planes kept out of line, for the copy-major kernel's rare chunks whose
+// segments do not fit: its registers then cost only on that path
+template <typename T, typename T4>
+__device__ __noinline__ void direct_planes_call(const T4 *__restrict__ g, int layer, double th, double ph,
+                                                cplx<T> *amps) {
+    direct_planes<T, T4>(g, layer, th, ph, amps);
+}
+
+constexpr int kZWin = 64;  // nodes of one plane's collapsed row segment (one per lane)
+
+// The MC average of a docking whose copies all turn about z (a helix): every
+// copy of a direction looks up the direction's own theta rows, at phi - alpha.
+// A block takes the same 256 directions (and writes the same partial sums) as
+// mc_kernel, but a wave's lanes are copies: it walks its 64 directions one at a
+// time (each broadcast from the lane that drew it) and, per direction, its
+// copies 64 at a time in (leaf, alpha) order, so the copies of one chunk span a
+// short segment of each theta row.  kCollapse (FP32 grids): the lookup is
+// linear in the node data and theta is the same for every copy, so per plane
+// the 4-row theta spline is applied once to the segment (at most kZWin nodes,
+// one per lane, its 16 row loads issued together and coalesced) into LDS, and
+// each copy takes the phi Hermite of two collapsed nodes; a segment that does
+// not fit, or a chunk spanning two leaves, takes the direct lookup, which
+// every copy takes without kCollapse.  The q spline is applied with per-q
+// weights.  The copies' amplitudes are summed over the lanes per direction,
+// then |A_j|^2.
+template <typename T, typename T4, bool kCollapse, int M>
+__global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void mc_zcopy_kernel(
+    const T4 *const *__restrict__ grids, const ZCopyDev *__restrict__ zc, int nz,
+    const GroupDev *__restrict__ groups, const int *__restrict__ active, int mstride,
+    const double *__restrict__ qs, double step, uint64_t seed, long long dir0, int nblk, int chunk, int nu, int nv,
+    bool split, double *__restrict__ partial) {
+    const int gi = blockIdx.y;
+    const GroupDev G = groups[active[gi]];
+    // split: the collapse instance takes the shell groups whose planes (up to
+    // shell layer + 2) fit the window, the direct instance the others
+    const bool collapse_group = G.layer + 2 <= zc[0].aux;
+    if (kCollapse ? !collapse_group : (split && collapse_group)) return;
+    const long long d = dir0 + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double u, v;
+    uniform_pair(seed, (uint32_t)d, (uint32_t)(d >> 32), (uint32_t)G.layer, kStreamMC, u, v);
+    mc_stratify(d, chunk, nu, nv, u, v);
+    // the wave's directions (theta, phi, unit vector), read back one at a time
+    __shared__ double dirs[4][5][64];
+    __shared__ double red[4][M];
+    __shared__ T wq[M][4];
+    __shared__ T wts[4][4][4];  // per wave and plane: the theta spline weights of the current direction
+    __shared__ T4 win[4][4][kCollapse ? kZWin : 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    {
+        const double th = acos(2.0 * u - 1.0), ph = k2Pi * v;
+        double st, ct, sp, cp;
+        sincos(th, &st, &ct);
+        sincos(ph, &sp, &cp);
+        dirs[w][0][lane] = th;
+        dirs[w][1][lane] = ph;
+        dirs[w][2][lane] = cp * st;
+        dirs[w][3][lane] = sp * st;
+        dirs[w][4][lane] = ct;
+    }
+    const double inv_step = 1.0 / step;
+    for (int jb = 0; jb < G.m; jb += M) {
+        const int m = min(M, G.m - jb);
+        const double *q = qs + G.q0 + jb;
+        if (threadIdx.x < m) spline4_weights<T>((T)((q[threadIdx.x] - G.layer * step) * inv_step), wq[threadIdx.x]);
+        __syncthreads();
+        double mine = 0.0;  // lane j < m: the wave's sum over its directions of |A_j|^2
+        for (int i = 0; i < 64; i++) {
+            const double thi = dirs[w][0][i], phi = dirs[w][1][i];
+            const double qv0 = dirs[w][2][i], qv1 = dirs[w][3][i], qv2 = dirs[w][4][i];
+            // per plane: theta (the antipode's for layer -1), its row, and (lane p) its spline weights
+            int tI[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const int lqi = G.layer - 1 + p;
+                const double tp = lqi == -1 ? fmin(fmax(kPi - thi, 0.0), kPi) : thi;
+                const int shell = lqi <= 0 ? 1 : lqi;
+                const double xt = tp * kInvPi * (double)(3 * shell);
+                tI[p] = (int)xt;
+                if (kCollapse && lane == p) spline4_weights<T>((T)(xt - (double)tI[p]), wts[w][p]);
+            }
+            if (kCollapse) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+            cplx<T> acc[M];
+#pragma unroll
+            for (int j = 0; j < M; j++) acc[j] = {(T)0, (T)0};
+            // lanes past the last copy of a chunk repeat it with scale 0
+            ZCopyDev zn = zc[min(lane, nz - 1)];
+            for (int c0 = 0; c0 < nz; c0 += 64) {
+                const int last = min(63, nz - 1 - c0);
+                const ZCopyDev z = zn;
+                if (c0 + 64 < nz) zn = zc[min(c0 + 64 + lane, nz - 1)];
+                const T sc = lane <= last ? (T)z.scale : (T)0;
+                double nph = phi - z.alpha;
+                if (nph < 0.0) nph += k2Pi;
+                cplx<T> amps[4];
+                if (kCollapse) {
+                    const int leaf0 = __builtin_amdgcn_readfirstlane(z.leaf);
+                    bool all_fit = __builtin_amdgcn_readlane(z.leaf, last) == leaf0;
+                    int off[4], kl[4], nseg[4];
+                    T tph[4];
+                    // each plane's segment: from the last copy's node (largest alpha)
+                    // forward to the first copy's node + 1, cyclically
+#pragma unroll
+                    for (int p = 0; p < 4; p++) {
+                        const int lqi = G.layer - 1 + p;
+                        const int shell = lqi <= 0 ? 1 : lqi;
+                        const int P = 6 * shell;
+                        double pp = nph;
+                        if (lqi < 0) {  // antipode(): phi + pi, wrapped
+                            pp = nph + kPi;
+                            if (pp >= k2Pi) pp = pp - (int)(pp / k2Pi) * k2Pi;
+                        }
+                        const double xp = (pp * kInv2Pi) * (double)P;
+                        int pi = (int)xp;
+                        tph[p] = (T)(xp - (double)pi);
+                        pi = (pi == P) ? 0 : pi;
+                        const int kf = __builtin_amdgcn_readfirstlane(pi);
+                        kl[p] = __builtin_amdgcn_readlane(pi, last);
+                        int span = kf - kl[p];
+                        if (span < 0) span += P;
+                        int o = pi - kl[p];
+                        if (o < 0) o += P;
+                        off[p] = o;
+                        nseg[p] = lqi == 0 ? 0 : span + 2;
+                        if (lqi != 0) all_fit = all_fit && span + 2 <= kZWin && __ballot(o > span) == 0;
+                    }
+                    if (all_fit) {
+                        const T4 *g0 = grids[leaf0];
+                        // two planes at a time: their 8 row loads issued together, then
+                        // the theta spline of each segment node into LDS
+#pragma unroll
+                        for (int h = 0; h < 2; h++) {
+                            T4 nd[2][4];
+#pragma unroll
+                            for (int e = 0; e < 2; e++) {
+                                const int p = 2 * h + e;
+                                const int lqi = G.layer - 1 + p;
+                                const int shell = lqi <= 0 ? 1 : lqi;
+                                const int P = 6 * shell, half = 3 * shell, thePoints = 3 * shell;
+                                // this lane's segment node (a valid node for lanes past the segment)
+                                const bool mine_node = lane < nseg[p];
+                                int k = kl[p] + lane;
+                                if (k >= P) k -= P;
+                                const T4 *plane = g0 + shell_base(shell);
+#pragma unroll
+                                for (int r = 0; r < 4; r++) {
+                                    const int vr = tI[p] + r - 1;
+                                    int pos;
+                                    if (vr < 0 || vr > thePoints) {
+                                        const int theInd = vr < 0 ? vr + thePoints + 1 : vr - (thePoints + 1);
+                                        pos = P * theInd + k + ((2 * k >= P) ? -half : half);
+                                    } else {
+                                        pos = P * vr + k;
+                                    }
+                                    nd[e][r] = plane[mine_node ? pos : 0];
+                                }
+                            }
+#pragma unroll
+                            for (int e = 0; e < 2; e++) {
+                                const int p = 2 * h + e;
+                                if (lane < nseg[p]) {
+                                    T wt[4];
+#pragma unroll
+                                    for (int r = 0; r < 4; r++) wt[r] = wts[w][p][r];
+                                    T4 cn;
+                                    cn.x = wt[0] * nd[e][0].x;
+                                    cn.y = wt[0] * nd[e][0].y;
+                                    cn.z = wt[0] * nd[e][0].z;
+                                    cn.w = wt[0] * nd[e][0].w;
+#pragma unroll
+                                    for (int r = 1; r < 4; r++) {
+                                        cn.x += wt[r] * nd[e][r].x;
+                                        cn.y += wt[r] * nd[e][r].y;
+                                        cn.z += wt[r] * nd[e][r].z;
+                                        cn.w += wt[r] * nd[e][r].w;
+                                    }
+                                    win[w][p][lane] = cn;
+                                }
+                            }
+                            __builtin_amdgcn_wave_barrier();
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        __builtin_amdgcn_wave_barrier();
+                        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+                        for (int p = 0; p < 4; p++) {
+                            if (nseg[p] == 0) {
+                                const T4 o = g0[0];
+                                amps[p] = {o.x, o.y};
+                            } else {
+                                const T4 A = win[w][p][off[p]], B = win[w][p][off[p] + 1];
+                                amps[p] = hermite<T>({A.x, A.y}, {B.x, B.y}, {A.z, A.w}, {B.z, B.w}, tph[p]);
+                            }
+                        }
+                    } else {
+                        direct_planes_call<T, T4>(grids[z.leaf], G.layer, thi, nph, amps);
+                    }
+                } else {
+                    direct_planes<T, T4>(grids[z.leaf], G.layer, thi, nph, amps);
+                }
+#pragma unroll
+                for (int p = 0; p < 4; p++) {
+                    amps[p].x *= sc;
+                    amps[p].y *= sc;
+                }
+                const double dot = z.t[0] * qv0 + z.t[1] * qv1 + z.t[2] * qv2;
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    if (j < m) {
+                        const T w0 = wq[j][0], w1 = wq[j][1], w2 = wq[j][2], w3 = wq[j][3];
+                        const T ax = w0 * amps[0].x + w1 * amps[1].x + w2 * amps[2].x + w3 * amps[3].x;
+                        const T ay = w0 * amps[0].y + w1 * amps[1].y + w2 * amps[2].y + w3 * amps[3].y;
+                        T s, co;
+                        phase_sincos<T>(q[j] * dot, s, co);
+                        acc[j].x += ax * co - ay * s;
+                        acc[j].y += ay * co + ax * s;
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                if (j < m) {
+                    const double ax = (double)wave_total(acc[j].x), ay = (double)wave_total(acc[j].y);
+                    if (lane == j) mine += ax * ax + ay * ay;
+                }
+            }
+        }
+        if (lane < m) red[w][lane] = mine;
+        __syncthreads();
+        if (threadIdx.x < m) {
+            const int j = threadIdx.x;
+            partial[((size_t)gi * nblk + blockIdx.x) * mstride + jb + j] =
+                red[0][j] + red[1][j] + red[2][j] + red[3][j];
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ gather ceiling
 // The node reads of the mixed MC average without its arithmetic: for every
 // (direction, layer) of an MC launch (the same stratified Philox directions) and
@@ -1025,12 +1323,34 @@ void launch_gk_tree_sums(GkNode *nodes, int lo, int hi, int nroots, double *phiv
 
 // ------------------------------------------------------------------ launchers
 void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *rots, int nrot,
-               const double4 *trans, const GroupDev *groups, const int *active, int nactive,
-               int mstride, const double *qs, double step, uint64_t seed, long long dir0, int nblk, int chunk,
-               double *partial, double *out, hipStream_t s) {
+               const double4 *trans, const ZCopyDev *zc, int nz, const GroupDev *groups, const int *active,
+               int nactive, int mstride, const double *qs, double step, uint64_t seed, long long dir0, int nblk,
+               int chunk, double *partial, double *out, hipStream_t s) {
     dim3 grid(nblk, nactive);
     int nu, nv;
     mc_strata(chunk, nu, nv);
+    if (!single && zc && nz >= kZCopyMin) {
+        // 8 q per pass when no shell group holds more (fewer accumulators);
+        // FP32 grids: the collapse instance and the direct one over the same
+        // grid, each block running in one of them
+#define DPLUS_MCZ(T, T4, C, M, SPLIT)                                                                             \
+    mc_zcopy_kernel<T, T4, C, M><<<grid, 256, 0, s>>>((const T4 *const *)grids, zc, nz, groups, active, mstride, \
+                                                      qs, step, seed, dir0, nblk, chunk, nu, nv, SPLIT, partial)
+        if (fp64 && mstride <= 8) {
+            DPLUS_MCZ(double, double4, false, 8, false);
+        } else if (fp64) {
+            DPLUS_MCZ(double, double4, false, kMaxM, false);
+        } else if (mstride <= 8) {
+            DPLUS_MCZ(float, float4, true, 8, true);
+            DPLUS_MCZ(float, float4, false, 8, true);
+        } else {
+            DPLUS_MCZ(float, float4, true, kMaxM, true);
+            DPLUS_MCZ(float, float4, false, kMaxM, true);
+        }
+#undef DPLUS_MCZ
+        mc_reduce<<<nactive, 256, 0, s>>>(groups, active, partial, nblk, mstride, out);
+        return;
+    }
 #define DPLUS_MC(T, T4, S)                                                                                      \
     mc_kernel<T, T4, S><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups, active, mstride, \
                                              qs, step, seed, dir0, nblk, chunk, nu, nv, partial)

@@ -292,6 +292,51 @@ def test_hybrid_docking_mc_matches_oracle(dplus, oracle, small, prec):
     assert np.max(np.abs(mine - ref) / ref) < (FP64_TOL if prec == 0 else MIXED_TOL)
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("case", ["two_leaves", "sparse"])
+def test_zcopy_docking_mc_matches_oracle(dplus, oracle, small, prec, case, monkeypatch):
+    """Copies that all turn about z run the copy-major MC kernel: against the
+    oracle's docked MC, against the general kernel on the same input
+    (DPLUS_ZCOPY=0), and in chunked launches.  two_leaves: a 70-copy helix whose
+    rows repeat at a second height (two translations per rotation) and a
+    37-copy helix of another twist, 177 copies (not a multiple of the wave, one
+    chunk spanning both leaves); every shell group takes the collapsed lookup.
+    sparse: 33 copies 137.5 deg apart, one chunk around the whole circle, so
+    only the low shell groups fit the collapse window and the others take the
+    direct lookup."""
+    say(f"[case] z-copy docking MC {case} precision={prec}")
+    import dplus_mi355x.synth as S
+    s = small
+    g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
+    if case == "two_leaves":
+        a = S.helix_rows(70, 3.0, 0.9, 27.7)
+        b = a.copy()
+        b[:, 2] += 0.45
+        rotA, ntA, trA = S.rows_to_docking(np.concatenate([a, b]))
+        assert (ntA == 2).all()
+        rotB, ntB, trB = S.rows_to_docking(S.helix_rows(37, 2.0, 0.6, 41.0), scale=0.7)
+        leaves = [(g, rotA, ntA, trA), (g, rotB, ntB, trB)]
+    else:
+        rot, nt, tr = S.rows_to_docking(S.helix_rows(33, 2.5, 0.3, 137.5))
+        leaves = [(g, rot, nt, tr)]
+    q = np.linspace(0.05, 4.0, 50)
+    mine, _ = dplus.orientation_average(dplus.Docking(leaves), q, seed=2, max_dirs=2048, chunk=1024)
+    ref, _ = oracle.mc_oa(oracle.Docked([(s["gs"], s["qmax"], s["data"], s["co"], r, n, t)
+                                         for _, r, n, t in leaves]), q, 2, 2048, 1024)
+    e = np.max(np.abs(mine - ref) / ref)
+    monkeypatch.setenv("DPLUS_ZCOPY", "0")
+    gen, _ = dplus.orientation_average(dplus.Docking(leaves), q, seed=2, max_dirs=2048, chunk=1024)
+    monkeypatch.delenv("DPLUS_ZCOPY")
+    # chunked MC (eps > 0): launches with a direction offset
+    ch, st = dplus.orientation_average(dplus.Docking(leaves), q, seed=2, max_dirs=2048, chunk=1024, eps=1e-12)
+    ab = np.max(np.abs(mine - gen) / gen)
+    say(f"    vs oracle {e:.2e}, vs general kernel {ab:.2e}, chunked {np.max(np.abs(ch - mine) / mine):.2e}")
+    assert e < (FP64_TOL if prec == 0 else MIXED_TOL)
+    assert ab < (1e-12 if prec == 0 else 2e-6)
+    assert st.dirs_used == 2048
+    np.testing.assert_allclose(ch, mine, rtol=1e-12 if prec == 0 else 1e-6)
+
+
 def test_dock_grid_matches_oracle(dplus, oracle, small):
     say("[case] grid-level manual symmetry")
     import dplus_mi355x.synth as S

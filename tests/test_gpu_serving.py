@@ -166,23 +166,29 @@ def test_fit_session_graph_replays(dplus):
     sig = np.full(80, 0.1)
     dirs = 16384
     sess = FitSession([g], q, y, sig, seed=4, directions=dirs)
-    cases = [(12, 2.0, 0.8, 30.0), (12, 2.2, 0.8, 31.5), (12, 2.0, 0.8, 30.0), (20, 1.8, 0.7, 25.0)]
+    # 48 copies turned about z alone run the copy-major MC kernel; the same count
+    # tilted about x goes back to the general kernel (a capture of its own)
+    cases = [(12, 2.0, 0.8, 30.0, 0.0), (12, 2.2, 0.8, 31.5, 0.0), (12, 2.0, 0.8, 30.0, 0.0),
+             (20, 1.8, 0.7, 25.0, 0.0), (48, 2.0, 0.8, 30.0, 0.0), (48, 2.1, 0.8, 29.0, 0.0),
+             (48, 2.0, 0.8, 30.0, 5.0)]
     prev = {}
-    for n, rad, rise, twist in cases:
-        rot, nt, tr = S.rows_to_docking(S.helix_rows(n, rad, rise, twist))
+    for n, rad, rise, twist, tilt in cases:
+        rows = S.helix_rows(n, rad, rise, twist)
+        rows[:, 3] = tilt
+        rot, nt, tr = S.rows_to_docking(rows)
         I, c2 = sess.evaluate([(rot, nt, tr)], fit_scale=1e-5, fit_constant=0.2)
         eager, _ = dplus.orientation_average(dplus.Docking([(g, rot, nt, tr)]), q, seed=4, max_dirs=dirs,
                                              chunk=8192)
         assert np.array_equal(I, eager)
         ref = float(np.sum(((1e-5 * eager + 0.2 - y) / sig) ** 2))
         assert c2 == pytest.approx(ref, rel=1e-12)
-        key = (n, rad, rise, twist)
+        key = (n, rad, rise, twist, tilt)
         if key in prev:
             assert np.array_equal(prev[key], I)
         prev[key] = I
     info = sess.info()
     say(f"    session {info}")
-    assert info["evaluations"] == 4 and info["graph_captures"] == 2
+    assert info["evaluations"] == 7 and info["graph_captures"] == 4
     sess.close()
 
 
