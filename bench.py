@@ -123,48 +123,84 @@ def cpu_baseline(W, g64_data, g64_interp, shells=(8, 16, 24)):
                       f"{total:.0f} s"}
 
 
-def docking_curve(D, S, directions=2_000_000):
+def docking_curve(D, S, mb, directions=2_000_000):
     """The docking workload: a 5,000-atom subunit (2.5 nm sphere) on a grid of 200
     (qmax 6) docked as a 1,000-copy helix (radius 12 nm, rise 0.9 nm, twist 27.7
     deg), 800 q on [0, 6], MC over `directions` directions (chunks of 8,192), one
-    curve in mixed precision and one in FP64, plus the gather ceiling of the
-    docked lookups on the same grid and rotations."""
+    curve in mixed precision and one in FP64.  The helix's copies all turn about z,
+    so the copy-major kernel runs it; the general kernel (DPLUS_ZCOPY=0) is timed
+    on the same curve beside it, with the gather ceiling of its lookups."""
     xyz, el = S.walk_structure(3, 5000, (2.5, 2.5, 2.5))
     coef9, rad, ion, _ = S.structure_tables(el)
     rot, nt, tr = S.rows_to_docking(S.helix_rows(1000, 12.0, 0.9, 27.7))
     q = np.linspace(0.0, 6.0, 800)
     st = D.Structure(xyz, ion, coef9, rad, sol_ed=333.0, flags=3)
     out = {}
+
+    def average(name, g, zcopy=True):
+        # the docked leaves are flattened (and the kernel chosen) inside the average call
+        if not zcopy:
+            os.environ["DPLUS_ZCOPY"] = "0"
+        try:
+            dock = D.Docking([(g, rot, nt, tr)])
+            D.timing_reset()
+            D.timing_enable(True)
+            sync()
+            t0 = time.perf_counter()
+            I, stt = D.orientation_average(dock, q, seed=5, max_dirs=directions, chunk=8192)
+            sync()
+            dt = time.perf_counter() - t0
+            D.timing_enable(False)
+        finally:
+            os.environ.pop("DPLUS_ZCOPY", None)
+        out[name] = (I, dt, D.timing_get("mc")[1], stt.dirs_used, g)
+        log(f"[docking] {name}: {stt.dirs_used} directions in {dt:.2f} s (MC kernels {out[name][2]:.0f} ms)")
+
     for name, prec in (("mixed", D.PRECISION_MIXED), ("fp64", D.PRECISION_FP64)):
         g = D.Grid(200, 6.0, prec)
         st.build(g)
-        D.timing_reset()
-        D.timing_enable(True)
-        sync()
-        t0 = time.perf_counter()
-        I, stt = D.orientation_average(D.Docking([(g, rot, nt, tr)]), q, seed=5, max_dirs=directions, chunk=8192)
-        sync()
-        dt = time.perf_counter() - t0
-        D.timing_enable(False)
-        out[name] = (I, dt, D.timing_get("mc")[1], stt.dirs_used, g)
-        log(f"[docking] {name}: {stt.dirs_used} directions in {dt:.2f} s (MC kernels {out[name][2]:.0f} ms)")
+        average(name, g)
+    average("mixed_general", out["mixed"][4], zcopy=False)
     ngroups = len(np.unique(np.maximum(np.ceil(q / out["mixed"][4].step_size) - 1, 0)))
     lookups = float(out["mixed"][3]) * ngroups * len(rot)
     rate = lookups * 512 / (out["mixed"][2] * 1e-3)
+    rate_general = lookups * 512 / (out["mixed_general"][2] * 1e-3)
     pms, plk = D.gather_probe(out["mixed"][4], 32768, 0, ngroups, rot)
     ceiling = plk * 512 / (pms * 1e-3)
     err = float(np.max(np.abs(out["mixed"][0] - out["fp64"][0]) / np.abs(out["fp64"][0])))
-    log(f"[docking] mixed vs FP64 {err:.2e}; docked lookups {rate / 1e12:.2f} TB/s of nodes, gather ceiling "
-        f"{ceiling / 1e12:.2f} TB/s")
+    err_general = float(np.max(np.abs(out["mixed"][0] - out["mixed_general"][0]) / np.abs(out["mixed_general"][0])))
+    log(f"[docking] mixed vs FP64 {err:.2e}, copy-major vs general kernel {err_general:.2e}; node bytes "
+        f"{rate / 1e12:.2f} TB/s copy-major, {rate_general / 1e12:.2f} TB/s general (its gather ceiling "
+        f"{ceiling / 1e12:.2f} TB/s)")
+    # counted: the copy-major launch's VALU wave instructions from rocprofv3 --pmc of this
+    # workload at 32,768 directions (profiles/docking_zcopy_valu.json), scaled to the
+    # directions run; their issue time at the microbenchmarked rates (FP32-rate, transcendental,
+    # FP64) over the live kernel time
+    roofline = None
+    vjs = os.path.join(ROOT, "profiles", "docking_zcopy_valu.json")
+    if os.path.exists(vjs):
+        P = json.load(open(vjs))
+        wi = P["wave_instructions"]
+        k = float(out["mixed"][3]) / P["directions"]
+        other = wi["valu"] - wi["valu_trans_f32"] - wi["valu_fp64"]
+        issue_s = k * 64.0 * (other / mb["fma32"] + wi["valu_trans_f32"] / mb["sin32"] + wi["valu_fp64"] / mb["fma64"])
+        t = out["mixed"][2] * 1e-3
+        roofline = {"bound": "valu", "kernel": P["kernel"], "achieved": k * wi["valu"] / t,
+                    "peak": k * wi["valu"] / issue_s, "unit": "VALU wave instructions/s (this kernel's mix)",
+                    "frac": issue_s / t, "valu_per_64_copies": wi["valu"] / (P["units"] / 64.0)}
+        log(f"[docking] copy-major VALU issue {issue_s * 1e3:.0f} ms of {t * 1e3:.0f} ms ({issue_s / t:.2f})")
     return {"workload": "5,000-atom subunit on grid 200 (qmax 6) docked as a 1,000-copy helix (R 12 nm, rise 0.9 nm, "
                         "twist 27.7 deg), 800 q, MC 2e6 directions",
             "value": 1.0 / out["mixed"][1], "unit": "curves/s", "fp64_value": 1.0 / out["fp64"][1],
+            "general_kernel_value": 1.0 / out["mixed_general"][1],
             "speedup_vs_fp64": out["fp64"][1] / out["mixed"][1], "max_rel_err_vs_fp64": err,
-            "directions": int(out["mixed"][3]),
-            "roofline": {"bound": "gather", "kernel": "mc_kernel<float, float4, false>",
-                         "achieved": rate / 1e9, "peak": ceiling / 1e9, "unit": "GB/s of nodes",
-                         "frac": rate / ceiling, "lookups": lookups, "bytes_per_lookup": 512,
-                         "ceiling": "dplus_gather_probe: the same lookups' node loads alone, 1,000 rotations"}}
+            "max_rel_err_vs_general_kernel": err_general,
+            "directions": int(out["mixed"][3]), "roofline": roofline,
+            "gather": {"kernel": "mc_kernel<float, float4, false> (the general kernel, DPLUS_ZCOPY=0)",
+                       "node_GBps_general": rate_general / 1e9, "node_GBps_copy_major": rate / 1e9,
+                       "gather_ceiling_GBps": ceiling / 1e9, "general_frac": rate_general / ceiling,
+                       "lookups": lookups, "bytes_per_lookup": 512,
+                       "ceiling": "dplus_gather_probe: the general kernel's lookups' node loads alone, 1,000 rotations"}}
 
 
 def plain_curve(D, S, steps=5):
@@ -471,7 +507,7 @@ def main():
         ng3 = len(np.unique(np.maximum(np.ceil(q3 / g3.step_size) - 1, 0)))
         d_ms, d_lk = D.gather_probe(g3, 32768, 0, ng3, rot)
         print(json.dumps({"probe": "build_atoms_pairs + copy_probe_kernel + vegas_sample + mc_kernel + gather_probe "
-                                   "(50k-atom grid 250), mc_kernel docked + gather_probe<true> (1,000-copy helix)",
+                                   "(50k-atom grid 250), mc_zcopy_kernel + gather_probe<true> (1,000-copy helix)",
                           "copy_bytes": nbytes, "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9,
                           "shell_groups": ngroups,
                           "vegas_ms": vegas_ms, "vegas_units": W["iters"] * 1024 * W["ppb"] * ngroups,
@@ -606,7 +642,7 @@ def main():
         cpu = cpu_baseline(W, g64.data.copy(), g64.interp.copy())
         log(f"[bench] cpu baseline: {cpu}")
 
-    docking = None if args.no_docking else docking_curve(D, S)
+    docking = None if args.no_docking else docking_curve(D, S, mb)
     plain = plain_curve(D, S)
 
     serving = sessions = None
