@@ -688,10 +688,10 @@ def main():
                      "transcendental_weight": w_t,
                      "achieved_from": "counted VALU instructions (rocprofv3 --pmc)" if counted else "flop model",
                      "counted": counted, "model_achieved": model_achieved,
-                     # section 8(d)'s unit: 8 flops + 2 transcendentals per (node, source) term over all
-                     # P nodes (the builder does half the nodes by Friedel symmetry and shares one sin/cos
-                     # pair between the phi and phi + pi nodes, so this exceeds the peak)
-                     "survey_unit_achieved": float(O_total_points(W["gs"])) * sources * (8.0 + 2.0 * w_t) / t_launch / 1e12,
+                     # the direct sum's unit: 8 flops + 2 transcendentals per (node, source) term over
+                     # all P nodes (the builder does half the nodes by Friedel symmetry and shares sin/cos
+                     # pairs between the unit's nodes, so this exceeds the peak)
+                     "direct_sum_unit_achieved": float(O_total_points(W["gs"])) * sources * (8.0 + 2.0 * w_t) / t_launch / 1e12,
                      # the same launch against the HBM roofline: PMC bytes per launch over its time
                      "hbm": None if traffic is None else {
                          "achieved": traffic / t_launch / 1e9, "peak": PEAK_HBM_GBPS, "unit": "GB/s",

@@ -1,8 +1,9 @@
 /*
  * oracle.c -- FP64 CPU restatement of D+'s scattering hot path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only the
- * tests, the smoke check and the benchmark's CPU-baseline leg may load it.
+ * TEST INFRASTRUCTURE ONLY: the FP64 reference the HIP implementation is
+ * tested against.  Only the tests, the smoke check and the benchmark's
+ * CPU-baseline leg may load it.
  * Every function restates the reference algorithm named in its comment
  * (paths relative to the reference tree) in plain double precision.
  *

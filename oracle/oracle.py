@@ -2,8 +2,8 @@
 build (oracle/_ref/libgridref.so).
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py as the checker; the product package never imports
-it.  Each wrapper names the oracle function it calls; the reference citations
+CPU-baseline leg of bench.py to compare results with; the product package never
+imports it.  Each wrapper names the oracle function it calls; the reference citations
 live beside those functions in oracle.c.
 """
 import ctypes

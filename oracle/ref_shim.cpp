@@ -1,8 +1,8 @@
 // ref_shim.cpp -- C entry points over the reference's own JacobianSphereGrid
 // (Backend/Backend/Grid.cpp), compiled unmodified from the reference tree by
-// oracle/Makefile into oracle/_ref/.  TEST INFRASTRUCTURE ONLY: it is the
-// checker for the oracle's grid, spline and lookup restatement and is never
-// linked into the product library.
+// oracle/Makefile into oracle/_ref/.  TEST INFRASTRUCTURE ONLY: the tests
+// compare the oracle's grid, spline and lookup restatement with it; it is
+// never linked into the product library.
 #include "Grid.h"
 
 #include <vector>

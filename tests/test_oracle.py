@@ -1,4 +1,4 @@
-"""Pin the FP64 oracle (oracle/oracle.c) before it is used as the checker.
+"""Pin the FP64 oracle (oracle/oracle.c) before the GPU tests compare against it.
 
 Each test anchors one oracle function on something outside the oracle: known
 answer vectors, closed forms, the reference's own Grid.cpp compiled in
