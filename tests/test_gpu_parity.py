@@ -293,7 +293,7 @@ def test_hybrid_docking_mc_matches_oracle(dplus, oracle, small, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("case", ["two_leaves", "sparse"])
+@pytest.mark.parametrize("case", ["two_leaves", "sparse", "one_copy_chunk"])
 def test_zcopy_docking_mc_matches_oracle(dplus, oracle, small, prec, case, monkeypatch):
     """Copies that all turn about z run the copy-major MC kernel: against the
     oracle's docked MC, against the general kernel on the same input
@@ -303,7 +303,8 @@ def test_zcopy_docking_mc_matches_oracle(dplus, oracle, small, prec, case, monke
     chunk spanning both leaves); every shell group takes the collapsed lookup.
     sparse: 33 copies 137.5 deg apart, one chunk around the whole circle, so
     only the low shell groups fit the collapse window and the others take the
-    direct lookup."""
+    direct lookup.  one_copy_chunk: 65 copies, the second chunk holding one
+    copy (a segment of two nodes, 63 lanes repeating it with scale 0)."""
     say(f"[case] z-copy docking MC {case} precision={prec}")
     import dplus_mi355x.synth as S
     s = small
@@ -316,8 +317,11 @@ def test_zcopy_docking_mc_matches_oracle(dplus, oracle, small, prec, case, monke
         assert (ntA == 2).all()
         rotB, ntB, trB = S.rows_to_docking(S.helix_rows(37, 2.0, 0.6, 41.0), scale=0.7)
         leaves = [(g, rotA, ntA, trA), (g, rotB, ntB, trB)]
-    else:
+    elif case == "sparse":
         rot, nt, tr = S.rows_to_docking(S.helix_rows(33, 2.5, 0.3, 137.5))
+        leaves = [(g, rot, nt, tr)]
+    else:
+        rot, nt, tr = S.rows_to_docking(S.helix_rows(65, 2.5, 0.3, 27.7))
         leaves = [(g, rot, nt, tr)]
     q = np.linspace(0.05, 4.0, 50)
     mine, _ = dplus.orientation_average(dplus.Docking(leaves), q, seed=2, max_dirs=2048, chunk=1024)
