@@ -861,7 +861,14 @@ std::vector<GroupDev> make_groups(const double *q, int nq, double step) {
         while (q[q0] > (double)(layer + 1) * step) layer++;
         int e = q0 + 1;
         while (e < nq && !(q[e] > (double)(layer + 1) * step)) e++;
-        gs.push_back({layer, q0, e - q0, 0});
+        int uniform = 0;
+        if (e - q0 > 1) {
+            const double dq = (q[e - 1] - q[q0]) / (double)(e - 1 - q0), tol = 1e-12 * std::fabs(q[e - 1]);
+            uniform = 1;
+            for (int j = q0; j < e; j++)
+                if (std::fabs(q[j] - (q[q0] + (double)(j - q0) * dq)) > tol) uniform = 0;
+        }
+        gs.push_back({layer, q0, e - q0, uniform});
         q0 = e;
         layer++;
     }

@@ -188,9 +188,12 @@ bool z_copies(const RotDev *rots, int nrot, const double4 *tr, std::vector<ZCopy
 // it has at least this many of them (a wave's lanes are its copies)
 constexpr int kZCopyMin = 32;
 
-// one shell group: q-values q0..q0+m-1 lie in (layer*step, (layer+1)*step]
+// one shell group: q-values q0..q0+m-1 lie in (layer*step, (layer+1)*step];
+// uniform = 1 when they are equally spaced (every q_j within 1e-12 of
+// q_0 + j (q_{m-1} - q_0) / (m - 1), relative to the largest), so the mixed
+// kernels step a translation's phase from one q to the next
 struct GroupDev {
-    int layer, q0, m, pad;
+    int layer, q0, m, uniform;
 };
 
 // the 15 Kronrod abscissae on [-1, 1] with the Kronrod and Gauss weights

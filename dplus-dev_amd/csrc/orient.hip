@@ -151,15 +151,21 @@ __device__ __forceinline__ void phase_sincos<double>(double qd, double &s, doubl
 constexpr int kMaxM = 16;  // q-values of one shell group handled per pass (one lookup serves all)
 
 
-// accumulate the docked amplitude along unit direction qv for q-values qs[0..m), m <= M
+// accumulate the docked amplitude along unit direction qv for q-values qs[0..m), m <= M.
+// stepq (equally spaced q, single precision): the translations of a rotation
+// with several step their phases in q, e^{i q_j d} = e^{i q_0 d} (e^{i dq d})^j:
+// two range-reduced sin/cos per translation and one complex rotation per further
+// q instead of one sin/cos per q (a single translation keeps the direct form)
 template <typename T, typename T4, int M = kMaxM>
 __device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ grids,
                                                  const RotDev *__restrict__ rots, int nrot,
                                                  const double4 *__restrict__ trans, int layer,
                                                  double step, const double qv[3],
-                                                 const double *__restrict__ qs, int m,
+                                                 const double *__restrict__ qs, int m, bool stepq,
                                                  cplx<T> acc[M]) {
     const double inv_step = 1.0 / step;
+    const bool rec = sizeof(T) == 4 && stepq && m > 1;
+    const double dq = rec ? (qs[m - 1] - qs[0]) / (double)(m - 1) : 0.0;
     for (int r = 0; r < nrot; r++) {
         const RotDev rd = rots[r];
         double rv[3];
@@ -173,6 +179,37 @@ __device__ __forceinline__ void docked_amplitude(const T4 *const *__restrict__ g
         const T sc = (T)rd.scale;
         const bool trivial = (rd.nt == 1) && trans[rd.t0].x == 0.0 && trans[rd.t0].y == 0.0 &&
                              trans[rd.t0].z == 0.0;
+        if (rec && rd.nt > 1) {
+            cplx<T> a[M];
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                a[j] = {(T)0, (T)0};
+                if (j < m) {
+                    const T t = (T)((qs[j] - layer * step) * inv_step);
+                    a[j] = hermite<T>(amps[1], amps[2], d1, d2, t);
+                    a[j].x *= sc;
+                    a[j].y *= sc;
+                }
+            }
+            for (int tr = 0; tr < rd.nt; tr++) {
+                const double4 tt = trans[rd.t0 + tr];
+                const double d = tt.x * qv[0] + tt.y * qv[1] + tt.z * qv[2];
+                T cs, sn, cd, sd;
+                phase_sincos<T>(qs[0] * d, sn, cs);
+                phase_sincos<T>(dq * d, sd, cd);
+#pragma unroll
+                for (int j = 0; j < M; j++) {
+                    if (j < m) {
+                        acc[j].x += a[j].x * cs - a[j].y * sn;
+                        acc[j].y += a[j].y * cs + a[j].x * sn;
+                        const T c2 = cs * cd - sn * sd;
+                        sn = sn * cd + cs * sd;
+                        cs = c2;
+                    }
+                }
+            }
+            continue;
+        }
 #pragma unroll
         for (int j = 0; j < M; j++) {
             if (j < m) {
@@ -229,7 +266,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ------------------------------------------------------------------ MC
 // grid: x = direction block, y = active group.  mstride >= every group's m.
 // partial[(gi * nblk + bx) * mstride + j] = sum over the block's directions of |A_j|^2
-template <typename T, typename T4, bool kSingle>
+template <typename T, typename T4, bool kSingle, int M = kMaxM>
 __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T4 *const *__restrict__ grids,
                                                  const RotDev *__restrict__ rots, int nrot,
                                                  const double4 *__restrict__ trans,
@@ -285,14 +322,16 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 1 : 2) void mc_kernel(const T
         }
         return;
     }
-    for (int jb = 0; jb < G.m; jb += kMaxM) {
-        const int m = min(kMaxM, G.m - jb);
-        cplx<T> acc[kMaxM];
+    // docked: M q per pass (8 when no group holds more: fewer accumulators)
+    for (int jb = 0; jb < G.m; jb += M) {
+        const int m = min(M, G.m - jb);
+        cplx<T> acc[M];
 #pragma unroll
-        for (int j = 0; j < kMaxM; j++) acc[j] = {(T)0, (T)0};
-        docked_amplitude<T, T4>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, acc);
+        for (int j = 0; j < M; j++) acc[j] = {(T)0, (T)0};
+        docked_amplitude<T, T4, M>(grids, rots, nrot, trans, G.layer, step, qv, qs + G.q0 + jb, m, G.uniform != 0,
+                                   acc);
 #pragma unroll
-        for (int j = 0; j < kMaxM; j++) {
+        for (int j = 0; j < M; j++) {
             double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
             I = wave_sum(I);
             if (lane == 0) red[w][j] = I;
@@ -904,7 +943,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 2 : 1) void vegas_sample(cons
 #pragma unroll
                 for (int j = 0; j < kVM; j++) acc[j] = {(T)0, (T)0};
                 docked_amplitude<T, T4, kVM>(grids, rots, nrot, trans, G.layer, step, qv[k], qs + G.q0 + jb, m,
-                                             acc);
+                                             G.uniform != 0, acc);
 #pragma unroll
                 for (int j = 0; j < kVM; j++) {
                     const double I = (double)acc[j].x * (double)acc[j].x + (double)acc[j].y * (double)acc[j].y;
@@ -1351,17 +1390,19 @@ void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *r
         mc_reduce<<<nactive, 256, 0, s>>>(groups, active, partial, nblk, mstride, out);
         return;
     }
-#define DPLUS_MC(T, T4, S)                                                                                      \
-    mc_kernel<T, T4, S><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups, active, mstride, \
-                                             qs, step, seed, dir0, nblk, chunk, nu, nv, partial)
+#define DPLUS_MC(T, T4, S, M)                                                                                      \
+    mc_kernel<T, T4, S, M><<<grid, 256, 0, s>>>((const T4 *const *)grids, rots, nrot, trans, groups, active, mstride, \
+                                                qs, step, seed, dir0, nblk, chunk, nu, nv, partial)
     if (fp64 && single)
-        DPLUS_MC(double, double4, true);
+        DPLUS_MC(double, double4, true, kMaxM);
     else if (fp64)
-        DPLUS_MC(double, double4, false);
+        DPLUS_MC(double, double4, false, kMaxM);
     else if (single)
-        DPLUS_MC(float, float4, true);
+        DPLUS_MC(float, float4, true, kMaxM);
+    else if (mstride <= 8)
+        DPLUS_MC(float, float4, false, 8);
     else
-        DPLUS_MC(float, float4, false);
+        DPLUS_MC(float, float4, false, kMaxM);
 #undef DPLUS_MC
     mc_reduce<<<nactive, 256, 0, s>>>(groups, active, partial, nblk, mstride, out);
 }
