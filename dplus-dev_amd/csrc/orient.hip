@@ -422,7 +422,8 @@ constexpr int kZWin = 64;  // nodes of one plane's collapsed row segment (one pe
 // short segment of each theta row.  kCollapse (FP32 grids): the lookup is
 // linear in the node data and theta is the same for every copy, so per plane
 // the 4-row theta spline is applied once to the segment (at most kZWin nodes,
-// one per lane, its 16 row loads issued together and coalesced) into LDS, and
+// one per lane; the row loads of two planes, 8 per lane, issued together and
+// coalesced) into LDS, and
 // each copy takes the phi Hermite of two collapsed nodes; a segment that does
 // not fit, or a chunk spanning two leaves, takes the direct lookup, which
 // every copy takes without kCollapse.  The q spline is applied with per-q
