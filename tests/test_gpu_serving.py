@@ -38,7 +38,8 @@ def requests(D):
         else:
             kw.update(method=D.OA_MC, max_dirs=8192, chunk=4096)
         if i % 3 == 1:
-            kw["placements"] = S.rows_to_docking(S.helix_rows(6, 2.0, 0.7, 31.0))
+            # request 4 (mixed MC): 40 copies turned about z, the copy-major kernel
+            kw["placements"] = S.rows_to_docking(S.helix_rows(40 if i == 4 else 6, 2.0, 0.7, 31.0))
         if i % 3 == 2:
             kw.update(data=np.ones(len(q)), sigma=np.full(len(q), 0.5), fit_scale=1e-6, fit_constant=0.1)
         out.append((q, kw))
