@@ -439,7 +439,8 @@ def main():
     ap.add_argument("--pmc-probe", "--probe", dest="pmc_probe", action="store_true",
                     help="one mixed build of the 50k-atom grid, a 256 MiB device copy (counter calibration), one "
                          "VEGAS curve, one MC average of 262,144 directions and the gather probe on that grid, then "
-                         "the 1,000-copy helix docked MC at 32,768 directions and its gather probe: for rocprofv3")
+                         "the 1,000-copy helix docked MC at 32,768 directions and its gather probe, and a "
+                         "1,800-copy icosahedral shell's docked MC at 32,768 directions: for rocprofv3")
     ap.add_argument("--no-docking", action="store_true", help="skip the 1,000-copy helix docking curve")
     ap.add_argument("--gk-only", action="store_true",
                     help="the 50k-atom grid built once and averaged by Gauss-Kronrod three times: for rocprofv3")
@@ -505,16 +506,27 @@ def main():
         D.orientation_average(D.Docking([(g3, rot, nt, tr)]), q3, seed=5, max_dirs=32768, chunk=8192)
         D.timing_enable(False)
         ng3 = len(np.unique(np.maximum(np.ceil(q3 / g3.step_size) - 1, 0)))
+        docked_ms = D.timing_get("mc")[1]
         d_ms, d_lk = D.gather_probe(g3, 32768, 0, ng3, rot)
+        # an icosahedral shell of 1,800 copies (60 rotations x 30 translations: the general
+        # kernel with the translations' phases stepped in q) on the same grid and q
+        rot_i, nt_i, tr_i = S.rows_to_docking(S.icosahedral_rows(15.0, 1800, seed=3))
+        D.timing_reset()
+        D.timing_enable(True)
+        D.orientation_average(D.Docking([(g3, rot_i, nt_i, tr_i)]), q3, seed=5, max_dirs=32768, chunk=8192)
+        D.timing_enable(False)
+        ico_ms = D.timing_get("mc")[1]
         print(json.dumps({"probe": "build_atoms_pairs + copy_probe_kernel + vegas_sample + mc_kernel + gather_probe "
-                                   "(50k-atom grid 250), mc_zcopy_kernel + gather_probe<true> (1,000-copy helix)",
+                                   "(50k-atom grid 250), mc_zcopy_kernel + gather_probe<true> (1,000-copy helix), "
+                                   "mc_kernel docked (1,800-copy icosahedral shell)",
                           "copy_bytes": nbytes, "copy_GBps": 2 * nbytes / (ms * 1e-3) / 1e9,
                           "shell_groups": ngroups,
                           "vegas_ms": vegas_ms, "vegas_units": W["iters"] * 1024 * W["ppb"] * ngroups,
                           "mc_ms": mc_ms, "mc_units": (1 << 18) * ngroups,
                           "gather_probe_ms": g_ms, "gather_probe_units": g_lk,
-                          "docked_mc_ms": D.timing_get("mc")[1], "docked_mc_units": 32768 * ng3 * len(rot),
+                          "docked_mc_ms": docked_ms, "docked_mc_units": 32768 * ng3 * len(rot),
                           "docked_gather_probe_ms": d_ms, "docked_gather_probe_units": d_lk,
+                          "icosahedral_mc_ms": ico_ms, "icosahedral_mc_units": 32768 * ng3 * 1800,
                           "node_bytes_per_unit": 512}))
         return
     g64 = D.Grid(W["gs"], W["qmax"], D.PRECISION_FP64)
