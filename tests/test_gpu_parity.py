@@ -342,23 +342,30 @@ def test_zcopy_docking_mc_matches_oracle(dplus, oracle, small, prec, case, monke
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-def test_grouped_translations_mc_matches_oracle(dplus, oracle, small, prec):
+@pytest.mark.parametrize("method", ["mc", "vegas"])
+def test_grouped_translations_mc_matches_oracle(dplus, oracle, small, prec, method):
     """An icosahedral shell of 180 copies (60 rotations x 3 translations): with
-    equally spaced q the mixed kernel steps each translation's phase from q to q
-    (two range-reduced sin/cos per translation), with unevenly spaced q it takes
-    one sin/cos per q; both against the oracle's docked MC."""
-    say(f"[case] grouped translations MC precision={prec}")
+    equally spaced q the mixed kernels (docked MC and VEGAS) step each
+    translation's phase from q to q (two range-reduced sin/cos per translation),
+    with unevenly spaced q they take one sin/cos per q; both against the
+    oracle's docked MC / VEGAS."""
+    say(f"[case] grouped translations {method} precision={prec}")
     import dplus_mi355x.synth as S
     s = small
     g = gpu_grid(dplus, s["gs"], s["qmax"], prec, s["xyz"], s["ion"], s["coef9"], s["rad"])
     rot, nt, tr = S.rows_to_docking(S.icosahedral_rows(4.0, 180, seed=1))
     assert len(rot) == 60 and (nt == 3).all()
     rng = np.random.default_rng(4)
+    dock = oracle.Docked([(s["gs"], s["qmax"], s["data"], s["co"], rot, nt, tr)])
     for q in (np.linspace(0.05, 4.0, 60), np.sort(rng.uniform(0.05, 4.0, 60))):
-        mine, _ = dplus.orientation_average(dplus.Docking([(g, rot, nt, tr)]), q, seed=3, max_dirs=2048,
-                                            chunk=1024)
-        ref, _ = oracle.mc_oa(oracle.Docked([(s["gs"], s["qmax"], s["data"], s["co"], rot, nt, tr)]), q, 3,
-                              2048, 1024)
+        if method == "mc":
+            mine, _ = dplus.orientation_average(dplus.Docking([(g, rot, nt, tr)]), q, seed=3, max_dirs=2048,
+                                                chunk=1024)
+            ref, _ = oracle.mc_oa(dock, q, 3, 2048, 1024)
+        else:
+            mine, _ = dplus.orientation_average(dplus.Docking([(g, rot, nt, tr)]), q, method=dplus.OA_VEGAS,
+                                                seed=3, vegas_iterations=3, vegas_points_per_bin=2)
+            ref, _ = oracle.vegas_oa(dock, q, 3, 3, 2)
         e = np.max(np.abs(mine - ref) / ref)
         say(f"    {'uniform' if np.allclose(np.diff(q), np.diff(q)[0]) else 'uneven'} q: vs oracle {e:.2e}")
         assert e < (FP64_TOL if prec == 0 else MIXED_TOL)
