@@ -1396,6 +1396,8 @@ void launch_mc(bool fp64, bool single, const void *const *grids, const RotDev *r
                                                 qs, step, seed, dir0, nblk, chunk, nu, nv, partial)
     if (fp64 && single)
         DPLUS_MC(double, double4, true, kMaxM);
+    else if (fp64 && mstride <= 8)
+        DPLUS_MC(double, double4, false, 8);
     else if (fp64)
         DPLUS_MC(double, double4, false, kMaxM);
     else if (single)
